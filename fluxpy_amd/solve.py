@@ -1,0 +1,272 @@
+"""Radiosity solvers on the GPU (drop-in for flux.solve, solve.py:4-109).
+
+``solve_radiosity(FF, E, rho, albedo_placement, method, tol)`` keeps the
+reference's signature, defaults and return value ``(B, niter)``:
+
+* ``jacobi`` -- B <- E + F (rho B) ('right') or E + rho (F B) ('left'),
+  starting at B = E, stopping when max |B1 - B| <= tol with the default
+  tol = finfo(E.dtype).resolution * max|E| (solve.py:6-8, 25-45).  Like the
+  reference it returns the iterate *before* the last update.
+* ``cg``     -- the reference's CG on (I - rho F) (solve.py:47-106),
+  column by column for block right-hand sides.
+* ``gmres``  -- restarted GMRES(m) with modified Gram-Schmidt in the
+  working precision; ``tol`` is the relative residual |r| / |E| per column
+  (default 1e-12).  Not in the reference; added for the 1e-12 solves.
+
+Per-facet albedo is ``diag(rho)``: 'right' is F diag(rho), 'left' is
+diag(rho) F.  E may be a NumPy array (result returned as NumPy) or a device
+tensor (result stays on the device).  Unlike the reference, iterations are
+capped by ``maxiter`` (a warning is emitted when it is reached): the
+reference loops forever when rho * rowsum(F) >= 1.
+"""
+import warnings
+
+import numpy as np
+import scipy.sparse
+
+from . import _kernels
+from . import _lib
+
+
+class _CsrOperator:
+    """Device CSR with the fused Jacobi step."""
+
+    def __init__(self, csr):
+        self.csr = csr
+        self.shape = csr.shape
+        self.fused_jacobi = True
+
+    def apply(self, X, out=None):
+        return self.csr.matmat(X, out=out)
+
+    def jacobi_step(self, E, B, rho, left, B1, colmax):
+        t = _lib.torch()
+        n = B.shape[0]
+        k = B.shape[1]
+        rho_t, rho_s = (rho, 0.0) if isinstance(rho, t.Tensor) else (None, float(rho))
+        c = self.csr
+        _lib.call("flux_csr_jacobi_step", n, c.indptr.data_ptr(), c.indices.data_ptr(),
+                  c.data.data_ptr(), _lib.dtype_code(c.data.dtype), E.data_ptr(), B.data_ptr(),
+                  k, _lib.ptr(rho_t), rho_s, int(left), B1.data_ptr(),
+                  _lib.dtype_code(B.dtype), colmax.data_ptr(), _lib.stream())
+
+
+def device_operator(FF):
+    """The device operator behind FF (ours, a SciPy sparse matrix or a dense
+    array).  Anything else is rejected: there is no host execution path."""
+    if hasattr(FF, "device_operator"):
+        return FF.device_operator()
+    if hasattr(FF, "device_csr"):
+        return _CsrOperator(FF.device_csr)
+    if isinstance(FF, _kernels.DeviceCSR):
+        return _CsrOperator(FF)
+    if scipy.sparse.issparse(FF):
+        return _CsrOperator(_kernels.DeviceCSR.from_scipy(FF))
+    if isinstance(FF, np.ndarray) and FF.ndim == 2:
+        return _CsrOperator(_kernels.DeviceCSR.from_scipy(scipy.sparse.csr_matrix(FF)))
+    raise TypeError(
+        f"solve_radiosity: unsupported operator {type(FF).__name__}; pass a fluxpy_amd "
+        "form-factor matrix, a SciPy sparse matrix or a dense array")
+
+
+def _prep_rho(rho, n, tdt):
+    """Scalar rho stays a float; per-facet rho becomes a device vector."""
+    t = _lib.torch()
+    if isinstance(rho, t.Tensor):
+        r = rho.to(device=_lib.device(), dtype=tdt).reshape(-1)
+        return r.contiguous() if r.numel() > 1 else float(r.item())
+    if np.ndim(rho) == 0:
+        return float(rho)
+    r = np.asarray(rho).reshape(-1)
+    if r.size == 1:
+        return float(r[0])
+    if r.size != n:
+        raise ValueError(f"rho must be a scalar or have {n} entries (got {r.size})")
+    return _lib.to_dev(r).to(tdt).contiguous()
+
+
+def _apply_K(op, X, rho, left, out=None):
+    """out = (I - F diag(rho)) X  (right)  or  (I - diag(rho) F) X  (left)."""
+    if left:
+        Y = op.apply(X)
+        _kernels.rowscale(rho, Y, out=Y)
+    else:
+        Y = op.apply(_kernels.rowscale(rho, X))
+    # out = X - Y
+    out = X.clone() if out is None else out.copy_(X)
+    t = _lib.torch()
+    k = X.shape[1]
+    minus = t.full((k,), -1.0, dtype=t.float64, device=X.device)
+    _kernels.axpby(minus, Y, None, out)
+    return out
+
+
+def solve_radiosity(FF, E, rho=1, albedo_placement='right', method='jacobi', tol=None,
+                    maxiter=None, restart=30, x0=None):
+    if albedo_placement not in {'left', 'right'}:
+        raise Exception('albedo_placement must be "left" or "right"')
+    if method not in {'jacobi', 'cg', 'gmres'}:
+        raise Exception('method must be one of: jacobi, cg, gmres')
+    t = _lib.require_gpu()
+    host = not isinstance(E, t.Tensor)
+    if host:
+        E = np.asarray(E)
+        if E.dtype not in (np.float32, np.float64):
+            E = E.astype(np.float64)
+    Ed = _lib.to_dev(E)
+    vec = Ed.dim() == 1
+    E2 = Ed.reshape(Ed.shape[0], -1).contiguous()
+    op = device_operator(FF)
+    if op.shape[0] != E2.shape[0] or op.shape[1] != E2.shape[0]:
+        raise ValueError(f"shape mismatch: F {op.shape}, E {tuple(Ed.shape)}")
+    rho_d = _prep_rho(rho, E2.shape[0], E2.dtype)
+    left = albedo_placement == 'left'
+    if method == 'jacobi':
+        if tol is None:
+            emax = float(abs(E).max()) if host else float(Ed.abs().max())
+            tol = np.finfo(np.dtype(str(E2.dtype).replace("torch.", ""))).resolution * emax
+        B, niter = _jacobi(op, E2, rho_d, left, tol, maxiter or 100000)
+    elif method == 'cg':
+        if tol is None:
+            emax = float(abs(E).max()) if host else float(Ed.abs().max())
+            tol = np.finfo(np.dtype(str(E2.dtype).replace("torch.", ""))).resolution * emax
+        B, niter = _cg(op, E2, rho_d, left, tol, maxiter or 100000)
+    else:
+        B, niter = _gmres(op, E2, rho_d, left, 1e-12 if tol is None else tol,
+                          maxiter or 10000, restart, x0)
+    B = B.reshape(-1) if vec else B
+    return (B.cpu().numpy() if host else B), niter
+
+
+def _jacobi(op, E, rho, left, tol, maxiter):
+    """solve.py:25-45, fused into one kernel per iteration for CSR operators."""
+    t = _lib.torch()
+    k = E.shape[1]
+    tol_c = np.broadcast_to(np.asarray(tol, dtype=np.float64), (k,))
+    B = E.clone()
+    B1 = t.empty_like(E)
+    colmax = t.empty(k, dtype=t.float64, device=E.device)
+    niter = 0
+    while True:
+        niter += 1
+        op.jacobi_step(E, B, rho, left, B1, colmax)
+        d = colmax.cpu().numpy()
+        if np.all(d <= tol_c):
+            break
+        if niter >= maxiter:
+            warnings.warn(f"Jacobi did not converge in {maxiter} iterations "
+                          f"(max |B1 - B| = {d.max():.3e} > tol)", RuntimeWarning)
+            break
+        B, B1 = B1, B
+    return B, niter
+
+
+def _cg(op, E, rho, left, tol, maxiter):
+    """solve.py:47-106 (CG applied to I - rho F), column by column."""
+    t = _lib.torch()
+    k = E.shape[1]
+    vo = _kernels.VecOps(k)
+    X = t.zeros_like(E)
+    R = E.clone()
+    if float(vo.maxabs(R).max()) <= tol:
+        return X, 0
+    P = R.clone()
+    niter = 0
+    while True:
+        FP = op.apply(P)
+        # KK_P = P - rho*(FF@P)  (solve.py:75)
+        KP = P.clone()
+        _kernels.rowscale(rho, FP, out=FP)
+        minus = t.full((k,), -1.0, dtype=t.float64, device=E.device)
+        _kernels.axpby(minus, FP, None, KP)
+        RR = vo.dot(R, R).clone()
+        alpha = RR / vo.dot(KP, P)
+        _kernels.axpby(alpha, P, None, X)
+        _kernels.axpby(-alpha, KP, None, R)
+        if float(vo.maxabs(R).max()) <= tol:
+            return X, niter + 1
+        beta = vo.dot(R, R) / RR
+        _kernels.axpby(None, R, beta, P)
+        niter += 1
+        if niter >= maxiter:
+            warnings.warn(f"CG did not converge in {maxiter} iterations", RuntimeWarning)
+            return X, niter
+
+
+def _gmres(op, E, rho, left, rtol, maxiter, restart, x0):
+    """Restarted GMRES(m), modified Gram-Schmidt, one Krylov space per column;
+    the small Hessenberg least-squares problems are solved with Givens
+    rotations on the host."""
+    t = _lib.torch()
+    n, k = E.shape
+    dev = E.device
+    vo = _kernels.VecOps(k)
+    X = t.zeros_like(E) if x0 is None else _lib.to_dev(x0).reshape(n, k).to(E.dtype).clone()
+    bnorm = np.sqrt(vo.dot(E, E).cpu().numpy())
+    bnorm[bnorm == 0] = 1.0
+    niter = 0
+    m = max(1, int(restart))
+    while True:
+        R = E.clone()
+        if x0 is not None or niter > 0:
+            KX = _apply_K(op, X, rho, left)
+            minus = t.full((k,), -1.0, dtype=t.float64, device=dev)
+            _kernels.axpby(minus, KX, None, R)
+        beta = np.sqrt(np.maximum(vo.dot(R, R).cpu().numpy(), 0.0))
+        if np.all(beta <= rtol * bnorm) or niter >= maxiter:
+            if niter >= maxiter and not np.all(beta <= rtol * bnorm):
+                warnings.warn(f"GMRES did not converge in {maxiter} iterations", RuntimeWarning)
+            return X, niter
+        inv = np.where(beta > 0, 1.0 / np.where(beta > 0, beta, 1.0), 0.0)
+        Vs = [_scale_cols(R, inv)]
+        H = np.zeros((m + 1, m, k))
+        cs = np.zeros((m, k))
+        sn = np.zeros((m, k))
+        g = np.zeros((m + 1, k))
+        g[0] = beta
+        done = beta <= rtol * bnorm
+        j_used = 0
+        for j in range(m):
+            w = _apply_K(op, Vs[j], rho, left)
+            for i in range(j + 1):
+                h = vo.dot(w, Vs[i]).cpu().numpy().copy()
+                H[i, j] = h
+                _kernels.axpby(t.tensor(-h, dtype=t.float64, device=dev), Vs[i], None, w)
+            hn = np.sqrt(np.maximum(vo.dot(w, w).cpu().numpy(), 0.0))
+            H[j + 1, j] = hn
+            Vs.append(_scale_cols(w, np.where(hn > 0, 1.0 / np.where(hn > 0, hn, 1.0), 0.0)))
+            # Givens rotations per column
+            for i in range(j):
+                a, b = H[i, j].copy(), H[i + 1, j].copy()
+                H[i, j] = cs[i] * a + sn[i] * b
+                H[i + 1, j] = -sn[i] * a + cs[i] * b
+            a, b = H[j, j].copy(), H[j + 1, j].copy()
+            r = np.hypot(a, b)
+            safe = r > 0
+            cs[j] = np.where(safe, a / np.where(safe, r, 1.0), 1.0)
+            sn[j] = np.where(safe, b / np.where(safe, r, 1.0), 0.0)
+            H[j, j] = np.where(safe, r, a)
+            H[j + 1, j] = 0.0
+            g[j + 1] = -sn[j] * g[j]
+            g[j] = cs[j] * g[j]
+            niter += 1
+            j_used = j + 1
+            done = done | (np.abs(g[j + 1]) <= rtol * bnorm)
+            if np.all(done) or niter >= maxiter:
+                break
+        # back substitution per column and update X
+        y = np.zeros((j_used, k))
+        for i in range(j_used - 1, -1, -1):
+            s = g[i] - (H[i, i + 1:j_used] * y[i + 1:j_used]).sum(0)
+            d = H[i, i]
+            y[i] = np.where(d != 0, s / np.where(d != 0, d, 1.0), 0.0)
+        for i in range(j_used):
+            _kernels.axpby(t.tensor(y[i], dtype=t.float64, device=dev), Vs[i], None, X)
+
+
+def _scale_cols(X, s):
+    """Return X[:, c] * s[c] (s: host FP64 per column) as a new tensor."""
+    t = _lib.torch()
+    out = t.zeros_like(X)
+    _kernels.axpby(t.tensor(np.asarray(s, dtype=np.float64), device=X.device), X, None, out)
+    return out

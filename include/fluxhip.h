@@ -1,0 +1,237 @@
+/*
+ * fluxhip.h -- C ABI of libfluxhip.so, the MI355X (gfx950) implementation of
+ * fluxpy's visibility / form-factor / compressed-solve path.
+ *
+ * Conventions (every entry point):
+ *   - returns int status: 0 = ok, < 0 = error; flux_last_error() gives a
+ *     thread-local message for the last failing call on this thread;
+ *   - array arguments are caller-owned DEVICE pointers unless the comment
+ *     says "host"; the library never keeps them after the call returns,
+ *     except the copies an opaque handle makes of its own inputs;
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream);
+ *     every launch goes on that stream and the call does not synchronise
+ *     unless the comment says so;
+ *   - dtype codes: FLUX_F32 = 0, FLUX_F64 = 1, FLUX_BF16 = 2, FLUX_F16 = 3;
+ *   - ray modes: FLUX_MODE_CGAL = 0, FLUX_MODE_EMBREE = 1 (see below);
+ *   - handles are immutable after creation, so queries on one handle from
+ *     several streams may run concurrently.
+ *
+ * Reference citations are relative to the fluxpy source tree (src/flux/).
+ */
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { FLUX_F32 = 0, FLUX_F64 = 1, FLUX_BF16 = 2, FLUX_F16 = 3 };
+
+/* CGAL semantics (cgal/aabb_wrapper.cpp:64-108): ray origin = centroid of
+ * facet i, the source facet is skipped, visible iff the closest hit is j,
+ * no hit => not visible.
+ * EMBREE semantics (shape.py:349-421): origin offset by 1e-3 along the unit
+ * ray (visibility) or the normal (occlusion), nothing skipped, pairs closer
+ * than 1e-3 are visible by default.  Both are evaluated with the same
+ * FP64 watertight ray/triangle test. */
+enum { FLUX_MODE_CGAL = 0, FLUX_MODE_EMBREE = 1 };
+
+typedef struct flux_scene flux_scene;
+typedef struct flux_hmat flux_hmat;
+
+/* Thread-local description of the last error. Never NULL. */
+const char *flux_last_error(void);
+
+/* Library version string (host). */
+const char *flux_version(void);
+
+/* ---------------------------------------------------------------------- */
+/* Geometry                                                                */
+/* ---------------------------------------------------------------------- */
+
+/* Centroids, unit normals and areas of a triangle mesh in the mesh dtype.
+ * Replaces get_centroids / get_surface_normals_and_face_areas
+ * (shape.py:16-45).  V: nv*3 (dtype), F: nf*3 int64, P,N: nf*3, A: nf.
+ * Any of P, N, A may be NULL.  Same operation order as the NumPy code:
+ * P = ((v0+v1)+v2)/3, C = (v1-v0) x (v2-v0), |C| = sqrt((c0^2+c1^2)+c2^2),
+ * N = C/|C|, A = |C|/2. */
+int flux_geometry(const void *V, int64_t nv, const int64_t *F, int64_t nf,
+                  int dtype, void *P, void *N, void *A, void *stream);
+
+/* ---------------------------------------------------------------------- */
+/* Scene: linear BVH over the mesh                                         */
+/* ---------------------------------------------------------------------- */
+
+/* Build a scene (Morton codes, on-device sort, Karras hierarchy, bottom-up
+ * refit, preorder layout with escape links).  Replaces
+ * cgal_aabb_alloc + cgal_aabb_init_from_trimesh (cgal/aabb_wrapper.h:8-10,
+ * aabb_wrapper.cpp:36-57) and the Embree scene of shape.py:296-344.
+ * V: nv*3 in `dtype` (converted to FP64 as shape.py:266-267 does),
+ * F: nf*3 int64.  Synchronises `stream` before returning. */
+int flux_scene_create(const void *V, int64_t nv, const int64_t *F, int64_t nf,
+                      int dtype, void *stream, flux_scene **out);
+
+/* Replaces cgal_aabb_dealloc (aabb_wrapper.h:11). NULL is accepted. */
+int flux_scene_destroy(flux_scene *scene);
+
+/* Host queries. */
+int flux_scene_num_faces(const flux_scene *scene, int64_t *nf);
+int flux_scene_num_nodes(const flux_scene *scene, int64_t *nnodes);
+
+/* Device pointer to the scene's FP64 centroids (nf*3, ((v0+v1)+v2)/3). */
+int flux_scene_centroids(const flux_scene *scene, const double **cen);
+
+/* Copy the preorder node array to `nodes_out` (device, 8 floats per node:
+ * lo.xyz, escape, hi.xyz, triangle-or--1; the last float of each half is an
+ * int32 bit pattern).  Used by the structural tests. */
+int flux_scene_copy_nodes(const flux_scene *scene, float *nodes_out,
+                          void *stream);
+
+/* m x n visibility mask (uint8, row-major): out[p*n+q] = vis(I[p], J[q]).
+ * Replaces cgal_aabb_test_face_to_face_vis + AABB.test_face_to_face_vis_MN
+ * (aabb_wrapper.cpp:64-93, cgal/aabb.pyx:58-70; i == j gives 0) and
+ * EmbreeTrimeshShapeModel._get_visibility (shape.py:349-398).
+ * P: model centroids (nf*3, `dtype`), used by EMBREE mode only (may be NULL
+ * in CGAL mode). */
+int flux_vis_mn(const flux_scene *scene, const int64_t *I, int64_t m,
+                const int64_t *J, int64_t n, int mode, const void *P,
+                int dtype, uint8_t *out, void *stream);
+
+/* m x M occlusion mask (uint8): out[p*M+q] = ray from facet I[p] along D[q]
+ * hits something.  Replaces cgal_aabb_ray_from_centroid_is_occluded and
+ * its 1-D/2-D Cython loops (aabb_wrapper.cpp:95-108, cgal/aabb.pyx:72-87)
+ * and EmbreeTrimeshShapeModel._is_occluded (shape.py:400-421).
+ * D: M*3 FP64.  If `per_face` != 0, M must equal m and row p uses D[p] only
+ * (out is m x 1).  P, Nrm: model centroids and unit normals (nf*3, `dtype`),
+ * read in EMBREE mode only (may be NULL in CGAL mode). */
+int flux_occluded(const flux_scene *scene, const int64_t *I, int64_t m,
+                  const double *D, int64_t M, int per_face, int mode,
+                  const void *P, const void *Nrm, int dtype, uint8_t *out,
+                  void *stream);
+
+/* nf x M direct insolation, one point sun per column:
+ *   E[f,k] = F0 * max(0, N_f . D[k]) if facet f is not occluded towards
+ *   D[k] (mode semantics as flux_occluded), else 0;
+ * the 1-D-sun branch of TrimeshShapeModel.get_direct_irradiance
+ * (shape.py:214-226) for M directions in one launch.  The dot product is
+ * FP64, the result is stored in `dtype`.  P, Nrm: nf*3 in `dtype`. */
+int flux_direct_irradiance(const flux_scene *scene, const void *P,
+                           const void *Nrm, int dtype, const double *D,
+                           int64_t M, double F0, int mode, void *E,
+                           void *stream);
+
+/* Closest hit of n_rays rays x + t d (t >= 0), nothing skipped.  Replaces
+ * cgal_aabb_intersect1 (aabb_wrapper.cpp:110-129).  hit_index[k] = -1 when
+ * ray k hits nothing; xt[k*3..] = x + t d.  x, d: n_rays*3 FP64. */
+int flux_intersect1(const flux_scene *scene, const double *x, const double *d,
+                    int64_t n_rays, int64_t *hit_index, double *xt,
+                    void *stream);
+
+/* ---------------------------------------------------------------------- */
+/* Form factors (form_factors.py:11-72)                                    */
+/* ---------------------------------------------------------------------- */
+
+/* Pass 1 of get_form_factor_matrix: for each row p and column q, bit q of
+ * row p of `mask` (m rows x ceil(n/64) uint64 words) is set iff
+ *   c = max(0, N_i.(P_j-P_i)) * max(0, P_i.N_j - N_j.P_j) > eps,  i != j,
+ * and the ray test of the scene's mode says i sees j (i = I[p], j = J[q]).
+ * row_nnz[p] = popcount of row p.  P, N: nf*3 in `dtype`. */
+int flux_ff_mask(const flux_scene *scene, const void *P, const void *N,
+                 int dtype, const int64_t *I, int64_t m, const int64_t *J,
+                 int64_t n, double eps, int mode, uint64_t *mask,
+                 int64_t *row_nnz, void *stream);
+
+/* Pass 2: CSR fill.  indptr: m+1 int64 (exclusive scan of row_nnz),
+ * indices: int32 column positions q, data: F_ij = c*A_j/(pi*|P_i-P_j|^4)
+ * in `dtype` (0 where P_i == P_j, as s == 0 -> inf in the reference). */
+int flux_ff_fill(const void *P, const void *N, const void *A, int dtype,
+                 const int64_t *I, int64_t m, const int64_t *J, int64_t n,
+                 const uint64_t *mask, const int64_t *indptr,
+                 int32_t *indices, void *data, void *stream);
+
+/* Exclusive scan of n int64 values: out[0] = 0, out[k] = sum(in[:k]),
+ * out has n+1 entries.  `total_host` (host, may be NULL) receives out[n]
+ * and then the call synchronises `stream`. */
+int flux_scan_i64(const int64_t *in, int64_t n, int64_t *out,
+                  int64_t *total_host, void *stream);
+
+/* ---------------------------------------------------------------------- */
+/* Sparse products and the Jacobi iteration (solve.py:4-45)                */
+/* ---------------------------------------------------------------------- */
+
+/* Y = A X for CSR A (m x n; int64 indptr, int32 indices, values in `dtype`
+ * F32 or F64), X: n x nrhs, Y: m x nrhs, both row-major in `xdtype`
+ * (F32 or F64; accumulation in the wider of the two). */
+int flux_csr_spmm(int64_t m, int64_t n, const int64_t *indptr,
+                  const int32_t *indices, const void *data, int dtype,
+                  const void *X, int64_t nrhs, void *Y, int xdtype,
+                  void *stream);
+
+/* Transpose of an m x n CSR (canonical: no duplicate columns in a row) into
+ * an n x m CSR whose rows are sorted by column index.  Deterministic.
+ * t_indptr: n+1, t_indices / t_data: nnz entries.  Used for F^T products
+ * (the adjoint solve; the reference's rmatvec, form_factors.py:141-173).
+ * Synchronises `stream`. */
+int flux_csr_transpose(int64_t m, int64_t n, const int64_t *indptr,
+                       const int32_t *indices, const void *data, int dtype,
+                       int64_t *t_indptr, int32_t *t_indices, void *t_data,
+                       void *stream);
+
+/* One Jacobi radiosity iteration on a square CSR F, fused with its update
+ * and convergence measure (solve.py:25-45):
+ *   right (left == 0): B1 = E + F (rho o B)      (solve.py:41)
+ *   left  (left != 0): B1 = E + rho o (F B)      (solve.py:30)
+ * rho: per-facet albedo (m values in `xdtype`) or NULL to use rho_scalar.
+ * colmax[c] (nrhs doubles, device) = max_i |B1[i,c] - B[i,c]|.
+ * E, B, B1: m x nrhs row-major in `xdtype`; B1 must not alias B. */
+int flux_csr_jacobi_step(int64_t m, const int64_t *indptr,
+                         const int32_t *indices, const void *data, int dtype,
+                         const void *E, const void *B, int64_t nrhs,
+                         const void *rho, double rho_scalar, int left,
+                         void *B1, int xdtype, double *colmax, void *stream);
+
+/* ---------------------------------------------------------------------- */
+/* Column-blocked vector kernels (solver building blocks; blocks are      */
+/* n x nrhs row-major, reductions per column in FP64, deterministic)       */
+/* ---------------------------------------------------------------------- */
+
+/* mode 0: out[c] = sum_i x[i,c] y[i,c]; 1: max_i |x[i,c] - y[i,c]|;
+ * 2: max_i |x[i,c]| (y unused).  work: >= 64*nrhs doubles (device). */
+int flux_vec_reduce(int64_t n, int64_t nrhs, const void *x, const void *y,
+                    int dtype, int mode, double *out, double *work,
+                    void *stream);
+
+/* y[:,c] = alpha[c] x[:,c] + beta[c] y[:,c]; alpha, beta: device FP64
+ * arrays of nrhs, NULL meaning all ones. */
+int flux_vec_axpby(int64_t n, int64_t nrhs, const double *alpha,
+                   const void *x, const double *beta, void *y, int dtype,
+                   void *stream);
+
+/* out[i,c] = s_i x[i,c] + add_s, s_i = rho[i] (or rho_s if rho is NULL),
+ * replaced by 1 - s_i when one_minus != 0 ((1-rho)*B + Fsurf of
+ * model.py:16).  out may alias x. */
+int flux_vec_rowscale(int64_t n, int64_t nrhs, const void *rho, double rho_s,
+                      int one_minus, double add_s, const void *x, void *out,
+                      int dtype, void *stream);
+
+/* out = max(0, x) (model.py:15, 20, 23). out may alias x. */
+int flux_vec_clamp0(int64_t total, const void *x, void *out, int dtype,
+                    void *stream);
+
+/* Element-wise dtype conversion between FP32 and FP64 blocks. */
+int flux_vec_convert(int64_t total, const void *x, int xdtype, void *y,
+                     int ydtype, void *stream);
+
+/* Equilibrium temperature (model.py:19-24, 264-270):
+ *   tot = (1 - rho_i) B + emiss_i Q + fsurf, clamped at 0 if clamp,
+ *   T = (tot / (emiss_i sigma))^(1/4) in FP64, stored in `dtype`.
+ * rho / emiss: per-row arrays or NULL for the scalars. */
+int flux_temperature(int64_t n, int64_t nrhs, const void *B, const void *Q,
+                     const void *rho, double rho_s, const void *emiss,
+                     double emiss_s, double fsurf, int clamp, double sigma,
+                     void *T, int dtype, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
