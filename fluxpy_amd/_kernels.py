@@ -143,6 +143,10 @@ def ff_csr(shape_model, I=None, J=None, eps=None, dtype=None):
                   indices.data_ptr(), data.data_ptr(), st)
         parts.append((indptr, indices[:nnz], data[:nnz]))
         del mask
+    if not parts:  # m == 0
+        return DeviceCSR(t.zeros(1, dtype=t.int64, device=dev),
+                         t.empty(0, dtype=t.int32, device=dev),
+                         t.empty(0, dtype=tdt, device=dev), (m, n))
     if len(parts) == 1:
         indptr, indices, data = parts[0]
     else:

@@ -86,8 +86,11 @@ def test_scene_single_facet_and_empty_queries(torch_dev):
 def test_bad_face_index_raises(torch_dev):
     from fluxpy_amd import shape
     V = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0.0]])
-    with pytest.raises(ValueError):
+    # V[F] raises IndexError in the geometry helpers, as in the reference
+    with pytest.raises((IndexError, ValueError)):
         shape.CgalTrimeshShapeModel(V, np.array([[0, 1, 3]]))
+    with pytest.raises(ValueError):
+        shape._Scene(V, np.array([[0, 1, 3]]))
 
 
 # --------------------------------------------------------------------------
