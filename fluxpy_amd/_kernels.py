@@ -56,7 +56,14 @@ class DeviceCSR:
              self.indptr.cpu().numpy()), shape=self.shape)
 
     def astype(self, torch_dtype):
-        return DeviceCSR(self.indptr, self.indices, self.data.to(torch_dtype), self.shape)
+        """Same pattern, values converted on the GPU (round to nearest even)."""
+        t = _lib.torch()
+        src = self.data
+        if src.dtype == torch_dtype:
+            return DeviceCSR(self.indptr, self.indices, src.clone(), self.shape)
+        if src.dtype == t.float64 and torch_dtype in (t.bfloat16, t.float16):
+            src = convert(src, t.float32)
+        return DeviceCSR(self.indptr, self.indices, convert(src, torch_dtype), self.shape)
 
     def matmat(self, X, out=None):
         """Y = A X for a device block X (n or n x k), in X's dtype."""

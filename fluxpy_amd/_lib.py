@@ -39,7 +39,9 @@ SIGNATURES = {
     "flux_intersect1": (_c_int, [_vp, _vp, _vp, _c_i64, _vp, _vp, _vp]),
     "flux_ff_mask": (_c_int, [_vp, _vp, _vp, _c_int, _vp, _c_i64, _vp, _c_i64, _c_dbl,
                               _c_int, _vp, _vp, _vp]),
-    "flux_ff_fill": (_c_int, [_vp, _vp, _vp, _c_int, _vp, _c_i64, _vp, _c_i64, _vp, _vp,
+    "flux_ff_candidates": (_c_int, [_vp, _vp, _vp, _c_int, _vp, _c_i64, _vp, _c_i64, _c_dbl,
+                                    _vp, _vp, _vp]),
+    "flux_ff_fill":(_c_int, [_vp, _vp, _vp, _c_int, _vp, _c_i64, _vp, _c_i64, _vp, _vp,
                               _vp, _vp, _vp]),
     "flux_scan_i64": (_c_int, [_vp, _c_i64, _vp, ctypes.POINTER(_c_i64), _vp]),
     "flux_csr_spmm": (_c_int, [_c_i64, _c_i64, _vp, _vp, _vp, _c_int, _vp, _c_i64, _vp,

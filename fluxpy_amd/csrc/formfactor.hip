@@ -44,8 +44,9 @@ __device__ __forceinline__ T ff_value(T c, const T *Pi, const T *Pj, T Aj) {
   return (c * Aj) / s;
 }
 
-// One wave per (row p, 64-column word w).
-template <typename T>
+// One wave per (row p, 64-column word w).  With CAST_RAYS == false the bit
+// is the prefilter alone (the set of pairs that get a ray).
+template <typename T, bool CAST_RAYS = true>
 __global__ void __launch_bounds__(256) k_ff_mask(flux_scene S, const T *__restrict__ P,
                                                  const T *__restrict__ N,
                                                  const int64_t *__restrict__ I, int64_t m,
@@ -69,9 +70,13 @@ __global__ void __launch_bounds__(256) k_ff_mask(flux_scene S, const T *__restri
     T Nj[3] = {N[(int64_t)j * 3], N[(int64_t)j * 3 + 1], N[(int64_t)j * 3 + 2]};
     T c = ff_prefilter(Pi, Ni, Pj, Nj);
     if (i != j && c > eps) {
-      double pi64[3] = {(double)Pi[0], (double)Pi[1], (double)Pi[2]};
-      double pj64[3] = {(double)Pj[0], (double)Pj[1], (double)Pj[2]};
-      bit = face_to_face_vis(S, i, j, mode, pi64, pj64);
+      if (CAST_RAYS) {
+        double pi64[3] = {(double)Pi[0], (double)Pi[1], (double)Pi[2]};
+        double pj64[3] = {(double)Pj[0], (double)Pj[1], (double)Pj[2]};
+        bit = face_to_face_vis(S, i, j, mode, pi64, pj64);
+      } else {
+        bit = true;
+      }
     }
   }
   const uint64_t b = __ballot(bit);
@@ -217,6 +222,35 @@ extern "C" int flux_ff_mask(const flux_scene *scene, const void *P, const void *
   }
   hipLaunchKernelGGL(k_row_count, dim3(ceil_div_u(m, 4)), dim3(256), 0, st, mask, m, W,
                      row_nnz);
+  FLUX_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int flux_ff_candidates(const flux_scene *scene, const void *P, const void *N,
+                                  int dtype, const int64_t *I, int64_t m, const int64_t *J,
+                                  int64_t n, double eps, uint64_t *mask, int64_t *row_cnt,
+                                  void *stream) {
+  FLUX_CHECK(scene != nullptr, "flux_ff_candidates: NULL scene");
+  FLUX_CHECK(m >= 0 && n >= 0, "flux_ff_candidates: negative sizes");
+  FLUX_CHECK(dtype == FLUX_F32 || dtype == FLUX_F64, "flux_ff_candidates: dtype");
+  if (m == 0) return 0;
+  FLUX_CHECK(P && N && I && mask && row_cnt && (n == 0 || J), "flux_ff_candidates: NULL array");
+  hipStream_t st = as_stream(stream);
+  const int64_t W = (n + 63) / 64;
+  if (W > 0) {
+    const unsigned blocks = ceil_div_u(m * W, 4);
+    if (dtype == FLUX_F64)
+      hipLaunchKernelGGL((k_ff_mask<double, false>), dim3(blocks), dim3(256), 0, st, *scene,
+                         (const double *)P, (const double *)N, I, m, J, n, (double)eps, 0,
+                         mask, W);
+    else
+      hipLaunchKernelGGL((k_ff_mask<float, false>), dim3(blocks), dim3(256), 0, st, *scene,
+                         (const float *)P, (const float *)N, I, m, J, n, (float)eps, 0, mask,
+                         W);
+    FLUX_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(k_row_count, dim3(ceil_div_u(m, 4)), dim3(256), 0, st, mask, m, W,
+                     row_cnt);
   FLUX_LAUNCH_CHECK();
   return 0;
 }

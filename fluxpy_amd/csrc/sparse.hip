@@ -5,6 +5,9 @@
 // nonzeros k = g, g+G, ... of the row for column c, then the groups are
 // reduced with cross-lane shuffles.  The summation order is fixed, so every
 // product is deterministic.
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+
 #include "common.h"
 
 namespace flux {
@@ -18,6 +21,26 @@ struct AccOf {
 template <>
 struct AccOf<double> {
   using type = double;
+};
+
+// Matrix values may be stored in BF16 / FP16 (the low-precision inner
+// operator of the refined solve); they widen to the accumulation type.
+__device__ __forceinline__ float widen(__hip_bfloat16 v) { return __bfloat162float(v); }
+__device__ __forceinline__ float widen(__half v) { return __half2float(v); }
+__device__ __forceinline__ float widen(float v) { return v; }
+__device__ __forceinline__ double widen(double v) { return v; }
+
+template <typename T>
+struct WideOf {
+  using type = T;
+};
+template <>
+struct WideOf<__hip_bfloat16> {
+  using type = float;
+};
+template <>
+struct WideOf<__half> {
+  using type = float;
 };
 
 __device__ inline void atomic_max_nonneg(double *addr, double v) {
@@ -46,7 +69,7 @@ __global__ void __launch_bounds__(256) k_csr_rows(
         const int64_t j = indices[k];
         TAcc x = (TAcc)X[j * nrhs + col];
         if (MODE == SPMM_JACOBI_RIGHT) x = (rho ? (TAcc)rho[j] : (TAcc)rho_s) * x;
-        acc += (TAcc)data[k] * x;
+        acc += (TAcc)widen(data[k]) * x;
       }
     }
     for (int off = R; off < 64; off <<= 1) acc += __shfl_xor(acc, off, 64);
@@ -70,7 +93,7 @@ template <typename TA, typename TX, int MODE>
 static int launch_rows(int64_t m, const int64_t *indptr, const int32_t *indices,
                        const void *data, const void *X, int64_t nrhs, void *Y, const void *E,
                        const void *rho, double rho_s, double *colmax, hipStream_t st) {
-  using TAcc = typename AccOf<decltype(TA() + TX())>::type;
+  using TAcc = typename AccOf<decltype(typename WideOf<TA>::type() + TX())>::type;
   int R = 1;
   while (R < nrhs && R < 64) R <<= 1;
   const unsigned blocks = ceil_div_u(m, 4);
@@ -94,6 +117,12 @@ static int dispatch_rows(int64_t m, const int64_t *indptr, const int32_t *indice
                                             rho_s, colmax, st);
   if (dtype == FLUX_F64 && xdtype == FLUX_F32)
     return launch_rows<double, float, MODE>(m, indptr, indices, data, X, nrhs, Y, E, rho,
+                                            rho_s, colmax, st);
+  if (dtype == FLUX_BF16 && xdtype == FLUX_F32)
+    return launch_rows<__hip_bfloat16, float, MODE>(m, indptr, indices, data, X, nrhs, Y, E,
+                                                    rho, rho_s, colmax, st);
+  if (dtype == FLUX_F16 && xdtype == FLUX_F32)
+    return launch_rows<__half, float, MODE>(m, indptr, indices, data, X, nrhs, Y, E, rho,
                                             rho_s, colmax, st);
   if (dtype == FLUX_F32 && xdtype == FLUX_F32)
     return launch_rows<float, float, MODE>(m, indptr, indices, data, X, nrhs, Y, E, rho,

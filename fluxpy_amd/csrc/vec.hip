@@ -2,6 +2,9 @@
 // GMRES, iterative refinement).  Vectors are n x nrhs row-major blocks;
 // every reduction is per column, in FP64, two-stage with a fixed order, so
 // results are deterministic.
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+
 #include "common.h"
 
 namespace flux {
@@ -104,11 +107,21 @@ __global__ void k_temperature(int64_t n, int64_t nrhs, const T *__restrict__ B,
   out[k] = (T)pow(tot / (e * sigma), 0.25);
 }
 
-// Dtype conversion between FP64 / FP32 blocks.
+// Dtype conversion between FP64 / FP32 blocks and FP32 <-> BF16 / FP16
+// (round to nearest even).
+__device__ __forceinline__ void cvt(float x, __hip_bfloat16 &y) { y = __float2bfloat16(x); }
+__device__ __forceinline__ void cvt(float x, __half &y) { y = __float2half(x); }
+__device__ __forceinline__ void cvt(__hip_bfloat16 x, float &y) { y = __bfloat162float(x); }
+__device__ __forceinline__ void cvt(__half x, float &y) { y = __half2float(x); }
+__device__ __forceinline__ void cvt(float x, float &y) { y = x; }
+__device__ __forceinline__ void cvt(float x, double &y) { y = (double)x; }
+__device__ __forceinline__ void cvt(double x, float &y) { y = (float)x; }
+__device__ __forceinline__ void cvt(double x, double &y) { y = x; }
+
 template <typename TI, typename TO>
 __global__ void k_convert(int64_t total, const TI *__restrict__ x, TO *__restrict__ y) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < total) y[k] = (TO)x[k];
+  if (k < total) cvt(x[k], y[k]);
 }
 
 }  // namespace flux
@@ -221,6 +234,18 @@ extern "C" int flux_vec_convert(int64_t total, const void *x, int xdtype, void *
   else if (xdtype == FLUX_F32 && ydtype == FLUX_F32)
     hipLaunchKernelGGL((k_convert<float, float>), dim3(g), dim3(256), 0, st, total,
                        (const float *)x, (float *)y);
+  else if (xdtype == FLUX_F32 && ydtype == FLUX_BF16)
+    hipLaunchKernelGGL((k_convert<float, __hip_bfloat16>), dim3(g), dim3(256), 0, st, total,
+                       (const float *)x, (__hip_bfloat16 *)y);
+  else if (xdtype == FLUX_F32 && ydtype == FLUX_F16)
+    hipLaunchKernelGGL((k_convert<float, __half>), dim3(g), dim3(256), 0, st, total,
+                       (const float *)x, (__half *)y);
+  else if (xdtype == FLUX_BF16 && ydtype == FLUX_F32)
+    hipLaunchKernelGGL((k_convert<__hip_bfloat16, float>), dim3(g), dim3(256), 0, st, total,
+                       (const __hip_bfloat16 *)x, (float *)y);
+  else if (xdtype == FLUX_F16 && ydtype == FLUX_F32)
+    hipLaunchKernelGGL((k_convert<__half, float>), dim3(g), dim3(256), 0, st, total,
+                       (const __half *)x, (float *)y);
   else
     FLUX_CHECK(false, "flux_vec_convert: unsupported dtypes");
   FLUX_LAUNCH_CHECK();

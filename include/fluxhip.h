@@ -142,6 +142,14 @@ int flux_ff_mask(const flux_scene *scene, const void *P, const void *N,
                  int64_t n, double eps, int mode, uint64_t *mask,
                  int64_t *row_nnz, void *stream);
 
+/* The prefilter alone: bit q of row p of `mask` is set iff the pair passes
+ * the eps test (form_factors.py:46-52), i.e. iff flux_ff_mask casts a ray
+ * for it; row_cnt[p] = popcount.  Used to count rays for Mrays/s. */
+int flux_ff_candidates(const flux_scene *scene, const void *P, const void *N,
+                       int dtype, const int64_t *I, int64_t m, const int64_t *J,
+                       int64_t n, double eps, uint64_t *mask, int64_t *row_cnt,
+                       void *stream);
+
 /* Pass 2: CSR fill.  indptr: m+1 int64 (exclusive scan of row_nnz),
  * indices: int32 column positions q, data: F_ij = c*A_j/(pi*|P_i-P_j|^4)
  * in `dtype` (0 where P_i == P_j, as s == 0 -> inf in the reference). */
