@@ -1,0 +1,736 @@
+"""GPU build of the compressed form-factor matrix and its device operator.
+
+Build (called from CompressedFormFactorMatrix.__init__):
+
+1. cluster tree of the centroids -> permutation (every cluster contiguous);
+2. F in cluster order on the GPU (form-factor kernels with I = J = perm) and
+   its transpose (flux_csr_transpose);
+3. level by level: batched block statistics (flux_block_stats on F and on
+   F^T), the reference's per-block decisions (compressed_form_factors.py:
+   285-386), and one batched randomized SVD per level for the blocks whose
+   k = 40 factors could beat the CSR bytes (linalg.py:20-50 rank rule);
+4. bottom-up choice CSR / SVD / children with the reference's byte counts
+   and the all-dense / all-sparse collapse;
+5. device operator: one near-field CSR (F minus the SVD leaves) plus the
+   far-field factors, all in cluster order.
+"""
+import time
+import warnings
+
+import numpy as np
+import scipy.sparse
+
+from . import _kernels
+from . import _lib
+from .compressed_form_factors import (
+    FormFactorCsrBlock, FormFactorDenseBlock, FormFactorNullBlock, FormFactorOctreeBlock,
+    FormFactorQuadtreeBlock, FormFactorSvdBlock, FormFactorZeroBlock, _ClusterTree,
+    _csr_nbytes, _svd_nbytes)
+
+K0 = 40               # linalg.py:20 (k0 = 40)
+L_MAX = 1024          # largest sketch the batched eigensolver accepts
+EIG_CUT = 1e-14       # drop sketch directions with D_i <= EIG_CUT * D_0
+SKETCH_BYTES = 2 << 30
+
+
+class _Node:
+    __slots__ = ("rc", "cc", "rng", "m", "n", "parts_r", "parts_c", "children", "max_depth",
+                 "force", "nnz", "nzr", "nzc", "kind", "nb_sparse", "svd", "final",
+                 "nbytes", "far_id")
+
+    def __init__(self, rc, cc, max_depth, force):
+        self.rc, self.cc = rc, cc
+        self.rng = (rc.start, rc.end, cc.start, cc.end)
+        self.m, self.n = rc.end - rc.start, cc.end - cc.start
+        self.children = None
+        self.max_depth = max_depth
+        self.force = force
+        self.svd = None
+        self.final = None
+        self.far_id = -1
+
+
+def _expand(ct, node, max_depth, force):
+    rk, rparts = ct.split(node.rc)
+    ck, cparts = ct.split(node.cc)
+    node.parts_r, node.parts_c = rparts, cparts
+    node.children = [[_Node(a, b, max_depth, force) for b in ck] for a in rk]
+    return [ch for row in node.children for ch in row]
+
+
+def _i64(a):
+    return _lib.to_dev(np.ascontiguousarray(np.asarray(a, dtype=np.int64)))
+
+
+def _stats(F, FT, nodes):
+    t = _lib.torch()
+    nb = len(nodes)
+    if nb == 0:
+        return
+    blk = np.array([nd.rng for nd in nodes], dtype=np.int64)
+    blkT = blk[:, [2, 3, 0, 1]].copy()
+    dev = _lib.device()
+    nnz = t.empty(nb, dtype=t.int64, device=dev)
+    nzr = t.empty(nb, dtype=t.int64, device=dev)
+    nnzT = t.empty(nb, dtype=t.int64, device=dev)
+    nzc = t.empty(nb, dtype=t.int64, device=dev)
+    st = _lib.stream()
+    bd, bdT = _i64(blk), _i64(blkT)
+    _lib.call("flux_block_stats", F.indptr.data_ptr(), F.indices.data_ptr(), nb, bd.data_ptr(),
+              nnz.data_ptr(), nzr.data_ptr(), st)
+    _lib.call("flux_block_stats", FT.indptr.data_ptr(), FT.indices.data_ptr(), nb,
+              bdT.data_ptr(), nnzT.data_ptr(), nzc.data_ptr(), st)
+    nnz, nzr, nzc = nnz.cpu().numpy(), nzr.cpu().numpy(), nzc.cpu().numpy()
+    for i, nd in enumerate(nodes):
+        nd.nnz, nd.nzr, nd.nzc = int(nnz[i]), int(nzr[i]), int(nzc[i])
+
+
+# --------------------------------------------------------------------------
+# batched randomized SVD of CSR sub-blocks
+# --------------------------------------------------------------------------
+
+class _SvdBatch:
+    """Sketch + Gram + eigensolver for a batch of blocks; keeps the
+    intermediates so accepted blocks can be factorised afterwards."""
+
+    def __init__(self, F, FT, rngs, ls, seed):
+        t = _lib.require_gpu()
+        dev = _lib.device()
+        st = _lib.stream()
+        self.rngs = np.array(rngs, dtype=np.int64).reshape(-1, 4)
+        self.ls = np.array(ls, dtype=np.int64)
+        nb = len(self.ls)
+        m = self.rngs[:, 1] - self.rngs[:, 0]
+        n = self.rngs[:, 3] - self.rngs[:, 2]
+        self.m, self.n, self.nb = m, n, nb
+        l = self.ls
+        offY = np.concatenate([[0], np.cumsum(m * l)])
+        offZ = np.concatenate([[0], np.cumsum(n * l)])
+        offG = np.concatenate([[0], np.cumsum(l * l)])
+        offE = np.concatenate([[0], np.cumsum(l)])
+        self.offY, self.offZ, self.offG, self.offE = offY, offZ, offG, offE
+        f64 = t.float64
+        self.Y = t.empty(max(int(offY[-1]), 1), dtype=f64, device=dev)
+        self.Z = t.empty(max(int(offZ[-1]), 1), dtype=f64, device=dev)
+        G = t.empty(max(int(offG[-1]), 1), dtype=f64, device=dev)
+        Vw = t.empty_like(G)
+        Vm = t.empty_like(G)
+        d = t.empty(max(int(offE[-1]), 1), dtype=f64, device=dev)
+        self.Tm = t.empty_like(G)
+        kept = t.empty(nb, dtype=t.int64, device=dev)
+        rowstart = np.concatenate([[0], np.cumsum(m)])[:-1]
+        rowstartT = np.concatenate([[0], np.cumsum(n)])[:-1]
+        d_blk, d_blkT = _i64(self.rngs), _i64(self.rngs[:, [2, 3, 0, 1]])
+        d_ls = _i64(l)
+        d_offY, d_offZ, d_offG, d_offE = _i64(offY[:-1]), _i64(offZ[:-1]), _i64(offG[:-1]), \
+            _i64(offE[:-1])
+        code = _lib.dtype_code(F.data.dtype)
+        _lib.call("flux_block_spmm", F.indptr.data_ptr(), F.indices.data_ptr(), F.data.data_ptr(),
+                  code, nb, d_blk.data_ptr(), d_ls.data_ptr(), _i64(rowstart).data_ptr(),
+                  int(m.sum()), None, None, self.Y.data_ptr(), d_offY.data_ptr(), seed, st)
+        _lib.call("flux_block_spmm", FT.indptr.data_ptr(), FT.indices.data_ptr(),
+                  FT.data.data_ptr(), code, nb, d_blkT.data_ptr(), d_ls.data_ptr(),
+                  _i64(rowstartT).data_ptr(), int(n.sum()), self.Y.data_ptr(),
+                  d_offY.data_ptr(), self.Z.data_ptr(), d_offZ.data_ptr(), seed, st)
+        lmax = int(l.max())
+        d_m, d_n = _i64(m), _i64(n)
+        _lib.call("flux_batched_gram", nb, d_m.data_ptr(), d_ls.data_ptr(), lmax,
+                  self.Y.data_ptr(), d_offY.data_ptr(), G.data_ptr(), d_offG.data_ptr(), st)
+        _lib.call("flux_batched_syevj", nb, d_ls.data_ptr(), lmax, G.data_ptr(),
+                  d_offG.data_ptr(), Vw.data_ptr(), Vm.data_ptr(), d.data_ptr(),
+                  d_offE.data_ptr(), 30, st)
+        _lib.call("flux_scale_basis", nb, d_ls.data_ptr(), Vm.data_ptr(), d_offG.data_ptr(),
+                  d.data_ptr(), d_offE.data_ptr(), EIG_CUT, self.Tm.data_ptr(),
+                  kept.data_ptr(), st)
+        _lib.call("flux_batched_gram", nb, d_n.data_ptr(), d_ls.data_ptr(), lmax,
+                  self.Z.data_ptr(), d_offZ.data_ptr(), G.data_ptr(), d_offG.data_ptr(), st)
+        # K = Tm^T (Z^T Z) Tm
+        H1 = Vw
+        desc = np.array([[li, li, li, oG, li, 0, oG, li, 0, oG, li, 0]
+                         for li, oG in zip(l, offG[:-1])], dtype=np.int64)
+        tiles = int(((l + 15) // 16).max() ** 2)
+        _lib.call("flux_batched_gemm", nb, _i64(desc).data_ptr(), tiles, G.data_ptr(),
+                  self.Tm.data_ptr(), H1.data_ptr(), st)
+        desc[:, 5] = 1
+        _lib.call("flux_batched_gemm", nb, _i64(desc).data_ptr(), tiles, self.Tm.data_ptr(),
+                  H1.data_ptr(), G.data_ptr(), st)
+        self.Xe = Vm
+        self.lam = d
+        _lib.call("flux_batched_syevj", nb, d_ls.data_ptr(), lmax, G.data_ptr(),
+                  d_offG.data_ptr(), Vw.data_ptr(), self.Xe.data_ptr(), self.lam.data_ptr(),
+                  d_offE.data_ptr(), 30, st)
+        lam = self.lam.cpu().numpy()
+        self.sigma = [np.sqrt(np.maximum(lam[offE[b]:offE[b + 1]], 0.0)) for b in range(nb)]
+        self.kept = kept.cpu().numpy()
+
+    def factors(self, picks, out_dtype):
+        """U (m x r), S (r), Vt (r x n) for (batch index, rank) pairs, as
+        device tensors in out_dtype."""
+        t = _lib.torch()
+        dev = _lib.device()
+        st = _lib.stream()
+        res = []
+        if not picks:
+            return res
+        bs = np.array([p[0] for p in picks], dtype=np.int64)
+        rs = np.array([p[1] for p in picks], dtype=np.int64)
+        l = self.ls[bs]
+        m, n = self.m[bs], self.n[bs]
+        np_ = len(picks)
+        offC = np.concatenate([[0], np.cumsum(l * rs)])
+        offU = np.concatenate([[0], np.cumsum(m * rs)])
+        offV = np.concatenate([[0], np.cumsum(n * rs)])
+        C = t.empty(max(int(offC[-1]), 1), dtype=t.float64, device=dev)
+        C2 = t.empty_like(C)
+        U = t.empty(max(int(offU[-1]), 1), dtype=t.float64, device=dev)
+        Vt = t.empty(max(int(offV[-1]), 1), dtype=t.float64, device=dev)
+        oG = self.offG[bs]
+        oE = self.offE[bs]
+        # C = Tm X[:, :r]
+        desc = np.array([[li, r, li, g, li, 0, g, li, 0, c, r, 0]
+                         for li, r, g, c in zip(l, rs, oG, offC[:-1])], dtype=np.int64)
+        tiles = int((((l + 15) // 16) * ((rs + 15) // 16)).max())
+        _lib.call("flux_batched_gemm", np_, _i64(desc).data_ptr(), tiles, self.Tm.data_ptr(),
+                  self.Xe.data_ptr(), C.data_ptr(), st)
+        C2.copy_(C)
+        dims = np.stack([l, rs], 1).astype(np.int64)
+        _lib.call("flux_scale_cols_inv_sqrt", np_, _i64(dims).data_ptr(), C2.data_ptr(),
+                  _i64(offC[:-1]).data_ptr(), self.lam.data_ptr(), _i64(oE).data_ptr(), st)
+        # U = Y C  (m x r)
+        desc = np.array([[mi, r, li, y, li, 0, c, r, 0, u, r, 0]
+                         for mi, r, li, y, c, u in zip(m, rs, l, self.offY[bs], offC[:-1],
+                                                       offU[:-1])], dtype=np.int64)
+        tiles = int((((m + 15) // 16) * ((rs + 15) // 16)).max())
+        _lib.call("flux_batched_gemm", np_, _i64(desc).data_ptr(), tiles, self.Y.data_ptr(),
+                  C.data_ptr(), U.data_ptr(), st)
+        # Vt = C2^T Z^T  (r x n)
+        desc = np.array([[r, ni, li, c, r, 1, z, li, 1, v, ni, 0]
+                         for r, ni, li, c, z, v in zip(rs, n, l, offC[:-1], self.offZ[bs],
+                                                       offV[:-1])], dtype=np.int64)
+        tiles = int((((rs + 15) // 16) * ((n + 15) // 16)).max())
+        _lib.call("flux_batched_gemm", np_, _i64(desc).data_ptr(), tiles, C2.data_ptr(),
+                  self.Z.data_ptr(), Vt.data_ptr(), st)
+        lam = self.lam
+        for q in range(np_):
+            b, r = int(bs[q]), int(rs[q])
+            u = U[offU[q]:offU[q + 1]].reshape(int(m[q]), r)
+            vt = Vt[offV[q]:offV[q + 1]].reshape(r, int(n[q]))
+            s = lam[self.offE[b]:self.offE[b] + r].clamp_min(0).sqrt()
+            res.append((_kernels.convert(u.contiguous(), out_dtype),
+                        _kernels.convert(s.contiguous(), out_dtype),
+                        _kernels.convert(vt.contiguous(), out_dtype)))
+        del C, C2, U, Vt
+        return res
+
+
+def _rank_rule(sigma, k, tol):
+    """First index r < k with s_r / s_0 <= tol (linalg.py:42-46), or None."""
+    s = np.zeros(k)
+    q = min(k, sigma.size)
+    s[:q] = sigma[:q]
+    if s[0] <= 0:
+        return None
+    below = np.where(s / s[0] <= tol)[0]
+    return int(below[0]) if below.size else None
+
+
+def _run_svds(H, F, FT, cands, itemsize, tdt, far):
+    """The estimate_rank loop (linalg.py:20-50) for a level's candidates."""
+    tol = H._tol
+    state = {}
+    for nd in cands:
+        state[id(nd)] = K0
+    pending = list(cands)
+    while pending:
+        # one batched sketch per round, chunked by memory
+        work = []
+        for nd in pending:
+            mm = min(nd.m, nd.n)
+            k = min(state[id(nd)], mm - 1)
+            l = min(2 * k + 8, mm, L_MAX)
+            work.append((nd, k, l))
+        nxt = []
+        i = 0
+        while i < len(work):
+            chunk, mem = [], 0
+            while i < len(work) and len(chunk) < 4096:
+                nd, k, l = work[i]
+                need = 8 * (nd.m + nd.n + 4 * l) * l
+                if chunk and mem + need > SKETCH_BYTES:
+                    break
+                chunk.append(work[i])
+                mem += need
+                i += 1
+            sb = _SvdBatch(F, FT, [c[0].rng for c in chunk], [c[2] for c in chunk],
+                           H._seed + len(far))
+            H._build_stats["svd_blocks"] = H._build_stats.get("svd_blocks", 0) + len(chunk)
+            picks = []
+            for q, (nd, k, l) in enumerate(chunk):
+                mm = min(nd.m, nd.n)
+                r = _rank_rule(sb.sigma[q], k, tol)
+                if r is not None:
+                    nb = _svd_nbytes(nd.m, nd.n, nd.nzr, nd.nzc, r, itemsize)
+                    nd.svd = (r, nb)
+                    picks.append((q, r, nd))
+                    continue
+                if k >= mm - 1 or l >= L_MAX:
+                    nd.svd = None
+                    continue
+                k2 = min(2 * k, mm - 1)
+                if _svd_nbytes(nd.m, nd.n, nd.nzr, nd.nzc, k2, itemsize) >= nd.nb_sparse:
+                    nd.svd = None
+                    continue
+                state[id(nd)] = k2
+                nxt.append(nd)
+            facs = sb.factors([(q, r) for q, r, _ in picks], tdt)
+            for (q, r, nd), f in zip(picks, facs):
+                nd.far_id = len(far)
+                far.append((nd.rng, r, f))
+            del sb
+        pending = nxt
+
+
+def _resolve(nd, itemsize):
+    """Bottom-up block choice; sets nd.final in {zero, null, dense, csr, svd,
+    tree} and nd.nbytes (compressed_form_factors.py:330-386)."""
+    if nd.final is not None:
+        return
+    if nd.children is not None:
+        for row in nd.children:
+            for ch in row:
+                _resolve(ch, itemsize)
+    if nd.kind == "forced":
+        kinds = [ch.final for row in nd.children for ch in row]
+        if all(k == "dense" for k in kinds):
+            nd.final, nd.nbytes = "dense", nd.m * nd.n * itemsize
+        elif all(k in ("csr", "zero") for k in kinds):
+            nd.final, nd.nbytes = "csr", nd.nnz * itemsize
+        else:
+            nd.final = "tree"
+            nd.nbytes = 8 * (nd.m + nd.n) + sum(ch.nbytes for row in nd.children for ch in row)
+        return
+    if nd.kind in ("zero", "null"):
+        nd.final, nd.nbytes = nd.kind, 0
+        return
+    if nd.kind in ("dense", "csr"):
+        nd.final = nd.kind
+        nd.nbytes = nd.m * nd.n * itemsize if nd.kind == "dense" else nd.nnz * itemsize
+        return
+    # pending: choose among sparse / svd / children
+    nb_sparse = nd.nb_sparse
+    nb_svd = np.inf if nd.svd is None else nd.svd[1]
+    if nd.children is not None:
+        nb_child = 8 * (nd.m + nd.n) + sum(ch.nbytes for row in nd.children for ch in row)
+    else:
+        nb_child = np.inf
+    nb_min = min(nb_sparse, nb_svd, nb_child)
+    if nb_sparse == nb_min:
+        nd.final, nd.nbytes = "csr", nd.nnz * itemsize
+    elif nb_svd == nb_min:
+        nd.final, nd.nbytes = "svd", nb_svd
+    else:
+        kinds = [ch.final for row in nd.children for ch in row]
+        if all(k == "dense" for k in kinds):
+            nd.final, nd.nbytes = "dense", nd.m * nd.n * itemsize
+        elif all(k in ("csr", "zero") for k in kinds):
+            nd.final, nd.nbytes = "csr", nd.nnz * itemsize
+        else:
+            nd.final, nd.nbytes = "tree", nb_child
+
+
+def _make_blocks(H, nd, TreeCls):
+    """Python block objects mirroring the reference classes."""
+    if nd.final == "tree":
+        blk = TreeCls.__new__(TreeCls)
+        blk._root, blk.shape = H, (nd.m, nd.n)
+        blk._row_block_inds = [np.asarray(p, dtype=np.int64) for p in nd.parts_r]
+        blk._col_block_inds = [np.asarray(p, dtype=np.int64) for p in nd.parts_c]
+        B = np.empty((len(nd.children), len(nd.children[0])), dtype=object)
+        for i, row in enumerate(nd.children):
+            for j, ch in enumerate(row):
+                B[i, j] = _make_blocks(H, ch, TreeCls)
+        blk._blocks = B
+        return blk
+    shape = (nd.m, nd.n)
+    if nd.final == "zero":
+        return FormFactorZeroBlock(H, shape, nd.rng)
+    if nd.final == "null":
+        return FormFactorNullBlock(H, shape, nd.rng)
+    if nd.final == "dense":
+        return FormFactorDenseBlock(H, shape, nd.rng)
+    if nd.final == "csr":
+        return FormFactorCsrBlock(H, shape, nd.rng, nd.nnz)
+    return FormFactorSvdBlock(H, shape, nd.rng, nd.svd[0], nd.svd[1], nd.far_id)
+
+
+def _collect(nd, out_far, out_ranges):
+    if nd.final == "tree":
+        for row in nd.children:
+            for ch in row:
+                _collect(ch, out_far, out_ranges)
+    elif nd.final == "svd":
+        out_far.append(nd.far_id)
+    out_ranges.append(nd)
+
+
+def build(H, dims):
+    t = _lib.require_gpu()
+    t0 = time.perf_counter()
+    sm = H.shape_model
+    dt = np.dtype(sm.dtype)
+    itemsize = dt.itemsize
+    tdt = _lib.torch_dtype(dt)
+    ct = _ClusterTree(np.asarray(sm.P), dims)
+    H._ctree = ct
+    t1 = time.perf_counter()
+    F = _kernels.ff_csr(sm, ct.perm, ct.perm)
+    FT = _kernels.csr_transpose(F)
+    t.cuda.synchronize()
+    t2 = time.perf_counter()
+    H._F = F
+    TreeCls = FormFactorQuadtreeBlock if dims == 2 else FormFactorOctreeBlock
+    root = _Node(ct.root, ct.root, H._max_depth, H._force_max_depth)
+    level = _expand(ct, root, H._max_depth, H._force_max_depth)
+    far = []
+    nlev = 0
+    while level:
+        nlev += 1
+        _stats(F, FT, level)
+        nxt, cands = [], []
+        for nd in level:
+            if nd.force and nd.max_depth is not None and nd.max_depth > 1:
+                nd.kind = "forced"
+                nxt.extend(_expand(ct, nd, nd.max_depth - 1, True))
+                continue
+            if nd.nnz == 0:
+                nd.kind = "zero"
+                continue
+            if nd.m == 0 or nd.n == 0:
+                nd.kind = "null"
+                continue
+            nd.nb_sparse = _csr_nbytes(nd.m, nd.nnz, itemsize)
+            if nd.m * nd.n < H._min_size:
+                nd.kind = "dense" if nd.m * nd.n * itemsize < nd.nb_sparse else "csr"
+                continue
+            nd.kind = "pending"
+            mm = min(nd.m, nd.n)
+            if mm >= 2 and _svd_nbytes(nd.m, nd.n, nd.nzr, nd.nzc, min(K0, mm - 1),
+                                       itemsize) < nd.nb_sparse:
+                cands.append(nd)
+            if nd.max_depth is None or nd.max_depth > 1:
+                nxt.extend(_expand(ct, nd, None if nd.max_depth is None else nd.max_depth - 1,
+                                   False))
+        if cands:
+            _run_svds(H, F, FT, cands, itemsize, tdt, far)
+        level = nxt
+    t3 = time.perf_counter()
+    for row in root.children:
+        for ch in row:
+            _resolve(ch, itemsize)
+    # root is always a tree block (it is constructed, not chosen)
+    root.final = "tree"
+    H._root = _make_blocks(H, root, TreeCls)
+    # device operator data
+    used = []
+    leaves = []
+    _collect(root, used, leaves)
+    H._far = [(far[i][0], far[i][1], far[i][2]) if i in set(used) else None
+              for i in range(len(far))]
+    H._far_used = sorted(set(used))
+    H._hop = {}
+    _build_near(H)
+    t.cuda.synchronize()
+    t4 = time.perf_counter()
+    H._build_stats.update({
+        "facets": sm.num_faces, "nnz": F.nnz, "levels": nlev,
+        "cluster_s": t1 - t0, "assembly_s": t2 - t1, "tree_s": t3 - t2, "finish_s": t4 - t3,
+        "total_s": t4 - t0, "svd_leaves": len(H._far_used),
+        "leaf_counts": _leaf_counts(root),
+    })
+
+
+def _leaf_counts(root):
+    out = {}
+
+    def rec(nd):
+        if nd.final == "tree":
+            for row in nd.children:
+                for ch in row:
+                    rec(ch)
+        else:
+            out[nd.final] = out.get(nd.final, 0) + 1
+    for row in root.children:
+        for ch in row:
+            rec(ch)
+    return out
+
+
+def _build_near(H):
+    """Near-field CSR = F minus the entries inside SVD leaves."""
+    F = H._F
+    if not H._far_used:
+        H._near = F
+        return
+    # drop the entries inside far rectangles (setup step, run once)
+    indptr = F.indptr.cpu().numpy()
+    indices = F.indices.cpu().numpy()
+    keep = np.ones(indices.size, dtype=bool)
+    for fid in H._far_used:
+        (r0, r1, c0, c1), r, _ = H._far[fid]
+        a, b = indptr[r0], indptr[r1]
+        seg = indices[a:b]
+        keep[a:b] &= ~((seg >= c0) & (seg < c1))
+    t = _lib.torch()
+    kd = t.from_numpy(keep).to(F.data.device)
+    ck = np.concatenate([[0], np.cumsum(keep.astype(np.int64))])
+    new_indptr = ck[indptr]
+    H._near = _kernels.DeviceCSR(_lib.to_dev(new_indptr), F.indices[kd].contiguous(),
+                                 F.data[kd].contiguous(), F.shape)
+
+
+# --------------------------------------------------------------------------
+# device operator
+# --------------------------------------------------------------------------
+
+_VALUE_DTYPES = {"fp64": "float64", "fp32": "float32", "bf16": "bfloat16", "fp16": "float16"}
+
+
+class HOperator:
+    """Compressed F in cluster order: near-field CSR + far-field factors."""
+
+    def __init__(self, H, values):
+        t = _lib.require_gpu()
+        tdt = getattr(t, _VALUE_DTYPES[values])
+        self.H = H
+        self.values = values
+        self.shape = H.shape
+        self.n = H.shape[0]
+        near = H._near
+        self.near = near if near.data.dtype == tdt else near.astype(tdt)
+        self.vec_dtype = t.float64 if values == "fp64" else t.float32
+        self.perm = _lib.to_dev(H._ctree.perm)
+        self.nfar = len(H._far_used)
+        self.fused_jacobi = self.nfar == 0
+        if self.nfar:
+            self._setup_far(tdt)
+
+    def _setup_far(self, tdt):
+        t = _lib.torch()
+        H = self.H
+        descs, Us, Ss, Vts = [], [], [], []
+        oU = oV = oS = 0
+        ranks = []
+        for fid in H._far_used:
+            (r0, r1, c0, c1), r, (u, s, vt) = H._far[fid]
+            descs.append([r0, r1, c0, c1, r, oU, oV, oS])
+            oU += u.numel()
+            oV += vt.numel()
+            oS += s.numel()
+            ranks.append(r)
+            Us.append(u.reshape(-1))
+            Ss.append(s.reshape(-1))
+            Vts.append(vt.reshape(-1))
+
+        def cat(xs):
+            x = t.cat(xs)
+            return x if x.dtype == tdt else _kernels.convert(
+                x if x.dtype == t.float32 else _kernels.convert(x, t.float32), tdt)
+
+        self.U, self.S, self.Vt = cat(Us), cat(Ss), cat(Vts)
+        self.desc = _i64(np.array(descs, dtype=np.int64))
+        self.ranks = np.array(ranks, dtype=np.int64)
+        # rows -> far blocks covering them, through a per-row cluster id
+        n = self.n
+        rowcl = np.arange(n, dtype=np.int32)
+        lists = [[] for _ in range(n)]
+        for b, d in enumerate(descs):
+            for i in range(d[0], d[1]):
+                lists[i].append(b)
+        cptr = np.zeros(n + 1, dtype=np.int64)
+        cptr[1:] = np.cumsum([len(x) for x in lists])
+        cblk = np.array([b for x in lists for b in x], dtype=np.int32)
+        self.rowcl = _lib.to_dev(rowcl)
+        self.cptr = _i64(cptr)
+        self.cblk = _lib.to_dev(cblk if cblk.size else np.zeros(1, np.int32))
+        self._tstart = {}
+
+    def _tst(self, nrhs):
+        if nrhs not in self._tstart:
+            ts = np.concatenate([[0], np.cumsum(self.ranks * nrhs)])
+            self._tstart[nrhs] = (_i64(ts[:-1]), int(ts[-1]))
+        return self._tstart[nrhs]
+
+    # cluster order <-> facet order
+    def to_internal(self, X):
+        t = _lib.torch()
+        X2 = X.reshape(X.shape[0], -1).contiguous()
+        out = t.empty_like(X2)
+        _lib.call("flux_permute_rows", X2.shape[0], X2.shape[1], self.perm.data_ptr(), 0,
+                  X2.data_ptr(), out.data_ptr(), _lib.dtype_code(X2.dtype), _lib.stream())
+        return out.reshape(X.shape)
+
+    def from_internal(self, X):
+        t = _lib.torch()
+        X2 = X.reshape(X.shape[0], -1).contiguous()
+        out = t.empty_like(X2)
+        _lib.call("flux_permute_rows", X2.shape[0], X2.shape[1], self.perm.data_ptr(), 1,
+                  X2.data_ptr(), out.data_ptr(), _lib.dtype_code(X2.dtype), _lib.stream())
+        return out.reshape(X.shape)
+
+    def apply(self, X, out=None):
+        """Y = F X in cluster order (X: n or n x k device tensor)."""
+        t = _lib.torch()
+        vec = X.dim() == 1
+        X2 = X.reshape(X.shape[0], -1).contiguous()
+        Y = self.near.matmat(X2, out=out)
+        if self.nfar:
+            k = X2.shape[1]
+            tst, total = self._tst(k)
+            acc = t.float64 if (X2.dtype == t.float64 or self.U.dtype == t.float64) \
+                else t.float32
+            T = t.empty(max(total, 1), dtype=acc, device=X2.device)
+            _lib.call("flux_far_matmat", self.n, k, self.nfar, self.desc.data_ptr(),
+                      tst.data_ptr(), total, self.U.data_ptr(), self.S.data_ptr(),
+                      self.Vt.data_ptr(), _lib.dtype_code(self.U.dtype),
+                      self.rowcl.data_ptr(), self.cptr.data_ptr(), self.cblk.data_ptr(),
+                      X2.data_ptr(), Y.data_ptr(), _lib.dtype_code(X2.dtype), T.data_ptr(),
+                      _lib.stream())
+        return Y.reshape(-1) if vec else Y
+
+    def jacobi_step(self, E, B, rho, left, B1, colmax):
+        t = _lib.torch()
+        if self.fused_jacobi:
+            n, k = B.shape
+            rho_t, rho_s = (rho, 0.0) if isinstance(rho, t.Tensor) else (None, float(rho))
+            c = self.near
+            _lib.call("flux_csr_jacobi_step", n, c.indptr.data_ptr(), c.indices.data_ptr(),
+                      c.data.data_ptr(), _lib.dtype_code(c.data.dtype), E.data_ptr(),
+                      B.data_ptr(), k, _lib.ptr(rho_t), rho_s, int(left), B1.data_ptr(),
+                      _lib.dtype_code(B.dtype), colmax.data_ptr(), _lib.stream())
+            return
+        X = B if left else _kernels.rowscale(rho, B)
+        Y = self.apply(X)
+        rho_t, rho_s = (rho, 0.0) if isinstance(rho, t.Tensor) else (None, float(rho))
+        n, k = B.shape
+        _lib.call("flux_jacobi_update", n, k, E.data_ptr(), Y.data_ptr(), B.data_ptr(),
+                  _lib.ptr(rho_t), rho_s, int(left), B1.data_ptr(), _lib.dtype_code(B.dtype),
+                  colmax.data_ptr(), _lib.stream())
+
+    def nbytes_device(self):
+        nb = self.near.nbytes
+        if self.nfar:
+            nb += (self.U.numel() + self.S.numel() + self.Vt.numel()) * self.U.element_size()
+        return nb
+
+
+def operator(H, values=None):
+    if values is None:
+        values = "fp64" if np.dtype(H.dtype) == np.float64 else "fp32"
+    if values not in _VALUE_DTYPES:
+        raise ValueError(f"values must be one of {sorted(_VALUE_DTYPES)}")
+    if getattr(H, "_hop", None) is None:
+        H._hop = {}
+    if values not in H._hop:
+        H._hop[values] = HOperator(H, values)
+    return H._hop[values]
+
+
+# --------------------------------------------------------------------------
+# host views, pickling
+# --------------------------------------------------------------------------
+
+def leaf_csr(H, rng):
+    r0, r1, c0, c1 = rng
+    F = H._F
+    ip = F.indptr[r0:r1 + 1].cpu().numpy()
+    a, b = int(ip[0]), int(ip[-1])
+    idx = F.indices[a:b].cpu().numpy().astype(np.int64)
+    dat = F.data[a:b].cpu().numpy()
+    rows = np.repeat(np.arange(r1 - r0), np.diff(ip))
+    sel = (idx >= c0) & (idx < c1)
+    perm = H._ctree.perm
+    ro = np.argsort(perm[r0:r1])          # ascending global row ids
+    co = np.argsort(perm[c0:c1])
+    rinv = np.empty_like(ro)
+    rinv[ro] = np.arange(ro.size)
+    cinv = np.empty_like(co)
+    cinv[co] = np.arange(co.size)
+    sp = scipy.sparse.csr_matrix((dat[sel], (rinv[rows[sel]], cinv[idx[sel] - c0])),
+                                 shape=(r1 - r0, c1 - c0))
+    sp.sort_indices()
+    return sp
+
+
+def far_factors(H, fid):
+    (r0, r1, c0, c1), r, (u, s, vt) = H._far[fid]
+    perm = H._ctree.perm
+    ro = np.argsort(perm[r0:r1])
+    co = np.argsort(perm[c0:c1])
+    return u.cpu().numpy()[ro], s.cpu().numpy(), vt.cpu().numpy()[:, co]
+
+
+def state(H):
+    F = H._F
+    far = []
+    for i, f in enumerate(H._far):
+        if f is None:
+            far.append(None)
+        else:
+            rng, r, (u, s, vt) = f
+            far.append((rng, r, (u.cpu().numpy(), s.cpu().numpy(), vt.cpu().numpy())))
+    return {
+        "shape_model": H.shape_model, "tol": H._tol, "min_size": H._min_size,
+        "max_depth": H._max_depth, "force": H._force_max_depth,
+        "RootBlock": H._RootBlock, "seed": H._seed, "ctree": H._ctree,
+        "root": _strip(H._root),
+        "F": (F.indptr.cpu().numpy(), F.indices.cpu().numpy(), F.data.cpu().numpy(), F.shape),
+        "far": far, "far_used": H._far_used, "stats": H._build_stats,
+    }
+
+
+def restore(st):
+    from .compressed_form_factors import CompressedFormFactorMatrix
+    H = CompressedFormFactorMatrix.__new__(CompressedFormFactorMatrix)
+    H.shape_model = st["shape_model"]
+    H._tol, H._min_size = st["tol"], st["min_size"]
+    H._max_depth, H._force_max_depth = st["max_depth"], st["force"]
+    H._RootBlock, H._seed = st["RootBlock"], st["seed"]
+    H._ctree = st["ctree"]
+    H._root = st["root"]
+    ip, ix, dt, shape = st["F"]
+    H._F = _kernels.DeviceCSR(_lib.to_dev(ip), _lib.to_dev(ix), _lib.to_dev(dt), shape)
+    H._far = []
+    for f in st["far"]:
+        if f is None:
+            H._far.append(None)
+        else:
+            rng, r, (u, s, vt) = f
+            H._far.append((rng, r, (_lib.to_dev(u), _lib.to_dev(s), _lib.to_dev(vt))))
+    H._far_used = st["far_used"]
+    H._build_stats = st["stats"]
+    H._hop = {}
+    _rebind(H._root, H)
+    _build_near(H)
+    return H
+
+
+def _strip(blk):
+    """Copy of a block tree without the back-references to the matrix (they
+    are restored by _rebind) and without host caches."""
+    import copy as _copy
+    c = _copy.copy(blk)
+    c._root = None
+    if hasattr(c, "_cache"):
+        c._cache = None
+    if not blk.is_leaf:
+        c._blocks = np.empty(blk._blocks.shape, dtype=object)
+        for idx in np.ndindex(blk._blocks.shape):
+            c._blocks[idx] = _strip(blk._blocks[idx])
+    return c
+
+
+def _rebind(blk, H):
+    blk._root = H
+    if not blk.is_leaf:
+        for b in blk._blocks.ravel():
+            _rebind(b, H)

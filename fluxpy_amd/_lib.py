@@ -52,7 +52,21 @@ SIGNATURES = {
                                     _vp]),
     "flux_direct_irradiance":(_c_int, [_vp, _vp, _vp, _c_int, _vp, _c_i64, _c_dbl, _c_int,
                                         _vp, _vp]),
-    "flux_vec_reduce": (_c_int, [_c_i64, _c_i64, _vp, _vp, _c_int, _c_int, _vp, _vp, _vp]),
+    "flux_block_stats": (_c_int, [_vp, _vp, _c_i64, _vp, _vp, _vp, _vp]),
+    "flux_block_spmm": (_c_int, [_vp, _vp, _vp, _c_int, _c_i64, _vp, _vp, _vp, _c_i64, _vp,
+                                 _vp, _vp, _vp, ctypes.c_uint64, _vp]),
+    "flux_batched_gram": (_c_int, [_c_i64, _vp, _vp, _c_i64, _vp, _vp, _vp, _vp, _vp]),
+    "flux_batched_gemm": (_c_int, [_c_i64, _vp, _c_i64, _vp, _vp, _vp, _vp]),
+    "flux_batched_syevj": (_c_int, [_c_i64, _vp, _c_i64, _vp, _vp, _vp, _vp, _vp, _vp, _c_int,
+                                    _vp]),
+    "flux_scale_basis": (_c_int, [_c_i64, _vp, _vp, _vp, _vp, _vp, _c_dbl, _vp, _vp, _vp]),
+    "flux_scale_cols_inv_sqrt": (_c_int, [_c_i64, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "flux_far_matmat": (_c_int, [_c_i64, _c_i64, _c_i64, _vp, _vp, _c_i64, _vp, _vp, _vp, _c_int,
+                                 _vp, _vp, _vp, _vp, _vp, _c_int, _vp, _vp]),
+    "flux_permute_rows": (_c_int, [_c_i64, _c_i64, _vp, _c_int, _vp, _vp, _c_int, _vp]),
+    "flux_jacobi_update": (_c_int, [_c_i64, _c_i64, _vp, _vp, _vp, _vp, _c_dbl, _c_int, _vp,
+                                    _c_int, _vp, _vp]),
+    "flux_vec_reduce":(_c_int, [_c_i64, _c_i64, _vp, _vp, _c_int, _c_int, _vp, _vp, _vp]),
     "flux_vec_axpby": (_c_int, [_c_i64, _c_i64, _vp, _vp, _vp, _vp, _c_int, _vp]),
     "flux_vec_rowscale": (_c_int, [_c_i64, _c_i64, _vp, _c_dbl, _c_int, _c_dbl, _vp, _vp,
                                    _c_int, _vp]),

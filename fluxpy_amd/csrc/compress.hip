@@ -1,0 +1,453 @@
+// Hierarchical compression on the GPU: block statistics of a permuted CSR,
+// randomized range sketches of CSR sub-blocks, batched Gram matrices, a
+// batched parallel-order Jacobi eigensolver (FP64) and a batched small GEMM.
+//
+// These replace the per-node work of the reference's tree build
+// (compressed_form_factors.py:285-405): spmat.nnz and nbytes(spmat), and the
+// ARPACK singular values / vectors of linalg.py:8-50.  The singular values
+// of a block A (m x n) come from a Rademacher sketch Y = A W (W: n x l),
+// Z = A^T Y, the Gram M = Y^T Y = V D V^T, the orthonormal basis
+// Q = Y V D^-1/2 (directions with D_i <= 1e-14 max D dropped) and the
+// eigenvalues of K = (Q^T A)(Q^T A)^T = D^-1/2 V^T (Z^T Z) V D^-1/2, whose
+// square roots are the singular values of Q^T A.  All of it is FP64.
+//
+// Block descriptors are int64 rows of 4: r0, r1, c0, c1 (half-open ranges
+// of the permuted index space).  Work buffers are addressed by per-block
+// int64 offsets computed on the host.
+#include "common.h"
+
+namespace flux {
+
+__device__ __forceinline__ int64_t lower_bound_i32(const int32_t *__restrict__ a, int64_t lo,
+                                                   int64_t hi, int64_t key) {
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if ((int64_t)a[mid] < key)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// One wave per block: nnz and number of nonzero rows of A[r0:r1, c0:c1].
+__global__ void k_block_stats(const int64_t *__restrict__ indptr,
+                              const int32_t *__restrict__ indices, int64_t nb,
+                              const int64_t *__restrict__ blk, int64_t *__restrict__ nnz,
+                              int64_t *__restrict__ nzrows) {
+  const int64_t b = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (b >= nb) return;
+  const int64_t r0 = blk[b * 4], r1 = blk[b * 4 + 1], c0 = blk[b * 4 + 2], c1 = blk[b * 4 + 3];
+  int64_t cnt = 0, rows = 0;
+  for (int64_t r = r0 + lane; r < r1; r += 64) {
+    const int64_t s = indptr[r], e = indptr[r + 1];
+    const int64_t lo = lower_bound_i32(indices, s, e, c0);
+    const int64_t hi = lower_bound_i32(indices, lo, e, c1);
+    cnt += hi - lo;
+    rows += (hi > lo) ? 1 : 0;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    cnt += __shfl_down(cnt, off, 64);
+    rows += __shfl_down(rows, off, 64);
+  }
+  if (lane == 0) {
+    nnz[b] = cnt;
+    nzrows[b] = rows;
+  }
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+// Rademacher entry W[j, c] = +-1 from a hash of (seed, global column, c).
+__device__ __forceinline__ double rademacher(uint64_t seed, int64_t j, int64_t c) {
+  const uint64_t h = mix64(seed ^ mix64((uint64_t)j * 0x9E3779B97F4A7C15ull + (uint64_t)c));
+  return (h >> 63) ? -1.0 : 1.0;
+}
+
+// Y_b (m_b x l_b, row-major) = A[r0:r1, c0:c1] X_b, where X_b is the
+// Rademacher matrix (X == NULL) or a stored (n_b x l_b) block.  One wave per
+// (block, row); `rowbase` maps the flat wave id to its block.
+template <typename TA>
+__global__ void __launch_bounds__(256) k_block_spmm(
+    const int64_t *__restrict__ indptr, const int32_t *__restrict__ indices,
+    const TA *__restrict__ data, int64_t nb, const int64_t *__restrict__ blk,
+    const int64_t *__restrict__ ls, const int64_t *__restrict__ rowstart,
+    int64_t total_rows, const double *__restrict__ X, const int64_t *__restrict__ offX,
+    double *__restrict__ Y, const int64_t *__restrict__ offY, uint64_t seed) {
+  const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (w >= total_rows) return;
+  // binary search the block owning flat row w
+  int64_t lo = 0, hi = nb;
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (rowstart[mid] <= w)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  const int64_t b = lo;
+  const int64_t r0 = blk[b * 4], c0 = blk[b * 4 + 2], c1 = blk[b * 4 + 3];
+  const int64_t rl = w - rowstart[b];
+  const int64_t r = r0 + rl;
+  const int64_t l = ls[b];
+  const int64_t s = indptr[r], e = indptr[r + 1];
+  const int64_t k0 = lower_bound_i32(indices, s, e, c0);
+  const int64_t k1 = lower_bound_i32(indices, k0, e, c1);
+  double *yrow = Y + offY[b] + rl * l;
+  for (int64_t cb = 0; cb < l; cb += 64) {
+    const int64_t c = cb + lane;
+    double acc = 0.0;
+    if (c < l) {
+      for (int64_t k = k0; k < k1; ++k) {
+        const int64_t j = indices[k];
+        const double x = X ? X[offX[b] + (j - c0) * l + c] : rademacher(seed, j, c);
+        acc += (double)data[k] * x;
+      }
+      yrow[c] = acc;
+    }
+  }
+}
+
+// G_b = X_b^T X_b (l_b x l_b) for X_b (rows_b x l_b) row-major.  One 16x16
+// tile of G per workgroup (grid.x over tiles, grid.y over blocks).
+__global__ void __launch_bounds__(256) k_batched_gram(
+    const int64_t *__restrict__ rows, const int64_t *__restrict__ ls,
+    const double *__restrict__ X, const int64_t *__restrict__ offX, double *__restrict__ G,
+    const int64_t *__restrict__ offG) {
+  const int64_t b = blockIdx.y;
+  const int64_t l = ls[b];
+  const int64_t nt = (l + 15) / 16;
+  const int64_t tile = blockIdx.x;
+  if (tile >= nt * nt) return;
+  const int64_t ti = tile / nt, tj = tile % nt;
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int64_t gi = ti * 16 + ty, gj = tj * 16 + tx;
+  __shared__ double sa[16][17], sb[16][17];
+  double acc = 0.0;
+  const double *Xb = X + offX[b];
+  const int64_t m = rows[b];
+  for (int64_t r0 = 0; r0 < m; r0 += 16) {
+    // load X[r0 + ty, ti*16 + tx] and X[r0 + ty, tj*16 + tx]
+    const int64_t rr = r0 + ty;
+    const int64_t ca = ti * 16 + tx, cb = tj * 16 + tx;
+    sa[ty][tx] = (rr < m && ca < l) ? Xb[rr * l + ca] : 0.0;
+    sb[ty][tx] = (rr < m && cb < l) ? Xb[rr * l + cb] : 0.0;
+    __syncthreads();
+    for (int q = 0; q < 16; ++q) acc += sa[q][ty] * sb[q][tx];
+    __syncthreads();
+  }
+  if (gi < l && gj < l) G[offG[b] + gi * l + gj] = acc;
+}
+
+// C_b = op(A_b) op(B_b), FP64, row-major, per-block shapes.  desc rows of 12
+// int64: M, N, K, offA, lda, transA, offB, ldb, transB, offC, ldc, beta_is_1.
+__global__ void __launch_bounds__(256) k_batched_gemm(const int64_t *__restrict__ desc,
+                                                      const double *__restrict__ A,
+                                                      const double *__restrict__ B,
+                                                      double *__restrict__ C) {
+  const int64_t *d = desc + (int64_t)blockIdx.y * 12;
+  const int64_t M = d[0], N = d[1], K = d[2];
+  const int64_t ntn = (N + 15) / 16;
+  const int64_t tile = blockIdx.x;
+  if (tile >= ((M + 15) / 16) * ntn) return;
+  const int64_t ti = tile / ntn, tj = tile % ntn;
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const double *Ab = A + d[3];
+  const double *Bb = B + d[6];
+  const int64_t lda = d[4], ldb = d[7];
+  const bool ta = d[5] != 0, tb = d[8] != 0;
+  __shared__ double sa[16][17], sb[16][17];
+  double acc = 0.0;
+  const int64_t gi = ti * 16 + ty, gj = tj * 16 + tx;
+  for (int64_t k0 = 0; k0 < K; k0 += 16) {
+    {  // A tile: rows ti*16.., cols k0..
+      const int64_t i = ti * 16 + ty, k = k0 + tx;
+      sa[ty][tx] = (i < M && k < K) ? (ta ? Ab[k * lda + i] : Ab[i * lda + k]) : 0.0;
+      const int64_t kk = k0 + ty, j = tj * 16 + tx;
+      sb[ty][tx] = (kk < K && j < N) ? (tb ? Bb[j * ldb + kk] : Bb[kk * ldb + j]) : 0.0;
+    }
+    __syncthreads();
+    for (int q = 0; q < 16; ++q) acc += sa[ty][q] * sb[q][tx];
+    __syncthreads();
+  }
+  if (gi < M && gj < N) {
+    double *c = C + d[9] + gi * d[10] + gj;
+    *c = d[11] ? *c + acc : acc;
+  }
+}
+
+// Batched symmetric eigensolver: cyclic Jacobi with the round-robin
+// (tournament) ordering, one workgroup per matrix, matrices in global memory
+// (n x n row-major, overwritten).  Outputs eigenvalues sorted descending and
+// the matching eigenvectors as COLUMNS of Vout (n x n row-major).
+__global__ void __launch_bounds__(256) k_batched_syevj(
+    const int64_t *__restrict__ ns, double *__restrict__ A, const int64_t *__restrict__ offA,
+    double *__restrict__ Vw, double *__restrict__ Vout, double *__restrict__ evals,
+    const int64_t *__restrict__ offE, int max_sweeps) {
+  const int64_t b = blockIdx.x;
+  const int n = (int)ns[b];
+  double *a = A + offA[b];
+  double *v = Vw + offA[b];
+  double *vo = Vout + offA[b];
+  double *ev = evals + offE[b];
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int npad = n + (n & 1);
+  __shared__ double cs[512], sn[512];
+  __shared__ int pp[512], qq[512];
+  __shared__ double red[256];
+  for (int64_t k = tid; k < (int64_t)n * n; k += nt) v[k] = (k / n == k % n) ? 1.0 : 0.0;
+  __syncthreads();
+  if (n <= 1) {
+    if (tid == 0 && n == 1) {
+      ev[0] = a[0];
+      vo[0] = 1.0;
+    }
+    return;
+  }
+  const int half = npad / 2;
+  for (int sweep = 0; sweep < max_sweeps; ++sweep) {
+    // convergence: off-diagonal mass vs diagonal mass
+    double off = 0.0, dia = 0.0;
+    for (int64_t k = tid; k < (int64_t)n * n; k += nt) {
+      const int64_t i = k / n, j = k % n;
+      const double x = a[k];
+      if (i != j)
+        off += x * x;
+      else
+        dia += x * x;
+    }
+    red[tid] = off;
+    __syncthreads();
+    for (int w = nt / 2; w > 0; w >>= 1) {
+      if (tid < w) red[tid] += red[tid + w];
+      __syncthreads();
+    }
+    const double off_all = red[0];
+    __syncthreads();
+    red[tid] = dia;
+    __syncthreads();
+    for (int w = nt / 2; w > 0; w >>= 1) {
+      if (tid < w) red[tid] += red[tid + w];
+      __syncthreads();
+    }
+    const double dia_all = red[0];
+    __syncthreads();
+    if (off_all <= 1e-30 * dia_all || off_all == 0.0) break;
+    for (int step = 0; step < npad - 1; ++step) {
+      // tournament pairing: position 0 fixed, others rotate
+      for (int k = tid; k < half; k += nt) {
+        int x = (k == 0) ? 0 : ((k - 1 + step) % (npad - 1)) + 1;
+        int y = ((npad - 2 - k + step) % (npad - 1)) + 1;
+        int p = min(x, y), q = max(x, y);
+        double c = 1.0, s = 0.0;
+        if (q < n) {
+          const double apq = a[(int64_t)p * n + q];
+          if (apq != 0.0) {
+            const double app = a[(int64_t)p * n + p], aqq = a[(int64_t)q * n + q];
+            const double tau = (aqq - app) / (2.0 * apq);
+            const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+            c = 1.0 / sqrt(1.0 + t * t);
+            s = t * c;
+          }
+        }
+        pp[k] = p;
+        qq[k] = q;
+        cs[k] = c;
+        sn[k] = s;
+      }
+      __syncthreads();
+      // columns: A <- A J, V <- V J
+      for (int64_t w = tid; w < (int64_t)half * n; w += nt) {
+        const int k = (int)(w / n);
+        const int64_t i = w % n;
+        const int p = pp[k], q = qq[k];
+        if (q >= n || sn[k] == 0.0) continue;
+        const double c = cs[k], s = sn[k];
+        const double aip = a[i * n + p], aiq = a[i * n + q];
+        a[i * n + p] = c * aip - s * aiq;
+        a[i * n + q] = s * aip + c * aiq;
+        const double vip = v[i * n + p], viq = v[i * n + q];
+        v[i * n + p] = c * vip - s * viq;
+        v[i * n + q] = s * vip + c * viq;
+      }
+      __syncthreads();
+      // rows: A <- J^T A
+      for (int64_t w = tid; w < (int64_t)half * n; w += nt) {
+        const int k = (int)(w / n);
+        const int64_t j = w % n;
+        const int p = pp[k], q = qq[k];
+        if (q >= n || sn[k] == 0.0) continue;
+        const double c = cs[k], s = sn[k];
+        const double apj = a[(int64_t)p * n + j], aqj = a[(int64_t)q * n + j];
+        a[(int64_t)p * n + j] = c * apj - s * aqj;
+        a[(int64_t)q * n + j] = s * apj + c * aqj;
+      }
+      __syncthreads();
+      for (int k = tid; k < half; k += nt) {
+        const int p = pp[k], q = qq[k];
+        if (q < n && sn[k] != 0.0) {
+          a[(int64_t)p * n + q] = 0.0;
+          a[(int64_t)q * n + p] = 0.0;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // sort descending (rank by counting; ties by index)
+  for (int i = tid; i < n; i += nt) {
+    const double di = a[(int64_t)i * n + i];
+    int rank = 0;
+    for (int j = 0; j < n; ++j) {
+      const double dj = a[(int64_t)j * n + j];
+      rank += (dj > di) || (dj == di && j < i);
+    }
+    ev[rank] = di;
+    for (int r = 0; r < n; ++r) vo[(int64_t)r * n + rank] = v[(int64_t)r * n + i];
+  }
+}
+
+// T_b[:, j] = V_b[:, j] / sqrt(d_j) for d_j > cut * d_0 (d sorted desc),
+// else 0;  kept[b] = number of kept columns.
+__global__ void k_scale_basis(const int64_t *__restrict__ ns, const double *__restrict__ V,
+                              const int64_t *__restrict__ offA, const double *__restrict__ d,
+                              const int64_t *__restrict__ offE, double cut,
+                              double *__restrict__ T, int64_t *__restrict__ kept) {
+  const int64_t b = blockIdx.y;
+  const int64_t n = ns[b];
+  const double d0 = d[offE[b]];
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n * n;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = k % n;
+    const double dj = d[offE[b] + j];
+    const bool keep = d0 > 0.0 && dj > cut * d0;
+    T[offA[b] + k] = keep ? V[offA[b] + k] / sqrt(dj) : 0.0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    int64_t c = 0;
+    for (int64_t j = 0; j < n; ++j) {
+      const double dj = d[offE[b] + j];
+      c += (d0 > 0.0 && dj > cut * d0) ? 1 : 0;
+    }
+    kept[b] = c;
+  }
+}
+
+// Columns of C_b (l x r) scaled by 1/s_j (s = sqrt of eigenvalues, desc).
+__global__ void k_scale_cols_inv_sqrt(const int64_t *__restrict__ dims, double *__restrict__ C,
+                                      const int64_t *__restrict__ offC,
+                                      const double *__restrict__ lam,
+                                      const int64_t *__restrict__ offL) {
+  const int64_t b = blockIdx.y;
+  const int64_t rows = dims[b * 2], cols = dims[b * 2 + 1];
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < rows * cols;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = k % cols;
+    const double l = lam[offL[b] + j];
+    C[offC[b] + k] = l > 0.0 ? C[offC[b] + k] / sqrt(l) : 0.0;
+  }
+}
+
+}  // namespace flux
+
+using namespace flux;
+
+extern "C" int flux_block_stats(const int64_t *indptr, const int32_t *indices, int64_t nb,
+                                const int64_t *blk, int64_t *nnz, int64_t *nzrows,
+                                void *stream) {
+  FLUX_CHECK(nb >= 0, "flux_block_stats: negative count");
+  if (nb == 0) return 0;
+  FLUX_CHECK(indptr && indices && blk && nnz && nzrows, "flux_block_stats: NULL array");
+  hipLaunchKernelGGL(k_block_stats, dim3(ceil_div_u(nb, 4)), dim3(256), 0, as_stream(stream),
+                     indptr, indices, nb, blk, nnz, nzrows);
+  FLUX_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int flux_block_spmm(const int64_t *indptr, const int32_t *indices, const void *data,
+                               int dtype, int64_t nb, const int64_t *blk, const int64_t *ls,
+                               const int64_t *rowstart, int64_t total_rows, const double *X,
+                               const int64_t *offX, double *Y, const int64_t *offY,
+                               uint64_t seed, void *stream) {
+  FLUX_CHECK(nb >= 0 && total_rows >= 0, "flux_block_spmm: bad sizes");
+  if (nb == 0 || total_rows == 0) return 0;
+  FLUX_CHECK(indptr && indices && data && blk && ls && rowstart && Y && offY,
+             "flux_block_spmm: NULL array");
+  FLUX_CHECK(X == nullptr || offX != nullptr, "flux_block_spmm: offX required with X");
+  hipStream_t st = as_stream(stream);
+  const unsigned g = ceil_div_u(total_rows, 4);
+  if (dtype == FLUX_F64)
+    hipLaunchKernelGGL(k_block_spmm<double>, dim3(g), dim3(256), 0, st, indptr, indices,
+                       (const double *)data, nb, blk, ls, rowstart, total_rows, X, offX, Y,
+                       offY, seed);
+  else if (dtype == FLUX_F32)
+    hipLaunchKernelGGL(k_block_spmm<float>, dim3(g), dim3(256), 0, st, indptr, indices,
+                       (const float *)data, nb, blk, ls, rowstart, total_rows, X, offX, Y,
+                       offY, seed);
+  else
+    FLUX_CHECK(false, "flux_block_spmm: dtype must be F32/F64");
+  FLUX_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int flux_batched_gram(int64_t nb, const int64_t *rows, const int64_t *ls,
+                                 int64_t lmax, const double *X, const int64_t *offX, double *G,
+                                 const int64_t *offG, void *stream) {
+  FLUX_CHECK(nb >= 0 && nb <= 65535 && lmax >= 0, "flux_batched_gram: bad sizes");
+  if (nb == 0 || lmax == 0) return 0;
+  const int64_t nt = (lmax + 15) / 16;
+  hipLaunchKernelGGL(k_batched_gram, dim3((unsigned)(nt * nt), (unsigned)nb), dim3(256), 0,
+                     as_stream(stream), rows, ls, X, offX, G, offG);
+  FLUX_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int flux_batched_gemm(int64_t nb, const int64_t *desc, int64_t max_tiles,
+                                 const double *A, const double *B, double *C, void *stream) {
+  FLUX_CHECK(nb >= 0 && nb <= 65535 && max_tiles >= 0, "flux_batched_gemm: bad sizes");
+  if (nb == 0 || max_tiles == 0) return 0;
+  hipLaunchKernelGGL(k_batched_gemm, dim3((unsigned)max_tiles, (unsigned)nb), dim3(256), 0,
+                     as_stream(stream), desc, A, B, C);
+  FLUX_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int flux_batched_syevj(int64_t nb, const int64_t *ns, int64_t nmax, double *A,
+                                  const int64_t *offA, double *Vwork, double *Vout,
+                                  double *evals, const int64_t *offE, int max_sweeps,
+                                  void *stream) {
+  FLUX_CHECK(nb >= 0 && nmax >= 0, "flux_batched_syevj: bad sizes");
+  FLUX_CHECK(nmax <= 1024, "flux_batched_syevj: matrices above 1024 x 1024 are not supported");
+  if (nb == 0) return 0;
+  hipLaunchKernelGGL(k_batched_syevj, dim3((unsigned)nb), dim3(256), 0, as_stream(stream), ns,
+                     A, offA, Vwork, Vout, evals, offE, max_sweeps);
+  FLUX_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int flux_scale_basis(int64_t nb, const int64_t *ns, const double *V,
+                                const int64_t *offA, const double *d, const int64_t *offE,
+                                double cut, double *T, int64_t *kept, void *stream) {
+  FLUX_CHECK(nb >= 0 && nb <= 65535, "flux_scale_basis: bad sizes");
+  if (nb == 0) return 0;
+  hipLaunchKernelGGL(k_scale_basis, dim3(64, (unsigned)nb), dim3(256), 0, as_stream(stream), ns,
+                     V, offA, d, offE, cut, T, kept);
+  FLUX_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int flux_scale_cols_inv_sqrt(int64_t nb, const int64_t *dims, double *C,
+                                        const int64_t *offC, const double *lam,
+                                        const int64_t *offL, void *stream) {
+  FLUX_CHECK(nb >= 0 && nb <= 65535, "flux_scale_cols_inv_sqrt: bad sizes");
+  if (nb == 0) return 0;
+  hipLaunchKernelGGL(k_scale_cols_inv_sqrt, dim3(64, (unsigned)nb), dim3(256), 0,
+                     as_stream(stream), dims, C, offC, lam, offL);
+  FLUX_LAUNCH_CHECK();
+  return 0;
+}

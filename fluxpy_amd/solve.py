@@ -51,6 +51,26 @@ class _CsrOperator:
                   _lib.dtype_code(B.dtype), colmax.data_ptr(), _lib.stream())
 
 
+def _to_internal(op, X):
+    return op.to_internal(X) if hasattr(op, "to_internal") else X
+
+
+def _from_internal(op, X):
+    return op.from_internal(X) if hasattr(op, "from_internal") else X
+
+
+def _rho_internal(op, rho):
+    t = _lib.torch()
+    if isinstance(rho, t.Tensor) and hasattr(op, "to_internal"):
+        return op.to_internal(rho.reshape(-1, 1)).reshape(-1)
+    return rho
+
+
+def apply_original(op, X):
+    """F X with X and the result in facet order, whatever order `op` uses."""
+    return _from_internal(op, op.apply(_to_internal(op, X)))
+
+
 def device_operator(FF):
     """The device operator behind FF (ours, a SciPy sparse matrix or a dense
     array).  Anything else is rejected: there is no host execution path."""
@@ -119,7 +139,8 @@ def solve_radiosity(FF, E, rho=1, albedo_placement='right', method='jacobi', tol
     op = device_operator(FF)
     if op.shape[0] != E2.shape[0] or op.shape[1] != E2.shape[0]:
         raise ValueError(f"shape mismatch: F {op.shape}, E {tuple(Ed.shape)}")
-    rho_d = _prep_rho(rho, E2.shape[0], E2.dtype)
+    rho_d = _rho_internal(op, _prep_rho(rho, E2.shape[0], E2.dtype))
+    E2 = _to_internal(op, E2)
     left = albedo_placement == 'left'
     if method == 'jacobi':
         if tol is None:
@@ -132,8 +153,11 @@ def solve_radiosity(FF, E, rho=1, albedo_placement='right', method='jacobi', tol
             tol = np.finfo(np.dtype(str(E2.dtype).replace("torch.", ""))).resolution * emax
         B, niter = _cg(op, E2, rho_d, left, tol, maxiter or 100000)
     else:
+        if x0 is not None:
+            x0 = _to_internal(op, _lib.to_dev(x0).reshape(E2.shape).to(E2.dtype))
         B, niter = _gmres(op, E2, rho_d, left, 1e-12 if tol is None else tol,
                           maxiter or 10000, restart, x0)
+    B = _from_internal(op, B)
     B = B.reshape(-1) if vec else B
     return (B.cpu().numpy() if host else B), niter
 
@@ -288,8 +312,10 @@ def solve_radiosity_refined(FF_lo, FF_hi, E, rho=1, albedo_placement='right', rt
     op_lo = device_operator(FF_lo)
     op_hi = device_operator(FF_hi)
     left = albedo_placement == 'left'
-    rho64 = _prep_rho(rho, n, t.float64)
-    rho32 = rho64.to(t.float32) if isinstance(rho64, t.Tensor) else rho64
+    # both operators must share one ordering (they are views of one matrix)
+    rho64 = _rho_internal(op_hi, _prep_rho(rho, n, t.float64))
+    E2 = _to_internal(op_hi, E2)
+    rho32 = _kernels.convert(rho64, t.float32) if isinstance(rho64, t.Tensor) else rho64
     vo = _kernels.VecOps(k)
     emax = vo.maxabs(E2).cpu().numpy()
     thresh = rtol * np.where(emax > 0, emax, 1.0)
@@ -328,6 +354,7 @@ def solve_radiosity_refined(FF_lo, FF_hi, E, rho=1, albedo_placement='right', rt
             warnings.warn(f"refinement stopped after {rounds} rounds (residual "
                           f"{hist[-1]:.2e} > {rtol:.1e})", RuntimeWarning)
             break
+    B = _from_internal(op_hi, B)
     out = B.reshape(-1) if vec else B
     out = out.cpu().numpy() if host else out
     if return_info:

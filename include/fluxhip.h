@@ -200,6 +200,88 @@ int flux_csr_jacobi_step(int64_t m, const int64_t *indptr,
                          void *B1, int xdtype, double *colmax, void *stream);
 
 /* ---------------------------------------------------------------------- */
+/* Hierarchical compression (compressed_form_factors.py:285-405 and        */
+/* linalg.py:8-50).  Block descriptors are int64 rows (r0, r1, c0, c1) of  */
+/* half-open ranges in the cluster (permuted) order; batched work buffers  */
+/* are addressed by per-block int64 element offsets.  All FP64.            */
+/* ---------------------------------------------------------------------- */
+
+/* nnz and number of nonzero rows of every block A[r0:r1, c0:c1] of a CSR
+ * with sorted rows (spmat.nnz and the zero-row count of
+ * compressed_form_factors.py:312 / linalg.py:21-24). */
+int flux_block_stats(const int64_t *indptr, const int32_t *indices,
+                     int64_t nb, const int64_t *blk, int64_t *nnz,
+                     int64_t *nzrows, void *stream);
+
+/* Y_b = A[r0:r1, c0:c1] X_b (m_b x l_b row-major at Y + offY[b]).  X == NULL
+ * uses the Rademacher matrix W[j, c] = +-1 hashed from (seed, j, c);
+ * otherwise X_b is (c1-c0) x l_b at X + offX[b].  rowstart[b] = sum of the
+ * row counts of blocks before b, total_rows = the sum over all blocks. */
+int flux_block_spmm(const int64_t *indptr, const int32_t *indices,
+                    const void *data, int dtype, int64_t nb, const int64_t *blk,
+                    const int64_t *ls, const int64_t *rowstart,
+                    int64_t total_rows, const double *X, const int64_t *offX,
+                    double *Y, const int64_t *offY, uint64_t seed, void *stream);
+
+/* G_b = X_b^T X_b (l_b x l_b) for X_b (rows[b] x l_b); lmax = max l_b. */
+int flux_batched_gram(int64_t nb, const int64_t *rows, const int64_t *ls,
+                      int64_t lmax, const double *X, const int64_t *offX,
+                      double *G, const int64_t *offG, void *stream);
+
+/* C_b = op(A_b) op(B_b) (or C_b += when the 12th field is 1).  desc: nb
+ * rows of 12 int64: M, N, K, offA, lda, transA, offB, ldb, transB, offC,
+ * ldc, accumulate.  max_tiles = max over b of ceil(M/16) ceil(N/16). */
+int flux_batched_gemm(int64_t nb, const int64_t *desc, int64_t max_tiles,
+                      const double *A, const double *B, double *C,
+                      void *stream);
+
+/* Symmetric eigen-decomposition of nb matrices (n_b <= 1024, row-major at
+ * A + offA[b], destroyed) by parallel-order cyclic Jacobi, one workgroup
+ * per matrix: eigenvalues descending at evals + offE[b], eigenvectors as
+ * the columns of Vout + offA[b].  Vwork: same size as A. */
+int flux_batched_syevj(int64_t nb, const int64_t *ns, int64_t nmax, double *A,
+                       const int64_t *offA, double *Vwork, double *Vout,
+                       double *evals, const int64_t *offE, int max_sweeps,
+                       void *stream);
+
+/* T_b[:, j] = V_b[:, j] / sqrt(d_j) where d_j > cut * d_0, else 0;
+ * kept[b] = number of kept columns. */
+int flux_scale_basis(int64_t nb, const int64_t *ns, const double *V,
+                     const int64_t *offA, const double *d, const int64_t *offE,
+                     double cut, double *T, int64_t *kept, void *stream);
+
+/* Columns j of C_b (dims[b] = rows, cols) divided by sqrt(lam_j). */
+int flux_scale_cols_inv_sqrt(int64_t nb, const int64_t *dims, double *C,
+                             const int64_t *offC, const double *lam,
+                             const int64_t *offL, void *stream);
+
+/* Far-field part of the H-matvec (SVD leaves, compressed_form_factors.py:
+ * 247-251): Y[r0:r1] += U_b (S_b o (Vt_b X[c0:c1])) for every far block b.
+ * desc: nfar rows of 8 int64 (r0, r1, c0, c1, rank, offU, offVt, offS);
+ * tstart[b] = sum of rank*nrhs before b, total = the full sum; T: work of
+ * `total` elements in the accumulation type (FP64 if X or the factors are
+ * FP64, else FP32).  The far blocks covering row i are
+ * cblk[cptr[rowcl[i]] : cptr[rowcl[i]+1]].  Factor dtype F64/F32/BF16/F16. */
+int flux_far_matmat(int64_t n, int64_t nrhs, int64_t nfar, const int64_t *desc,
+                    const int64_t *tstart, int64_t total, const void *U,
+                    const void *S, const void *Vt, int fdtype,
+                    const int32_t *rowcl, const int64_t *cptr,
+                    const int32_t *cblk, const void *X, void *Y, int xdtype,
+                    void *T, void *stream);
+
+/* Gather (scatter == 0: out[k] = X[perm[k]]) or scatter (out[perm[k]] =
+ * X[k]) of the rows of an n x nrhs block: the cluster ordering. */
+int flux_permute_rows(int64_t n, int64_t nrhs, const int64_t *perm,
+                      int scatter, const void *X, void *out, int dtype,
+                      void *stream);
+
+/* B1 = E + Y (right) or E + rho o Y (left) and colmax[c] = max |B1 - B|:
+ * the update half of a Jacobi step whose product Y was formed separately. */
+int flux_jacobi_update(int64_t n, int64_t nrhs, const void *E, const void *Y,
+                       const void *B, const void *rho, double rho_s, int left,
+                       void *B1, int dtype, double *colmax, void *stream);
+
+/* ---------------------------------------------------------------------- */
 /* Column-blocked vector kernels (solver building blocks; blocks are      */
 /* n x nrhs row-major, reductions per column in FP64, deterministic)       */
 /* ---------------------------------------------------------------------- */
