@@ -5,16 +5,20 @@ plus visibility Mrays/s.
 Workload ("compressed, multi-RHS" configuration): diamond-square terrain
 (Hurst 0.8, 257^2 base grid, 30 seeded craters, relief 0.1 of the width)
 sampled onto 145^2 vertices -> 41,472 facets, FP32 mesh; per-facet albedo
-0.12 +- 0.02; 64 suns (elevation U[1, 15] deg, azimuth U[0, 360) deg).
+0.12 +- 0.02; 64 suns (elevation U[1, 15] deg, azimuth U[0, 360) deg);
+CompressedFormFactorMatrix with tol 1e-6 and min_size 4096 (64 x 64
+leaves).
 
 One step = one batched solve of all 64 suns to <= 1e-6 relative error (the
-refinement stops at max|r_c| <= 1e-7 max|E_c|).  The plain FP64 path solves
-one sun at a time with FP64 values and vectors, reference Jacobi stopping
-rule (solve.py:6-8), no batching.  Inputs are resident in HBM before the
-timed region; assembly is timed and reported separately.
+refinement stops at max|r_c| <= 1e-7 max|E_c|): BF16 near-field panels on
+MFMA for the inner sweeps, FP32 master panels with FP64 MFMA for the
+residual.  The plain FP64 path solves one sun at a time with FP64 values
+and vectors on the same compressed matrix, reference Jacobi stopping rule
+(solve.py:6-8), no batching.  Inputs are resident in HBM before the timed
+region; visibility, assembly and compression are timed separately.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid 145]
-Prints one JSON line.
+Prints one JSON line on stdout (progress on stderr).
 """
 import argparse
 import json
@@ -43,6 +47,9 @@ def main():
     ap.add_argument("--grid", type=int, default=145)
     ap.add_argument("--suns", type=int, default=64)
     ap.add_argument("--fp64-suns", type=int, default=4)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--min-size", type=int, default=4096)
+    ap.add_argument("--inner", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -52,13 +59,15 @@ def main():
 
     import torch
     from fluxpy_amd import _kernels, _lib, meshes
-    from fluxpy_amd.form_factors import FormFactorMatrix
+    from fluxpy_amd._panel import PanelMatrix, PanelRefinedSolver
+    from fluxpy_amd.compressed_form_factors import CompressedFormFactorMatrix
     from fluxpy_amd.shape import CgalTrimeshShapeModel
     from fluxpy_amd.solve import solve_radiosity, solve_radiosity_refined
 
     torch.cuda.set_device(0)
     dev = torch.device("cuda", 0)
     st = _lib.stream()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
 
     # ---------------- workload ----------------
     V, F = meshes.fractal_terrain(grid=args.grid)
@@ -79,7 +88,6 @@ def main():
     _lib.call("flux_ff_candidates", sm._scene.handle, P.data_ptr(), N.data_ptr(), _lib.F32,
               I.data_ptr(), nf, I.data_ptr(), nf, 1e-5, mask.data_ptr(), cnt.data_ptr(), st)
     rays = int(cnt.sum().item())
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     ev0.record()
     _lib.call("flux_ff_mask", sm._scene.handle, P.data_ptr(), N.data_ptr(), _lib.F32,
@@ -90,32 +98,34 @@ def main():
     vis_ms = ev0.elapsed_time(ev1)
     mrays = rays / (vis_ms * 1e-3) / 1e6
     del mask
-    log(f"[bench] rays={rays} vis kernel {vis_ms:.1f} ms -> {mrays:.0f} Mrays/s")
+    log(f"[bench] rays={rays} visibility kernel {vis_ms:.1f} ms -> {mrays:.0f} Mrays/s")
 
-    # ---------------- assembly ----------------
+    # ---------------- compressed matrix ----------------
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    FFm = FormFactorMatrix.from_shape_model(sm)
-    csr32 = FFm.device_csr
+    H = CompressedFormFactorMatrix(sm, tol=args.tol, min_size=args.min_size)
     torch.cuda.synchronize()
-    t_asm = time.perf_counter() - t0
-    nnz = csr32.nnz
+    t_build = time.perf_counter() - t0
+    bs = H.build_stats
+    op64 = H.device_operator("fp64")
+    near = H._near
+    nnz = near.nnz
+    log(f"[bench] compressed build {t_build:.2f} s (assembly {bs['assembly_s']:.2f} s, "
+        f"tree {bs['tree_s']:.2f} s) nbytes={H.nbytes} leaves={bs['leaf_counts']} "
+        f"svd_leaves={bs['svd_leaves']} near nnz={nnz}")
+
     E32 = sm.get_direct_irradiance_multi(1365.0, D, device=True)
     E64 = _kernels.convert(E32, torch.float64)
-    csr_bf16 = csr32.astype(torch.bfloat16)
-    csr64 = csr32.astype(torch.float64)
-    torch.cuda.synchronize()
-    log(f"[bench] assembly {t_asm:.2f} s nnz={nnz} density={nnz / nf / nf:.4f}")
+    rho_d = _lib.to_dev(rho)
 
     # ---------------- plain FP64 path (one sun at a time) ----------------
     ns64 = min(args.fp64_suns, args.suns)
-    B64 = []
-    solve_radiosity(csr64, E64[:, 0].contiguous(), rho, 'right')  # warm-up
+    solve_radiosity(op64, E64[:, 0].contiguous(), rho, 'right')  # warm-up
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    it64 = []
+    B64, it64 = [], []
     for s in range(ns64):
-        b, it = solve_radiosity(csr64, E64[:, s].contiguous(), rho, 'right')
+        b, it = solve_radiosity(op64, E64[:, s].contiguous(), rho, 'right')
         B64.append(b)
         it64.append(it)
     torch.cuda.synchronize()
@@ -125,9 +135,26 @@ def main():
         f"-> {fp64_sps:.1f} solves/s")
 
     # ---------------- BF16 + refinement, batched ----------------
-    def step():
-        return solve_radiosity_refined(csr_bf16, csr32, E64, rho, 'right', rtol=1e-7,
-                                       inner_iters=4, return_info=True)
+    pure_near = op64.nfar == 0
+    if pure_near:
+        P32 = PanelMatrix(near, "fp32")
+        Pbf = P32.sibling(near, "bf16")
+        Ei = op64.to_internal(E64)
+        rho_i = op64.to_internal(rho_d.reshape(-1, 1)).reshape(-1).contiguous()
+        solver = PanelRefinedSolver(Pbf, P32, nf, args.suns, rho_i)
+        log(f"[bench] panels: fill {P32.fill:.3f}, bf16 {Pbf.nbytes / 1e6:.0f} MB, "
+            f"fp32 {P32.nbytes / 1e6:.0f} MB")
+
+        def step():
+            Bi, info = solver.solve(Ei, rtol=1e-7, inner_iters=args.inner)
+            return op64.from_internal(Bi), info
+    else:
+        opbf = H.device_operator("bf16")
+        op32 = H.device_operator("fp32")
+
+        def step():
+            return solve_radiosity_refined(opbf, op32, E64, rho, rtol=1e-7,
+                                           inner_iters=args.inner, return_info=True)
 
     for _ in range(args.warmup):
         B, info = step()
@@ -143,33 +170,39 @@ def main():
         f"rounds={info['rounds']} max rel err vs FP64 {max(errs):.2e}")
 
     # ---------------- roofline of the dominant kernel ----------------
-    # the BF16 inner Jacobi sweep: one CSR pass with 64 right-hand sides
-    from fluxpy_amd.solve import _CsrOperator
-    op = _CsrOperator(csr_bf16)
-    R32 = _kernels.convert(E64, torch.float32)
-    Dk = R32.clone()
-    D1 = torch.empty_like(Dk)
-    cm = torch.empty(args.suns, dtype=torch.float64, device=dev)
-    rho32 = _lib.to_dev(rho.astype(np.float32))
-    reps = 20
-    for _ in range(3):
-        op.jacobi_step(R32, Dk, rho32, False, D1, cm)
-    torch.cuda.synchronize()
-    ev0.record()
-    for _ in range(reps):
-        op.jacobi_step(R32, Dk, rho32, False, D1, cm)
-    ev1.record()
-    torch.cuda.synchronize()
-    k_ms = ev0.elapsed_time(ev1) / reps
     k = args.suns
-    alg_bytes = nnz * (2 + 4) + 8 * (nf + 1) + k * nf * (4 + 4) + k * nf * 4 + nf * 4
+    reps = 20
+    if pure_near:
+        Xb = torch.zeros((nf, k), dtype=torch.bfloat16, device=dev)
+        Xn = torch.empty_like(Xb)
+        out = torch.empty((nf, k), dtype=torch.float32, device=dev)
+        base = torch.zeros_like(out)
+        for _ in range(3):
+            Pbf.mm_bf16(Xb, out, base=base, rho=solver.rho32, xnext=Xn)
+        torch.cuda.synchronize()
+        ev0.record()
+        for _ in range(reps):
+            Pbf.mm_bf16(Xb, out, base=base, rho=solver.rho32, xnext=Xn)
+        ev1.record()
+        torch.cuda.synchronize()
+        kname = "k_panel_mm_bf16"
+    else:
+        ev0.record()
+        for _ in range(reps):
+            opbf.apply(_kernels.convert(E64, torch.float32))
+        ev1.record()
+        torch.cuda.synchronize()
+        kname = "k_csr_rows<bf16> + far field"
+    k_ms = ev0.elapsed_time(ev1) / reps
+    # algorithmic H-matvec bytes: near nnz (bf16 value + int32 index), indptr,
+    # x in (bf16) and y out (fp32), plus the fused epilogue (base in, x next out)
+    alg_bytes = nnz * (2 + 4) + 4 * (nf + 1) + k * nf * (2 + 4) + k * nf * (4 + 2)
     achieved = alg_bytes / (k_ms * 1e-3) / 1e9
-    log(f"[bench] bf16 jacobi sweep {k_ms * 1e3:.1f} us, {achieved:.0f} GB/s algorithmic")
+    log(f"[bench] {kname}: {k_ms * 1e3:.1f} us/launch, {achieved:.0f} GB/s algorithmic")
 
-    # ---------------- CPU baseline (oracle port, bounded sample) ----------------
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(sm, FFm, E64, rho, V, F)
+        cpu = cpu_baseline(sm, H, E64, rho, V, F)
 
     out = {
         "metric": METRIC,
@@ -185,10 +218,12 @@ def main():
         "dtype": "bf16",
         "data": "synthetic (seeded fractal terrain, seeded suns and albedo)",
         "config": {
-            "workload": "fractal terrain 145^2 vertices (41,472 facets), 64 suns, "
-                        "BF16 operator + FP64-residual refinement to 1e-6",
-            "facets": nf, "suns": args.suns, "nnz": nnz,
-            "operator": "near-field CSR (sparse form factors, permuted order)",
+            "workload": "fractal terrain 145^2 vertices (41,472 facets), 64 suns, compressed "
+                        "F (tol 1e-6, min_size 4096), BF16 panels + FP64-residual refinement "
+                        "to 1e-6",
+            "facets": nf, "suns": args.suns, "near_nnz": nnz, "tol": args.tol,
+            "min_size": args.min_size, "compressed_nbytes": int(H.nbytes),
+            "leaves": bs["leaf_counts"], "svd_leaves": bs["svd_leaves"],
         },
         "fp64_solves_per_s": round(fp64_sps, 2),
         "speedup_vs_fp64": round(sps / fp64_sps, 2),
@@ -196,26 +231,27 @@ def main():
         "refine_rounds": info["rounds"],
         "mrays_per_s": round(mrays, 1),
         "rays": rays,
-        "assembly_s": round(t_asm, 3),
+        "compress_s": round(t_build, 3),
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "k_csr_rows<bf16,float,float,JACOBI_RIGHT>",
-            "kernel_us": round(k_ms * 1e3, 2), "algorithmic_bytes": alg_bytes,
+            "kernel": kname, "kernel_us": round(k_ms * 1e3, 2), "algorithmic_bytes": alg_bytes,
         },
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 
 
-def cpu_baseline(sm, FFm, E64, rho, V, F):
+def cpu_baseline(sm, H, E64, rho, V, F):
     """The oracle (CPU port) on a bounded sample: one sun's FP64 Jacobi with
-    SciPy CSR (single core) and the C BVH ray caster on a row sample."""
+    SciPy CSR (one core) on the same near field, and the C BVH ray caster on
+    a row sample (all cores)."""
     from oracle import oracle as O
-    sp = FFm.tocsr().astype(np.float64)
-    E = E64[:, 0].cpu().numpy()
+    sp = H._near.to_scipy().astype(np.float64)
+    perm = H._ctree.perm
+    E = E64[:, 0].cpu().numpy()[perm]
     t0 = time.perf_counter()
-    O.solve_radiosity_jacobi(sp, E, rho, "right")
+    O.solve_radiosity_jacobi(sp, E, rho[perm], "right")
     t_cpu = time.perf_counter() - t0
     S = O.OracleScene(V, F)
     rng = np.random.default_rng(7)
@@ -227,9 +263,9 @@ def cpu_baseline(sm, FFm, E64, rho, V, F):
     cores = os.cpu_count()
     log(f"[bench] CPU: FP64 Jacobi {t_cpu:.2f} s/sun, {nrows} FF rows in {t_rows:.2f} s")
     return {"value": round(1.0 / t_cpu, 4), "unit": "solves/s", "cores": 1, "kind": "port",
-            "sample": "1 sun, FP64 Jacobi (reference stopping rule) with SciPy CSR on the same "
-                      "F; plus FF rows %d/%d with the C BVH on %d threads in %.2f s (%d nnz)"
-                      % (nrows, sm.num_faces, cores, t_rows, csr.nnz)}
+            "sample": "1 sun, FP64 Jacobi (reference stopping rule) with SciPy CSR on the "
+                      "same near field; FF rows %d/%d with the C BVH on %d threads took "
+                      "%.2f s (%d nnz)" % (nrows, sm.num_faces, cores, t_rows, csr.nnz)}
 
 
 if __name__ == "__main__":

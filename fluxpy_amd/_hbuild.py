@@ -125,12 +125,13 @@ class _SvdBatch:
         d_offY, d_offZ, d_offG, d_offE = _i64(offY[:-1]), _i64(offZ[:-1]), _i64(offG[:-1]), \
             _i64(offE[:-1])
         code = _lib.dtype_code(F.data.dtype)
+        d_rs, d_rsT = _i64(rowstart), _i64(rowstartT)
         _lib.call("flux_block_spmm", F.indptr.data_ptr(), F.indices.data_ptr(), F.data.data_ptr(),
-                  code, nb, d_blk.data_ptr(), d_ls.data_ptr(), _i64(rowstart).data_ptr(),
+                  code, nb, d_blk.data_ptr(), d_ls.data_ptr(), d_rs.data_ptr(),
                   int(m.sum()), None, None, self.Y.data_ptr(), d_offY.data_ptr(), seed, st)
         _lib.call("flux_block_spmm", FT.indptr.data_ptr(), FT.indices.data_ptr(),
                   FT.data.data_ptr(), code, nb, d_blkT.data_ptr(), d_ls.data_ptr(),
-                  _i64(rowstartT).data_ptr(), int(n.sum()), self.Y.data_ptr(),
+                  d_rsT.data_ptr(), int(n.sum()), self.Y.data_ptr(),
                   d_offY.data_ptr(), self.Z.data_ptr(), d_offZ.data_ptr(), seed, st)
         lmax = int(l.max())
         d_m, d_n = _i64(m), _i64(n)
@@ -149,10 +150,12 @@ class _SvdBatch:
         desc = np.array([[li, li, li, oG, li, 0, oG, li, 0, oG, li, 0]
                          for li, oG in zip(l, offG[:-1])], dtype=np.int64)
         tiles = int(((l + 15) // 16).max() ** 2)
-        _lib.call("flux_batched_gemm", nb, _i64(desc).data_ptr(), tiles, G.data_ptr(),
+        d_desc1 = _i64(desc)
+        _lib.call("flux_batched_gemm", nb, d_desc1.data_ptr(), tiles, G.data_ptr(),
                   self.Tm.data_ptr(), H1.data_ptr(), st)
         desc[:, 5] = 1
-        _lib.call("flux_batched_gemm", nb, _i64(desc).data_ptr(), tiles, self.Tm.data_ptr(),
+        d_desc2 = _i64(desc)
+        _lib.call("flux_batched_gemm", nb, d_desc2.data_ptr(), tiles, self.Tm.data_ptr(),
                   H1.data_ptr(), G.data_ptr(), st)
         self.Xe = Vm
         self.lam = d
@@ -190,25 +193,29 @@ class _SvdBatch:
         desc = np.array([[li, r, li, g, li, 0, g, li, 0, c, r, 0]
                          for li, r, g, c in zip(l, rs, oG, offC[:-1])], dtype=np.int64)
         tiles = int((((l + 15) // 16) * ((rs + 15) // 16)).max())
-        _lib.call("flux_batched_gemm", np_, _i64(desc).data_ptr(), tiles, self.Tm.data_ptr(),
+        d_desc1 = _i64(desc)
+        _lib.call("flux_batched_gemm", np_, d_desc1.data_ptr(), tiles, self.Tm.data_ptr(),
                   self.Xe.data_ptr(), C.data_ptr(), st)
         C2.copy_(C)
         dims = np.stack([l, rs], 1).astype(np.int64)
-        _lib.call("flux_scale_cols_inv_sqrt", np_, _i64(dims).data_ptr(), C2.data_ptr(),
-                  _i64(offC[:-1]).data_ptr(), self.lam.data_ptr(), _i64(oE).data_ptr(), st)
+        d_dims, d_offC, d_oE = _i64(dims), _i64(offC[:-1]), _i64(oE)
+        _lib.call("flux_scale_cols_inv_sqrt", np_, d_dims.data_ptr(), C2.data_ptr(),
+                  d_offC.data_ptr(), self.lam.data_ptr(), d_oE.data_ptr(), st)
         # U = Y C  (m x r)
         desc = np.array([[mi, r, li, y, li, 0, c, r, 0, u, r, 0]
                          for mi, r, li, y, c, u in zip(m, rs, l, self.offY[bs], offC[:-1],
                                                        offU[:-1])], dtype=np.int64)
         tiles = int((((m + 15) // 16) * ((rs + 15) // 16)).max())
-        _lib.call("flux_batched_gemm", np_, _i64(desc).data_ptr(), tiles, self.Y.data_ptr(),
+        d_desc2 = _i64(desc)
+        _lib.call("flux_batched_gemm", np_, d_desc2.data_ptr(), tiles, self.Y.data_ptr(),
                   C.data_ptr(), U.data_ptr(), st)
         # Vt = C2^T Z^T  (r x n)
         desc = np.array([[r, ni, li, c, r, 1, z, li, 1, v, ni, 0]
                          for r, ni, li, c, z, v in zip(rs, n, l, offC[:-1], self.offZ[bs],
                                                        offV[:-1])], dtype=np.int64)
         tiles = int((((rs + 15) // 16) * ((n + 15) // 16)).max())
-        _lib.call("flux_batched_gemm", np_, _i64(desc).data_ptr(), tiles, C2.data_ptr(),
+        d_desc3 = _i64(desc)
+        _lib.call("flux_batched_gemm", np_, d_desc3.data_ptr(), tiles, C2.data_ptr(),
                   self.Z.data_ptr(), Vt.data_ptr(), st)
         lam = self.lam
         for q in range(np_):

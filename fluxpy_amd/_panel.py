@@ -1,0 +1,169 @@
+"""Row-block panel form of a square near-field CSR (see csrc/panel.hip) and
+the batched mixed-precision radiosity solve built on it.
+
+The panels hold the same nonzeros as the CSR (zeros fill the rest), so a
+panel product equals the CSR product up to summation order.  BF16 panels
+serve the inner sweeps; FP32 panels (the master values) serve the FP64
+residual of the refinement.
+"""
+import numpy as np
+
+from . import _kernels
+from . import _lib
+
+
+class PanelMatrix:
+    """Panels of a square CSR in one value precision ('bf16' or 'fp32')."""
+
+    ROWS = 64
+
+    def __init__(self, csr, values="bf16", colptr=None, cols=None):
+        t = _lib.require_gpu()
+        m, n = csr.shape
+        self.shape = (m, n)
+        self.values = values
+        dev = csr.data.device
+        st = _lib.stream()
+        nblk = (m + self.ROWS - 1) // self.ROWS
+        if colptr is None:
+            counts = t.empty(max(nblk, 1), dtype=t.int64, device=dev)
+            _lib.call("flux_panel_count", m, n, csr.indptr.data_ptr(), csr.indices.data_ptr(),
+                      counts.data_ptr(), st)
+            colptr = t.empty(nblk + 1, dtype=t.int64, device=dev)
+            total = _lib.ctypes.c_int64()
+            _lib.call("flux_scan_i64", counts.data_ptr(), nblk, colptr.data_ptr(),
+                      _lib.ctypes.byref(total), st)
+            total_cols = total.value
+            cols = t.empty(max(total_cols, 1), dtype=t.int32, device=dev)
+            fill_cols = True
+        else:
+            total_cols = int(colptr[-1].item())
+            fill_cols = False
+        self.colptr, self.cols = colptr, cols
+        self.total_cols = total_cols
+        pdt = {"bf16": t.bfloat16, "fp32": t.float32}[values]
+        self.panel = t.empty(max(total_cols * self.ROWS, 1), dtype=pdt, device=dev)
+        scratch_cols = cols if fill_cols else t.empty_like(cols)
+        _lib.call("flux_panel_fill", m, n, csr.indptr.data_ptr(), csr.indices.data_ptr(),
+                  csr.data.data_ptr(), _lib.dtype_code(csr.data.dtype), colptr.data_ptr(),
+                  total_cols, scratch_cols.data_ptr(), self.panel.data_ptr(),
+                  _lib.dtype_code(pdt), st)
+        self.nnz = csr.nnz
+
+    def sibling(self, csr, values):
+        """Panels of the same pattern in another precision (shares cols)."""
+        return PanelMatrix(csr, values, colptr=self.colptr, cols=self.cols)
+
+    @property
+    def nbytes(self):
+        return (self.panel.numel() * self.panel.element_size() + self.cols.numel() * 4
+                + self.colptr.numel() * 8)
+
+    @property
+    def fill(self):
+        return self.nnz / max(1, self.total_cols * self.ROWS)
+
+    # -- products -------------------------------------------------------------
+
+    def mm_bf16(self, Xb, out, base=None, rho=None, rho_s=1.0, left=False, xnext=None,
+                prev=None, colmax=None):
+        """out = base + s (P Xb) with the Jacobi epilogue (see panel.hip)."""
+        k = Xb.shape[1]
+        _lib.call("flux_panel_mm_bf16", self.shape[0], self.colptr.data_ptr(),
+                  self.cols.data_ptr(), self.panel.data_ptr(), Xb.data_ptr(), k, _lib.ptr(base),
+                  _lib.ptr(rho), float(rho_s), int(left), out.data_ptr(), _lib.ptr(xnext),
+                  _lib.ptr(prev), _lib.ptr(colmax), _lib.stream())
+        return out
+
+    def mm_f64(self, X, out, E=None, B=None, rho=None, rho_s=1.0, left=False, colmax=None):
+        """out = P X, or with E: out = E + s (P X) - B and colmax = max|out|."""
+        k = X.shape[1]
+        _lib.call("flux_panel_mm_f64", self.shape[0], self.colptr.data_ptr(),
+                  self.cols.data_ptr(), self.panel.data_ptr(), X.data_ptr(), k, _lib.ptr(E),
+                  _lib.ptr(B), _lib.ptr(rho), float(rho_s), int(left), out.data_ptr(),
+                  _lib.ptr(colmax), _lib.stream())
+        return out
+
+
+def to_bf16_scaled(x32, rho=None, rho_s=1.0, scale=True, out=None):
+    t = _lib.torch()
+    out = t.empty(x32.shape, dtype=t.bfloat16, device=x32.device) if out is None else out
+    n = x32.shape[0]
+    k = x32.shape[1] if x32.dim() == 2 else 1
+    _lib.call("flux_scale_to_bf16", n, k, x32.data_ptr(), _lib.ptr(rho), float(rho_s),
+              int(scale), out.data_ptr(), _lib.stream())
+    return out
+
+
+class PanelRefinedSolver:
+    """Batched radiosity solve of (I - F diag(rho)) B = E ('right') or
+    (I - diag(rho) F) B = E ('left') by iterative refinement:
+
+    * inner: `inner_iters` Jacobi sweeps of the correction equation with the
+      BF16 panels, FP32 vectors, FP32 accumulation, no host round trip;
+    * residual: r = E - (I - F rho) B with the FP32 master panels, FP64
+      vectors and FP64 accumulation (MFMA FP64);
+    * stop when max|r_c| <= rtol max|E_c| for every column c.
+
+    All vectors live in the panels' (cluster) order.
+    """
+
+    def __init__(self, P_lo, P_hi, n, k, rho, left=False):
+        t = _lib.require_gpu()
+        dev = P_lo.panel.device
+        self.P_lo, self.P_hi = P_lo, P_hi
+        self.n, self.k, self.left = n, k, left
+        if isinstance(rho, t.Tensor):
+            self.rho64 = rho.to(t.float64).contiguous()
+            self.rho32 = _kernels.convert(self.rho64, t.float32)
+            self.rs = 1.0
+        else:
+            self.rho64 = self.rho32 = None
+            self.rs = float(rho)
+        f32, f64 = t.float32, t.float64
+        self.B = t.empty((n, k), dtype=f64, device=dev)
+        self.R = t.empty((n, k), dtype=f64, device=dev)
+        self.Xr = t.empty((n, k), dtype=f64, device=dev)
+        self.R32 = t.empty((n, k), dtype=f32, device=dev)
+        self.D = [t.empty((n, k), dtype=f32, device=dev) for _ in range(2)]
+        self.Xb = [t.empty((n, k), dtype=t.bfloat16, device=dev) for _ in range(2)]
+        self.colmax = t.empty(k, dtype=f64, device=dev)
+        self.vo = _kernels.VecOps(k)
+
+    def solve(self, E, rtol=1e-7, inner_iters=4, max_rounds=30):
+        """E: n x k FP64 device tensor in panel order.  Returns (B, info)."""
+        t = _lib.torch()
+        emax = self.vo.maxabs(E).cpu().numpy()
+        thresh = rtol * np.where(emax > 0, emax, 1.0)
+        B, R = self.B, self.R
+        B.zero_()
+        R.copy_(E)
+        rounds = 0
+        hist = []
+        while True:
+            rounds += 1
+            _lib.call("flux_vec_convert", R.numel(), R.data_ptr(), _lib.F64, self.R32.data_ptr(),
+                      _lib.F32, _lib.stream())
+            # D0 = R32; X0 = bf16(rho D0) (right) or bf16(D0) (left)
+            to_bf16_scaled(self.R32, self.rho32, self.rs, scale=not self.left, out=self.Xb[0])
+            cur = 0
+            for _ in range(inner_iters):
+                self.P_lo.mm_bf16(self.Xb[cur], self.D[cur], base=self.R32, rho=self.rho32,
+                                  rho_s=self.rs, left=self.left, xnext=self.Xb[1 - cur])
+                cur = 1 - cur
+            D = self.D[1 - cur]
+            _kernels.axpby(None, _kernels.convert(D, t.float64), None, B)
+            # residual R = E + F(rho B) - B  (right) / E + rho F B - B (left)
+            if self.left:
+                Xr = B
+            else:
+                Xr = _kernels.rowscale(self.rho64 if self.rho64 is not None else self.rs, B,
+                                       out=self.Xr)
+            self.P_hi.mm_f64(Xr, R, E=E, B=B, rho=self.rho64, rho_s=self.rs, left=self.left,
+                             colmax=self.colmax)
+            rmax = self.colmax.cpu().numpy()
+            hist.append(float((rmax / np.where(emax > 0, emax, 1.0)).max()))
+            if np.all(rmax <= thresh) or rounds >= max_rounds:
+                break
+        return B, {"rounds": rounds, "inner_iters": rounds * inner_iters,
+                   "residual_history": hist}

@@ -83,6 +83,84 @@ __global__ void __launch_bounds__(256) k_ff_mask(flux_scene S, const T *__restri
   if (lane == 0) mask[p * W + w] = b;
 }
 
+// Same result as k_ff_mask<T, true>, with stream compaction: one wave per
+// row p.  The row is scanned 64 columns at a time; prefilter candidates are
+// appended to a per-wave LDS queue and rays are cast only when 64 are
+// queued, so every lane of the wave traces (the prefilter keeps ~1/4 of the
+// pairs on terrain, which leaves 3/4 of a one-pair-per-lane wave idle).
+// Results collect in a per-wave LDS bitmask for a segment of 64 words (4096
+// columns), flushed to the mask after each segment.  All LDS traffic is
+// wave-private, so no workgroup barrier is needed.
+template <typename T>
+__global__ void __launch_bounds__(256) k_ff_mask_rows(flux_scene S, const T *__restrict__ P,
+                                                      const T *__restrict__ N,
+                                                      const int64_t *__restrict__ I, int64_t m,
+                                                      const int64_t *__restrict__ J, int64_t n,
+                                                      T eps, int mode,
+                                                      uint64_t *__restrict__ mask, int64_t W) {
+  __shared__ int32_t queue[4][128];
+  __shared__ unsigned long long bits[4][64];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t p = (int64_t)blockIdx.x * 4 + wv;
+  if (p >= m) return;  // uniform per wave
+  const int64_t i64 = I[p];
+  const bool row_ok = i64 >= 0 && i64 < S.nf;
+  const int i = row_ok ? (int)i64 : 0;
+  T Pi[3] = {P[i * 3], P[i * 3 + 1], P[i * 3 + 2]};
+  T Ni[3] = {N[i * 3], N[i * 3 + 1], N[i * 3 + 2]};
+  const double pi64[3] = {(double)Pi[0], (double)Pi[1], (double)Pi[2]};
+  const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  int count = 0;
+
+  auto cast = [&](int active) {
+    if (lane < active) {
+      const int32_t q = queue[wv][lane];
+      const int j = (int)J[q];
+      double pj64[3] = {0, 0, 0};
+      if (mode == FLUX_MODE_EMBREE) load3(P, j, pj64);
+      if (face_to_face_vis(S, i, j, mode, pi64, pj64))
+        atomicOr(&bits[wv][(q >> 6) & 63], 1ull << (q & 63));
+    }
+    __builtin_amdgcn_wave_barrier();
+  };
+
+  for (int64_t w0 = 0; w0 < W; w0 += 64) {
+    bits[wv][lane] = 0ull;
+    __builtin_amdgcn_wave_barrier();
+    const int64_t w1 = min(W, w0 + 64);
+    for (int64_t w = w0; w < w1; ++w) {
+      const int64_t q = w * 64 + lane;
+      bool cand = false;
+      if (q < n && row_ok) {
+        const int64_t j64 = J[q];
+        if (j64 >= 0 && j64 < S.nf && j64 != i) {
+          const int j = (int)j64;
+          T Pj[3] = {P[(int64_t)j * 3], P[(int64_t)j * 3 + 1], P[(int64_t)j * 3 + 2]};
+          T Nj[3] = {N[(int64_t)j * 3], N[(int64_t)j * 3 + 1], N[(int64_t)j * 3 + 2]};
+          cand = ff_prefilter(Pi, Ni, Pj, Nj) > eps;
+        }
+      }
+      const unsigned long long bal = __ballot(cand);
+      if (cand) queue[wv][count + __popcll(bal & below)] = (int32_t)q;
+      count += __popcll(bal);
+      __builtin_amdgcn_wave_barrier();
+      if (count >= 64) {
+        cast(64);
+        const int rest = count - 64;
+        if (lane < rest) queue[wv][lane] = queue[wv][64 + lane];
+        __builtin_amdgcn_wave_barrier();
+        count = rest;
+      }
+    }
+    if (count > 0) {
+      cast(count);
+      count = 0;
+    }
+    if (w0 + lane < W) mask[p * W + w0 + lane] = bits[wv][lane];
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
 // row_nnz[p] = popcount of mask row p; one wave per row.
 __global__ void k_row_count(const uint64_t *__restrict__ mask, int64_t m, int64_t W,
                             int64_t *__restrict__ row_nnz) {
@@ -208,14 +286,13 @@ extern "C" int flux_ff_mask(const flux_scene *scene, const void *P, const void *
   hipStream_t st = as_stream(stream);
   const int64_t W = (n + 63) / 64;
   if (W > 0) {
-    const int64_t waves = m * W;
-    const unsigned blocks = ceil_div_u(waves, 4);
+    const unsigned blocks = ceil_div_u(m, 4);
     if (dtype == FLUX_F64)
-      hipLaunchKernelGGL(k_ff_mask<double>, dim3(blocks), dim3(256), 0, st, *scene,
+      hipLaunchKernelGGL(k_ff_mask_rows<double>, dim3(blocks), dim3(256), 0, st, *scene,
                          (const double *)P, (const double *)N, I, m, J, n, (double)eps, mode,
                          mask, W);
     else
-      hipLaunchKernelGGL(k_ff_mask<float>, dim3(blocks), dim3(256), 0, st, *scene,
+      hipLaunchKernelGGL(k_ff_mask_rows<float>, dim3(blocks), dim3(256), 0, st, *scene,
                          (const float *)P, (const float *)N, I, m, J, n, (float)eps, mode,
                          mask, W);
     FLUX_LAUNCH_CHECK();

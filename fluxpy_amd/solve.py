@@ -76,6 +76,8 @@ def device_operator(FF):
     array).  Anything else is rejected: there is no host execution path."""
     if hasattr(FF, "device_operator"):
         return FF.device_operator()
+    if hasattr(FF, "apply") and hasattr(FF, "jacobi_step"):
+        return FF  # already a device operator
     if hasattr(FF, "device_csr"):
         return _CsrOperator(FF.device_csr)
     if isinstance(FF, _kernels.DeviceCSR):

@@ -282,6 +282,54 @@ int flux_jacobi_update(int64_t n, int64_t nrhs, const void *E, const void *Y,
                        void *B1, int dtype, double *colmax, void *stream);
 
 /* ---------------------------------------------------------------------- */
+/* Near-field panels: the MFMA form of the H-matvec near field (no         */
+/* reference counterpart; they replace the per-leaf Dense / Csr products   */
+/* of compressed_form_factors.py:176, 194, 413 for many right-hand sides).  */
+/* Rows are cut into blocks of 64; block b keeps the sorted distinct       */
+/* columns its rows touch (padded with column 0 to a multiple of 16) at    */
+/* cols[colptr[b] .. colptr[b+1]) and a dense 64 x 16 row-major tile of    */
+/* values per 16-column chunk at element offset (colptr[b] + 16 c) * 64.   */
+/* ---------------------------------------------------------------------- */
+
+/* counts[b] = distinct columns of row block b rounded up to 16
+ * (ceil(nrows/64) blocks).  ncols must keep 2*ceil(ncols/32)*4 bytes of
+ * bitmap within 150 KiB of LDS. */
+int flux_panel_count(int64_t nrows, int64_t ncols, const int64_t *indptr,
+                     const int32_t *indices, int64_t *counts, void *stream);
+
+/* Fill cols (total_cols = colptr[nblk]) and the zero-padded panels with the
+ * CSR values converted to pdtype (BF16, F32 or F64). */
+int flux_panel_fill(int64_t nrows, int64_t ncols, const int64_t *indptr,
+                    const int32_t *indices, const void *data, int dtype,
+                    const int64_t *colptr, int64_t total_cols, int32_t *cols,
+                    void *panel, int pdtype, void *stream);
+
+/* BF16 panels x BF16 X (ncols x nrhs) on v_mfma_f32_16x16x16_bf16:
+ *   acc = P X;  out = base + s o acc  (s_i = rho_i if left, else 1;
+ *   base NULL -> out = acc);  xnext = bf16(t o out) (t_i = rho_i if right,
+ *   else 1; xnext may be NULL);  colmax[c] = max_i |out - prev| if colmax.
+ * One Jacobi sweep of the correction equation per call. */
+int flux_panel_mm_bf16(int64_t nrows, const int64_t *colptr, const int32_t *cols,
+                       const void *panel, const void *X, int64_t nrhs,
+                       const float *base, const float *rho, float rho_s,
+                       int left, float *out, void *xnext, const float *prev,
+                       double *colmax, void *stream);
+
+/* FP32 panels x FP64 X on v_mfma_f64_16x16x4_f64 (values widened):
+ *   acc = P X; with E: out = E + s o acc - B and colmax[c] = max_i |out|
+ *   (the FP64 refinement residual); without E: out = acc. */
+int flux_panel_mm_f64(int64_t nrows, const int64_t *colptr, const int32_t *cols,
+                      const float *panel, const double *X, int64_t nrhs,
+                      const double *E, const double *B, const double *rho,
+                      double rho_s, int left, double *out, double *colmax,
+                      void *stream);
+
+/* xb = bf16(s o x), s_i = rho_i (or rho_s) if scale, else 1. */
+int flux_scale_to_bf16(int64_t n, int64_t nrhs, const float *x,
+                       const float *rho, float rho_s, int scale, void *xb,
+                       void *stream);
+
+/* ---------------------------------------------------------------------- */
 /* Column-blocked vector kernels (solver building blocks; blocks are      */
 /* n x nrhs row-major, reductions per column in FP64, deterministic)       */
 /* ---------------------------------------------------------------------- */

@@ -76,10 +76,15 @@ def test_max_depth_2_for_stretched_sphere(torch_dev, n):
             for ii, jj in it.product(range(8), repeat=2):
                 sub = block._blocks[ii, jj]
                 assert sub.is_leaf
-                if sub.shape == (1, 1):
-                    assert isinstance(sub, FormFactorZeroBlock)
                 II = block._row_block_inds[ii]
                 JJ = block._col_block_inds[jj]
+                # The reference asserts Zero for every 1 x 1 sub-block, which
+                # holds on its own meshes where such a block is a facet
+                # against itself; in general a 1 x 1 block is Zero exactly
+                # when its entry is 0 (compressed_form_factors.py:330-333).
+                if sub.shape == (1, 1):
+                    zero = FF_gt[I[II], :][:, J[JJ]][0, 0] == 0
+                    assert isinstance(sub, FormFactorZeroBlock) == zero
                 diff = FF_gt[I[II], :][:, J[JJ]] - sub._mat
                 if diff.size:
                     assert abs(diff).max() == 0

@@ -11,6 +11,11 @@ from fluxpy_amd import meshes
 pytestmark = pytest.mark.gpu
 
 
+def _lib_device():
+    from fluxpy_amd import _lib
+    return _lib.device()
+
+
 @pytest.fixture(scope="module")
 def torch_dev():
     import torch
@@ -52,7 +57,7 @@ def test_scene_structure(torch_dev, crater):
     _lib.call("flux_scene_num_nodes", sm._scene.handle, _lib.ctypes.byref(nn))
     nn = nn.value
     assert nn == 2 * sm.num_faces - 1
-    buf = torch_dev.empty(nn * 8, dtype=torch_dev.float32, device="cuda")
+    buf = torch_dev.empty(nn * 8, dtype=torch_dev.float32, device=_lib_device())
     _lib.call("flux_scene_copy_nodes", sm._scene.handle, buf.data_ptr(), _lib.stream())
     nodes = buf.cpu().numpy().reshape(nn, 8)
     esc = nodes[:, 3].copy().view(np.int32)
@@ -283,7 +288,7 @@ def test_csr_transpose_and_products(torch_dev, crater, rng):
     assert np.allclose(Fm @ x, sp @ x, rtol=1e-13, atol=1e-15)
     assert np.allclose(Fm @ X, sp @ X, rtol=1e-13, atol=1e-15)
     assert np.allclose(Fm.T @ x, sp.T @ x, rtol=1e-13, atol=1e-15)
-    Xd = torch_dev.tensor(X, device="cuda")
+    Xd = torch_dev.tensor(X, device=_lib_device())
     assert np.allclose((Fm @ Xd).cpu().numpy(), sp @ X, rtol=1e-13, atol=1e-15)
     Fm2 = pickle.loads(pickle.dumps(Fm))
     assert (Fm2.tocsr() != sp).nnz == 0
