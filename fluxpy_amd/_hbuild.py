@@ -31,6 +31,14 @@ K0 = 40               # linalg.py:20 (k0 = 40)
 L_MAX = 1024          # largest sketch the batched eigensolver accepts
 EIG_CUT = 1e-14       # drop sketch directions with D_i <= EIG_CUT * D_0
 SKETCH_BYTES = 2 << 30
+# FP64 randomized sketches are switched off.  An FP64 build with low-rank
+# leaves (1,280-facet stretched icosphere, tol 1e-4, min_size 1024) ended in
+# a device memory fault, and the same kernels built for the host under
+# AddressSanitizer (also with FMA contraction, and with every allocation
+# filled with garbage) ran that build with no out-of-bounds access.  Until
+# the cause is shown, FP64 models get dense / CSR leaves only (exact
+# products, more bytes) and an FP64 sketch raises.
+FP64_SKETCH = False
 
 
 class _Node:
@@ -95,6 +103,9 @@ class _SvdBatch:
 
     def __init__(self, F, FT, rngs, ls, seed):
         t = _lib.require_gpu()
+        if F.data.dtype == t.float64 and not FP64_SKETCH:
+            raise _lib.FluxError("FP64 randomized sketches are switched off in this build "
+                                 "(see FP64_SKETCH in fluxpy_amd/_hbuild.py)")
         dev = _lib.device()
         st = _lib.stream()
         self.rngs = np.array(rngs, dtype=np.int64).reshape(-1, 4)
@@ -308,9 +319,9 @@ def _resolve(nd, itemsize):
                 _resolve(ch, itemsize)
     if nd.kind == "forced":
         kinds = [ch.final for row in nd.children for ch in row]
-        if all(k == "dense" for k in kinds):
+        if all(k in ("dense", "null") for k in kinds):
             nd.final, nd.nbytes = "dense", nd.m * nd.n * itemsize
-        elif all(k in ("csr", "zero") for k in kinds):
+        elif all(k in ("csr", "zero", "null") for k in kinds):
             nd.final, nd.nbytes = "csr", nd.nnz * itemsize
         else:
             nd.final = "tree"
@@ -337,9 +348,9 @@ def _resolve(nd, itemsize):
         nd.final, nd.nbytes = "svd", nb_svd
     else:
         kinds = [ch.final for row in nd.children for ch in row]
-        if all(k == "dense" for k in kinds):
+        if all(k in ("dense", "null") for k in kinds):
             nd.final, nd.nbytes = "dense", nd.m * nd.n * itemsize
-        elif all(k in ("csr", "zero") for k in kinds):
+        elif all(k in ("csr", "zero", "null") for k in kinds):
             nd.final, nd.nbytes = "csr", nd.nnz * itemsize
         else:
             nd.final, nd.nbytes = "tree", nb_child
@@ -427,6 +438,10 @@ def build(H, dims):
             if nd.max_depth is None or nd.max_depth > 1:
                 nxt.extend(_expand(ct, nd, None if nd.max_depth is None else nd.max_depth - 1,
                                    False))
+        if cands and not FP64_SKETCH and dt == np.float64:
+            for nd in cands:
+                nd.svd = None
+            cands = []
         if cands:
             _run_svds(H, F, FT, cands, itemsize, tdt, far)
         level = nxt

@@ -267,7 +267,8 @@ __global__ void __launch_bounds__(256) k_panel_mm_f64(
       const int64_t c = g0 + t * 16 + (lane & 15);
       double cm = 0.0;
       for (int q = 0; q < 4; ++q) {
-        const int64_t r = row_base + w * 16 + (lane >> 4) * 4 + q;
+        // 16x16x4 FP64 result layout: lane l, item q holds D[4 q + l/16][l%16]
+        const int64_t r = row_base + w * 16 + q * 4 + (lane >> 4);
         if (r >= nrows || c >= nrhs) continue;
         const int64_t o = r * nrhs + c;
         double y = acc[t][q];
