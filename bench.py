@@ -51,6 +51,9 @@ def main():
     ap.add_argument("--min-size", type=int, default=4096)
     ap.add_argument("--inner", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--lowrank", action="store_true",
+                    help="offer blocks to the randomized sketch (low-rank leaves); off by "
+                         "default, so every leaf is dense / CSR and the operator is exact")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -103,6 +106,8 @@ def main():
     # ---------------- compressed matrix ----------------
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    from fluxpy_amd import _hbuild
+    _hbuild.LOWRANK = args.lowrank
     H = CompressedFormFactorMatrix(sm, tol=args.tol, min_size=args.min_size)
     torch.cuda.synchronize()
     t_build = time.perf_counter() - t0
@@ -219,11 +224,13 @@ def main():
         "data": "synthetic (seeded fractal terrain, seeded suns and albedo)",
         "config": {
             "workload": "fractal terrain 145^2 vertices (41,472 facets), 64 suns, compressed "
-                        "F (tol 1e-6, min_size 4096), BF16 panels + FP64-residual refinement "
-                        "to 1e-6",
+                        "F tree (tol 1e-6, min_size 4096%s), BF16 panels + FP64-residual "
+                        "refinement to 1e-6" % ("" if args.lowrank else
+                                                ", low-rank leaves off: exact F"),
             "facets": nf, "suns": args.suns, "near_nnz": nnz, "tol": args.tol,
             "min_size": args.min_size, "compressed_nbytes": int(H.nbytes),
             "leaves": bs["leaf_counts"], "svd_leaves": bs["svd_leaves"],
+            "lowrank": bool(args.lowrank),
         },
         "fp64_solves_per_s": round(fp64_sps, 2),
         "speedup_vs_fp64": round(sps / fp64_sps, 2),

@@ -39,6 +39,9 @@ SKETCH_BYTES = 2 << 30
 # the cause is shown, FP64 models get dense / CSR leaves only (exact
 # products, more bytes) and an FP64 sketch raises.
 FP64_SKETCH = False
+# False: no block is offered to the sketch, so every leaf is dense / CSR /
+# zero (the product is exact; nbytes is that of the near field).
+LOWRANK = True
 
 
 class _Node:
@@ -438,7 +441,7 @@ def build(H, dims):
             if nd.max_depth is None or nd.max_depth > 1:
                 nxt.extend(_expand(ct, nd, None if nd.max_depth is None else nd.max_depth - 1,
                                    False))
-        if cands and not FP64_SKETCH and dt == np.float64:
+        if cands and (not LOWRANK or (not FP64_SKETCH and dt == np.float64)):
             for nd in cands:
                 nd.svd = None
             cands = []
