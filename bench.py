@@ -51,6 +51,8 @@ def main():
     ap.add_argument("--min-size", type=int, default=4096)
     ap.add_argument("--inner", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib", action="store_true",
+                    help="end with a 512 MiB device copy (counter calibration)")
     ap.add_argument("--lowrank", action="store_true",
                     help="offer blocks to the randomized sketch (low-rank leaves); off by "
                          "default, so every leaf is dense / CSR and the operator is exact")
@@ -204,6 +206,18 @@ def main():
     alg_bytes = nnz * (2 + 4) + 4 * (nf + 1) + k * nf * (2 + 4) + k * nf * (4 + 2)
     achieved = alg_bytes / (k_ms * 1e-3) / 1e9
     log(f"[bench] {kname}: {k_ms * 1e3:.1f} us/launch, {achieved:.0f} GB/s algorithmic")
+
+    if args.calib:
+        # a plain device copy of known size: its FETCH_SIZE / WRITE_SIZE counters
+        # scale the panel kernel's counters to HBM bytes
+        src = torch.ones(128 << 20, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] calibration copy: {src.numel() * 4} bytes read, "
+            f"{dst.numel() * 4} bytes written per launch")
+        del src, dst
 
     cpu = None
     if not args.no_cpu:

@@ -88,6 +88,10 @@ __global__ void __launch_bounds__(256) k_ff_mask(flux_scene S, const T *__restri
 // appended to a per-wave LDS queue and rays are cast only when 64 are
 // queued, so every lane of the wave traces (the prefilter keeps ~1/4 of the
 // pairs on terrain, which leaves 3/4 of a one-pair-per-lane wave idle).
+// Each lane walks the tree on its own: tracing the 64 queued rays as one
+// wave-coherent packet (one shared node sequence) measured 1.9x slower on
+// the 41,472-facet terrain, because the rays of a queue fan out widely and
+// the union of their node sets is far larger than any single ray's.
 // Results collect in a per-wave LDS bitmask for a segment of 64 words (4096
 // columns), flushed to the mask after each segment.  All LDS traffic is
 // wave-private, so no workgroup barrier is needed.

@@ -14,8 +14,16 @@
 //   FP32 values x FP64 vectors -> FP64   v_mfma_f64_16x16x4_f64 (values
 //                                         widened to FP64 in registers)
 // Workgroup = one row block, 4 waves, wave w owns rows 16w..16w+15 and all
-// right-hand sides; the gathered X chunk is staged once in LDS for the four
-// waves.  Right-hand sides come in groups of 64 (4 tiles of 16).
+// right-hand sides; right-hand sides come in groups of 64 (4 tiles of 16).
+// The MFMA computes Y^T = X^T P^T: the staged X block is the A operand (from
+// LDS), the panel tile the B operand (straight from HBM into registers), and
+// each lane ends with consecutive right-hand sides of one row, so the
+// epilogue reads and writes whole vectors.  Columns are consumed in stages of
+// several chunks; X is double-buffered in LDS and the panel fragments in
+// registers, so the loads of stage s+1 are in flight during the MFMAs of
+// stage s and a workgroup waits at one barrier per stage.  Row blocks are
+// dealt to the 8 XCDs in contiguous runs so neighbouring blocks, which
+// gather mostly the same X rows, share an L2.
 #include <hip/hip_bf16.h>
 
 #include "common.h"
@@ -153,68 +161,176 @@ __global__ void k_zero_any(T *p, int64_t n) {
 // base == NULL -> out = acc; xnext == NULL -> not written;
 // colmax != NULL -> colmax[c] = max_i |out[i, c] - prev[i, c]|.
 // ---------------------------------------------------------------------------
+// Row block of workgroup `wg`: XCD x = wg % 8 (the dispatch order) owns the
+// contiguous run of blocks [x q + min(x, r), ...), q = nblk / 8, r = nblk % 8.
+__device__ __forceinline__ int64_t xcd_block(int64_t wg, int64_t nblk) {
+  const int64_t x = wg & 7, j = wg >> 3, q = nblk >> 3, r = nblk & 7;
+  return x * q + min(x, r) + j;
+}
+
+// Max over the 16 lanes that share lane / 16 (bits 0-3 of the lane id).
+__device__ __forceinline__ double max16(double v) {
+  for (int m = 1; m < 16; m <<= 1) v = fmax(v, __shfl_xor(v, m, 64));
+  return v;
+}
+
+// kBf16Stage chunks of 16 columns per stage.  LDS block per stage (bf16):
+// xs[kg][rhs][4], kg = group of 4 consecutive columns of the stage, so the
+// A fragment of a lane (4 consecutive columns at one right-hand side) is one
+// 8-byte read.
+constexpr int kBf16Stage = 4;
+
 __global__ void __launch_bounds__(256) k_panel_mm_bf16(
-    int64_t nrows, const int64_t *__restrict__ colptr, const int32_t *__restrict__ cols,
-    const __hip_bfloat16 *__restrict__ panel, const __hip_bfloat16 *__restrict__ X,
-    int64_t nrhs, const float *__restrict__ base, const float *__restrict__ rho, float rho_s,
-    int left, float *__restrict__ out, __hip_bfloat16 *__restrict__ xnext,
-    const float *__restrict__ prev, double *__restrict__ colmax) {
-  __shared__ __align__(16) short xs[64 * 16];  // xs[rhs][kk] of the staged chunk (bf16)
-  const int64_t b = blockIdx.x;
+    int64_t nrows, int64_t nblk, const int64_t *__restrict__ colptr,
+    const int32_t *__restrict__ cols, const __hip_bfloat16 *__restrict__ panel,
+    const __hip_bfloat16 *__restrict__ X, int64_t nrhs, const float *__restrict__ base,
+    const float *__restrict__ rho, float rho_s, int left, float *__restrict__ out,
+    __hip_bfloat16 *__restrict__ xnext, const float *__restrict__ prev,
+    double *__restrict__ colmax) {
+  constexpr int SC = kBf16Stage, KG = SC * 4, GPT = KG / 16;  // groups per thread
+  __shared__ __align__(16) uint2 xs[2][KG * 64];
+  const int64_t b = xcd_block(blockIdx.x, nblk);
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int64_t c0 = colptr[b], c1 = colptr[b + 1];
-  const int64_t nchunk = (c1 - c0) >> 4;
+  const int li = lane & 15, lg = lane >> 4;
+  const int64_t c0 = colptr[b];
+  const int64_t nch = (colptr[b + 1] - c0) >> 4;
+  const int64_t nst = (nch + SC - 1) / SC;
   const int64_t row_base = b * kPanelRows;
-  const short *P = reinterpret_cast<const short *>(panel);
-  const short *Xs = reinterpret_cast<const short *>(X);
+  const unsigned short *Xs = reinterpret_cast<const unsigned short *>(X);
+  const uint2 *P2 = reinterpret_cast<const uint2 *>(panel);
+  const bool vec_x = (nrhs & 3) == 0;
+  // staging role: column group kg = tid / 16 (+16 i), right-hand sides 4 rq .. 4 rq + 3
+  const int sg = tid >> 4, rq = tid & 15;
   for (int64_t g0 = 0; g0 < nrhs; g0 += 64) {
-    const int ntile = (int)min((int64_t)4, (nrhs - g0 + 15) / 16);
+    uint2 xr[GPT][4];
+    uint2 pcur[SC], pnext[SC];
+    // global loads of stage s: X groups into xr, panel fragments into pf
+    auto load = [&](int64_t s, uint2 (&pf)[SC]) {
+      for (int i = 0; i < GPT; ++i) {
+        const int kg = sg + 16 * i;
+        const int64_t lc = s * SC * 16 + kg * 4;  // first column (block-local)
+        const int64_t rc = g0 + 4 * rq;
+        for (int j = 0; j < 4; ++j) {
+          uint2 v = make_uint2(0u, 0u);
+          if (lc < nch * 16) {
+            const int64_t col = cols[c0 + lc + j];
+            const int64_t e = col * nrhs + rc;
+            if (vec_x && rc + 3 < nrhs) {
+              v = *reinterpret_cast<const uint2 *>(Xs + e);
+            } else {
+              unsigned short h[4] = {0, 0, 0, 0};
+              for (int q = 0; q < 4; ++q)
+                if (rc + q < nrhs) h[q] = Xs[e + q];
+              v = make_uint2(h[0] | ((unsigned)h[1] << 16), h[2] | ((unsigned)h[3] << 16));
+            }
+          }
+          xr[i][j] = v;
+        }
+      }
+      for (int c = 0; c < SC; ++c) {
+        const int64_t ch = s * SC + c;
+        // B fragment: row 16 w + li, columns 4 lg .. 4 lg + 3 of chunk ch
+        pf[c] = ch < nch ? P2[((c0 + ch * 16) * kPanelRows + (w * 16 + li) * 16 + lg * 4) / 4]
+                         : make_uint2(0u, 0u);
+      }
+    };
+    // LDS store of the staged X groups: xs[kg][rhs][0..3] = 4 columns at one rhs
+    auto store = [&](int buf) {
+      for (int i = 0; i < GPT; ++i) {
+        const int kg = sg + 16 * i;
+        for (int r = 0; r < 4; ++r) {
+          const int sh = (r & 1) * 16;
+          const unsigned e0 = ((r < 2 ? xr[i][0].x : xr[i][0].y) >> sh) & 0xffffu;
+          const unsigned e1 = ((r < 2 ? xr[i][1].x : xr[i][1].y) >> sh) & 0xffffu;
+          const unsigned e2 = ((r < 2 ? xr[i][2].x : xr[i][2].y) >> sh) & 0xffffu;
+          const unsigned e3 = ((r < 2 ? xr[i][3].x : xr[i][3].y) >> sh) & 0xffffu;
+          xs[buf][kg * 64 + 4 * rq + r] = make_uint2(e0 | (e1 << 16), e2 | (e3 << 16));
+        }
+      }
+    };
     v4f acc[4];
     for (int t = 0; t < 4; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
-    for (int64_t ch = 0; ch < nchunk; ++ch) {
-      // stage X[cols[chunk], g0 .. g0+63] transposed into LDS
-      {
-        const int kk = tid >> 4, part = tid & 15;
-        const int64_t col = cols[c0 + ch * 16 + kk];
-        const int64_t rbase = col * nrhs + g0 + part * 4;
-        for (int q = 0; q < 4; ++q) {
-          const int64_t rc = g0 + part * 4 + q;
-          xs[(part * 4 + q) * 16 + kk] = rc < nrhs ? Xs[rbase + q] : (short)0;
+    if (nst > 0) {
+      load(0, pnext);
+      store(0);
+    }
+    __syncthreads();
+    for (int64_t s = 0; s < nst; ++s) {
+      const int buf = (int)(s & 1);
+      for (int c = 0; c < SC; ++c) pcur[c] = pnext[c];
+      if (s + 1 < nst) load(s + 1, pnext);
+      for (int c = 0; c < SC; ++c) {
+        v4s bfrag;
+        bfrag[0] = (short)(pcur[c].x & 0xffffu);
+        bfrag[1] = (short)(pcur[c].x >> 16);
+        bfrag[2] = (short)(pcur[c].y & 0xffffu);
+        bfrag[3] = (short)(pcur[c].y >> 16);
+        for (int t = 0; t < 4; ++t) {
+          // A fragment: X^T[rhs 16 t + li][columns 4 (4c + lg) .. +3]
+          const uint2 av = xs[buf][(c * 4 + lg) * 64 + t * 16 + li];
+          v4s afrag;
+          afrag[0] = (short)(av.x & 0xffffu);
+          afrag[1] = (short)(av.x >> 16);
+          afrag[2] = (short)(av.y & 0xffffu);
+          afrag[3] = (short)(av.y >> 16);
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(afrag, bfrag, acc[t], 0, 0, 0);
         }
       }
-      __syncthreads();
-      // A fragment: rows 16w + lane%16, k = 4 (lane/16) .. +3
-      const int64_t aoff = (c0 + ch * 16) * kPanelRows + (int64_t)(w * 16 + (lane & 15)) * 16 +
-                           (lane >> 4) * 4;
-      const v4s a = *reinterpret_cast<const v4s *>(P + aoff);
-      for (int t = 0; t < ntile; ++t) {
-        const v4s bb = *reinterpret_cast<const v4s *>(&xs[(t * 16 + (lane & 15)) * 16 +
-                                                          (lane >> 4) * 4]);
-        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bb, acc[t], 0, 0, 0);
-      }
+      if (s + 1 < nst) store(buf ^ 1);
       __syncthreads();
     }
-    // epilogue: acc[t][q] = Y[row 16w + 4 (lane/16) + q][rhs g0 + 16 t + lane%16]
-    for (int t = 0; t < ntile; ++t) {
-      const int64_t c = g0 + t * 16 + (lane & 15);
-      double cm = 0.0;
-      for (int q = 0; q < 4; ++q) {
-        const int64_t r = row_base + w * 16 + (lane >> 4) * 4 + q;
-        if (r >= nrows || c >= nrhs) continue;
-        const int64_t o = r * nrhs + c;
-        const float rr = rho ? rho[r] : rho_s;
-        float y = acc[t][q];
-        if (left) y *= rr;
-        const float v = base ? base[o] + y : y;
-        out[o] = v;
-        if (xnext) {
-          const float xv = left ? v : rr * v;
-          xnext[o] = __float2bfloat16(xv);
+    // epilogue: acc[t][q] = Y[row 16 w + li][rhs g0 + 16 t + 4 lg + q]
+    const int64_t r = row_base + w * 16 + li;
+    const bool row_ok = r < nrows;
+    const float rr = row_ok ? (rho ? rho[r] : rho_s) : 0.f;
+    for (int t = 0; t < 4; ++t) {
+      const int64_t cb = g0 + t * 16 + lg * 4;
+      double cm[4] = {0.0, 0.0, 0.0, 0.0};
+      if (row_ok && cb < nrhs) {
+        const int64_t o = r * nrhs + cb;
+        float v[4];
+        for (int q = 0; q < 4; ++q) v[q] = left ? acc[t][q] * rr : acc[t][q];
+        if (vec_x && cb + 3 < nrhs) {
+          if (base) {
+            const float4 bv = *reinterpret_cast<const float4 *>(base + o);
+            v[0] += bv.x; v[1] += bv.y; v[2] += bv.z; v[3] += bv.w;
+          }
+          *reinterpret_cast<float4 *>(out + o) = make_float4(v[0], v[1], v[2], v[3]);
+          if (xnext) {
+            const float s0 = left ? 1.f : rr;
+            unsigned short h[4];
+            for (int q = 0; q < 4; ++q) {
+              const __hip_bfloat16 hb = __float2bfloat16(s0 * v[q]);
+              h[q] = *reinterpret_cast<const unsigned short *>(&hb);
+            }
+            *reinterpret_cast<uint2 *>(xnext + o) =
+                make_uint2(h[0] | ((unsigned)h[1] << 16), h[2] | ((unsigned)h[3] << 16));
+          }
+          if (colmax) {
+            const float4 pv = *reinterpret_cast<const float4 *>(prev + o);
+            cm[0] = fabs((double)v[0] - (double)pv.x);
+            cm[1] = fabs((double)v[1] - (double)pv.y);
+            cm[2] = fabs((double)v[2] - (double)pv.z);
+            cm[3] = fabs((double)v[3] - (double)pv.w);
+          }
+        } else {
+          for (int q = 0; q < 4; ++q) {
+            if (cb + q >= nrhs) break;
+            if (base) v[q] += base[o + q];
+            out[o + q] = v[q];
+            if (xnext) xnext[o + q] = __float2bfloat16(left ? v[q] : rr * v[q]);
+            if (colmax) cm[q] = fabs((double)v[q] - (double)prev[o + q]);
+          }
         }
-        if (colmax) cm = fmax(cm, fabs((double)v - (double)prev[o]));
       }
-      if (colmax && c < nrhs)
-        atomicMax((unsigned long long *)(colmax + c), (unsigned long long)__double_as_longlong(cm));
+      if (colmax) {
+        for (int q = 0; q < 4; ++q) {
+          const double m = max16(cm[q]);
+          if (li == 0 && cb + q < nrhs)
+            atomicMax((unsigned long long *)(colmax + cb + q),
+                      (unsigned long long)__double_as_longlong(m));
+        }
+      }
     }
   }
 }
@@ -226,61 +342,119 @@ __global__ void __launch_bounds__(256) k_panel_mm_bf16(
 //   colmax[c] = max_i |out[i, c]|
 // E == NULL -> out = acc (plain product).
 // ---------------------------------------------------------------------------
+// kF64Stage chunks of 16 columns per stage; LDS block per stage (FP64):
+// xs[column][rhs].  In K step ks of a chunk, lane group lg = lane / 16 takes
+// column 4 lg + ks, so a lane's panel fragment for the whole chunk is one
+// 16-byte load (4 consecutive FP32 values of its row).
+constexpr int kF64Stage = 2;
+
 __global__ void __launch_bounds__(256) k_panel_mm_f64(
-    int64_t nrows, const int64_t *__restrict__ colptr, const int32_t *__restrict__ cols,
-    const float *__restrict__ panel, const double *__restrict__ X, int64_t nrhs,
-    const double *__restrict__ E, const double *__restrict__ B,
-    const double *__restrict__ rho, double rho_s, int left, double *__restrict__ out,
-    double *__restrict__ colmax) {
-  __shared__ double xs[16 * 64];  // xs[kk][rhs]
-  const int64_t b = blockIdx.x;
+    int64_t nrows, int64_t nblk, const int64_t *__restrict__ colptr,
+    const int32_t *__restrict__ cols, const float *__restrict__ panel,
+    const double *__restrict__ X, int64_t nrhs, const double *__restrict__ E,
+    const double *__restrict__ B, const double *__restrict__ rho, double rho_s, int left,
+    double *__restrict__ out, double *__restrict__ colmax) {
+  constexpr int SC = kF64Stage, NC = SC * 16;
+  __shared__ __align__(16) double xs[2][NC * 64];
+  const int64_t b = xcd_block(blockIdx.x, nblk);
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int64_t c0 = colptr[b], c1 = colptr[b + 1];
-  const int64_t nchunk = (c1 - c0) >> 4;
+  const int li = lane & 15, lg = lane >> 4;
+  const int64_t c0 = colptr[b];
+  const int64_t nch = (colptr[b + 1] - c0) >> 4;
+  const int64_t nst = (nch + SC - 1) / SC;
   const int64_t row_base = b * kPanelRows;
+  const float4 *P4 = reinterpret_cast<const float4 *>(panel);
+  const bool vec_x = (nrhs & 3) == 0;
+  // staging role: columns sc + 16 i of the stage, right-hand sides 4 rq .. 4 rq + 3
+  const int sc = tid >> 4, rq = tid & 15;
   for (int64_t g0 = 0; g0 < nrhs; g0 += 64) {
-    const int ntile = (int)min((int64_t)4, (nrhs - g0 + 15) / 16);
+    double2 xr[SC][2];
+    float4 pcur[SC], pnext[SC];
+    auto load = [&](int64_t s, float4 (&pf)[SC]) {
+      const int64_t rc = g0 + 4 * rq;
+      for (int i = 0; i < SC; ++i) {
+        const int64_t lc = s * NC + sc + 16 * i;
+        double v[4] = {0.0, 0.0, 0.0, 0.0};
+        if (lc < nch * 16) {
+          const int64_t e = (int64_t)cols[c0 + lc] * nrhs + rc;
+          if (vec_x && rc + 3 < nrhs) {
+            const double2 a0 = *reinterpret_cast<const double2 *>(X + e);
+            const double2 a1 = *reinterpret_cast<const double2 *>(X + e + 2);
+            v[0] = a0.x;
+            v[1] = a0.y;
+            v[2] = a1.x;
+            v[3] = a1.y;
+          } else {
+            for (int q = 0; q < 4; ++q)
+              if (rc + q < nrhs) v[q] = X[e + q];
+          }
+        }
+        xr[i][0] = double2{v[0], v[1]};
+        xr[i][1] = double2{v[2], v[3]};
+      }
+      for (int c = 0; c < SC; ++c) {
+        const int64_t ch = s * SC + c;
+        pf[c] = ch < nch ? P4[((c0 + ch * 16) * kPanelRows + (w * 16 + li) * 16 + lg * 4) / 4]
+                         : make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+    };
+    auto store = [&](int buf) {
+      for (int i = 0; i < SC; ++i) {
+        double2 *d = reinterpret_cast<double2 *>(&xs[buf][(sc + 16 * i) * 64 + 4 * rq]);
+        d[0] = xr[i][0];
+        d[1] = xr[i][1];
+      }
+    };
     v4d acc[4];
     for (int t = 0; t < 4; ++t) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
-    for (int64_t ch = 0; ch < nchunk; ++ch) {
-      {
-        const int kk = tid >> 4, part = tid & 15;
-        const int64_t col = cols[c0 + ch * 16 + kk];
-        for (int q = 0; q < 4; ++q) {
-          const int64_t rc = g0 + part * 4 + q;
-          xs[kk * 64 + part * 4 + q] = rc < nrhs ? X[col * nrhs + rc] : 0.0;
+    if (nst > 0) {
+      load(0, pnext);
+      store(0);
+    }
+    __syncthreads();
+    for (int64_t s = 0; s < nst; ++s) {
+      const int buf = (int)(s & 1);
+      for (int c = 0; c < SC; ++c) pcur[c] = pnext[c];
+      if (s + 1 < nst) load(s + 1, pnext);
+      for (int c = 0; c < SC; ++c) {
+        const float pv[4] = {pcur[c].x, pcur[c].y, pcur[c].z, pcur[c].w};
+        for (int ks = 0; ks < 4; ++ks) {
+          // B: P^T[column 4 lg + ks][row li];  A: X^T[rhs 16 t + li][column 4 lg + ks]
+          const double bv = (double)pv[ks];
+          const double *xrow = &xs[buf][(c * 16 + lg * 4 + ks) * 64 + li];
+          for (int t = 0; t < 4; ++t)
+            acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(xrow[t * 16], bv, acc[t], 0, 0, 0);
         }
       }
-      __syncthreads();
-      const int64_t abase = (c0 + ch * 16) * kPanelRows + (int64_t)(w * 16 + (lane & 15)) * 16;
-      for (int s = 0; s < 4; ++s) {
-        // K step s: A[i][k] with i = lane%16, k = 4 s + lane/16
-        const double a = (double)panel[abase + s * 4 + (lane >> 4)];
-        for (int t = 0; t < ntile; ++t) {
-          const double bb = xs[(s * 4 + (lane >> 4)) * 64 + t * 16 + (lane & 15)];
-          acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bb, acc[t], 0, 0, 0);
-        }
-      }
+      if (s + 1 < nst) store(buf ^ 1);
       __syncthreads();
     }
-    for (int t = 0; t < ntile; ++t) {
-      const int64_t c = g0 + t * 16 + (lane & 15);
-      double cm = 0.0;
+    // epilogue: 16x16x4 FP64 result layout, lane l item q = D[4 q + l/16][l%16]:
+    // acc[t][q] = Y[row 16 w + li][rhs g0 + 16 t + 4 q + lg]
+    const int64_t r = row_base + w * 16 + li;
+    const bool row_ok = r < nrows;
+    const double rr = row_ok ? (rho ? rho[r] : rho_s) : 0.0;
+    for (int t = 0; t < 4; ++t) {
       for (int q = 0; q < 4; ++q) {
-        // 16x16x4 FP64 result layout: lane l, item q holds D[4 q + l/16][l%16]
-        const int64_t r = row_base + w * 16 + q * 4 + (lane >> 4);
-        if (r >= nrows || c >= nrhs) continue;
-        const int64_t o = r * nrhs + c;
-        double y = acc[t][q];
-        if (E) {
-          if (left) y *= rho ? rho[r] : rho_s;
-          y = E[o] + y - B[o];
-          cm = fmax(cm, fabs(y));
+        const int64_t c = g0 + t * 16 + 4 * q + lg;
+        double cm = 0.0;
+        if (row_ok && c < nrhs) {
+          const int64_t o = r * nrhs + c;
+          double y = acc[t][q];
+          if (E) {
+            if (left) y *= rr;
+            y = E[o] + y - B[o];
+            cm = fabs(y);
+          }
+          out[o] = y;
         }
-        out[o] = y;
+        if (colmax && E) {
+          const double m = max16(cm);
+          if (li == 0 && c < nrhs)
+            atomicMax((unsigned long long *)(colmax + c),
+                      (unsigned long long)__double_as_longlong(m));
+        }
       }
-      if (colmax && E && c < nrhs)
-        atomicMax((unsigned long long *)(colmax + c), (unsigned long long)__double_as_longlong(cm));
     }
   }
 }
@@ -377,9 +551,9 @@ extern "C" int flux_panel_mm_bf16(int64_t nrows, const int64_t *colptr, const in
   if (colmax)
     hipLaunchKernelGGL(k_zero_f64_p, dim3(ceil_div_u(nrhs, 256)), dim3(256), 0, st, colmax,
                        nrhs);
-  hipLaunchKernelGGL(k_panel_mm_bf16, dim3((unsigned)nblk), dim3(256), 0, st, nrows, colptr,
-                     cols, (const __hip_bfloat16 *)panel, (const __hip_bfloat16 *)X, nrhs, base,
-                     rho, rho_s, left, out, (__hip_bfloat16 *)xnext, prev, colmax);
+  hipLaunchKernelGGL(k_panel_mm_bf16, dim3((unsigned)nblk), dim3(256), 0, st, nrows, nblk,
+                     colptr, cols, (const __hip_bfloat16 *)panel, (const __hip_bfloat16 *)X, nrhs,
+                     base, rho, rho_s, left, out, (__hip_bfloat16 *)xnext, prev, colmax);
   FLUX_LAUNCH_CHECK();
   return 0;
 }
@@ -398,8 +572,8 @@ extern "C" int flux_panel_mm_f64(int64_t nrows, const int64_t *colptr, const int
   if (colmax)
     hipLaunchKernelGGL(k_zero_f64_p, dim3(ceil_div_u(nrhs, 256)), dim3(256), 0, st, colmax,
                        nrhs);
-  hipLaunchKernelGGL(k_panel_mm_f64, dim3((unsigned)nblk), dim3(256), 0, st, nrows, colptr, cols,
-                     panel, X, nrhs, E, B, rho, rho_s, left, out, colmax);
+  hipLaunchKernelGGL(k_panel_mm_f64, dim3((unsigned)nblk), dim3(256), 0, st, nrows, nblk, colptr,
+                     cols, panel, X, nrhs, E, B, rho, rho_s, left, out, colmax);
   FLUX_LAUNCH_CHECK();
   return 0;
 }
