@@ -33,6 +33,7 @@ namespace flux {
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef double v4d __attribute__((ext_vector_type(4)));
+typedef double v2d __attribute__((ext_vector_type(2)));
 
 constexpr int kPanelRows = 64;
 
@@ -174,20 +175,22 @@ __device__ __forceinline__ double max16(double v) {
   return v;
 }
 
-// kBf16Stage chunks of 16 columns per stage.  LDS block per stage (bf16):
+// SC chunks of 16 columns per stage.  LDS block per stage (bf16):
 // xs[kg][rhs][4], kg = group of 4 consecutive columns of the stage, so the
 // A fragment of a lane (4 consecutive columns at one right-hand side) is one
 // 8-byte read.
-constexpr int kBf16Stage = 4;
-
-__global__ void __launch_bounds__(256) k_panel_mm_bf16(
+// Waves per SIMD: 3 for SC = 4 (all 648 row blocks of the 41k-facet
+// terrain resident at once, no spills), 2 for SC = 8.
+template <int SC, bool VEC>
+__global__ void __launch_bounds__(256, SC == 4 ? 3 : 2) k_panel_mm_bf16(
     int64_t nrows, int64_t nblk, const int64_t *__restrict__ colptr,
     const int32_t *__restrict__ cols, const __hip_bfloat16 *__restrict__ panel,
     const __hip_bfloat16 *__restrict__ X, int64_t nrhs, const float *__restrict__ base,
     const float *__restrict__ rho, float rho_s, int left, float *__restrict__ out,
     __hip_bfloat16 *__restrict__ xnext, const float *__restrict__ prev,
     double *__restrict__ colmax) {
-  constexpr int SC = kBf16Stage, KG = SC * 4, GPT = KG / 16;  // groups per thread
+  static_assert(SC % 4 == 0, "a stage holds a multiple of 4 chunks");
+  constexpr int KG = SC * 4, GPT = KG / 16;  // column groups per stage / per thread
   __shared__ __align__(16) uint2 xs[2][KG * 64];
   const int64_t b = xcd_block(blockIdx.x, nblk);
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -202,19 +205,46 @@ __global__ void __launch_bounds__(256) k_panel_mm_bf16(
   // staging role: column group kg = tid / 16 (+16 i), right-hand sides 4 rq .. 4 rq + 3
   const int sg = tid >> 4, rq = tid & 15;
   for (int64_t g0 = 0; g0 < nrhs; g0 += 64) {
-    uint2 xr[GPT][4];
-    uint2 pcur[SC], pnext[SC];
-    // global loads of stage s: X groups into xr, panel fragments into pf
-    auto load = [&](int64_t s, uint2 (&pf)[SC]) {
+    // Registers: X of stages s+1 and s+2 (xa / xb, alternating), the column
+    // ids of the next X load (ci), panel fragments of stages s, s+1, s+2
+    // (p0, p1, p2).  Stage s issues X(s+2), ids(s+3), panel(s+2), computes
+    // from LDS and p(s), and stores X(s+1) (loaded a stage earlier) to LDS,
+    // so every load has a full stage of other work behind it.
+    uint2 xa[GPT][4], xb[GPT][4];
+    int32_t ci[GPT][4];
+    uint2 p0[SC], p1[SC], p2[SC];
+    const int64_t rc = g0 + 4 * rq;
+    // With VEC (nrhs % 4 == 0) every load is unconditional from a clamped,
+    // valid address and nothing is masked: a guard or a select right after
+    // a load makes the compiler wait for it, which defeats the prefetch.
+    // Loads for chunks past the block's end (the pipeline only runs when
+    // nch >= 1, so column c0 and chunk 0 are valid fallbacks) feed MFMAs
+    // that are skipped; right-hand sides past nrhs read column 0 of their
+    // row and only reach outputs the epilogue discards.
+    auto load_ids = [&](int64_t s, int32_t (&ci)[GPT][4]) {
       for (int i = 0; i < GPT; ++i) {
-        const int kg = sg + 16 * i;
-        const int64_t lc = s * SC * 16 + kg * 4;  // first column (block-local)
-        const int64_t rc = g0 + 4 * rq;
+        const int64_t lc = s * SC * 16 + (sg + 16 * i) * 4;  // first column (block-local)
+        const bool ok = lc < nch * 16;
         for (int j = 0; j < 4; ++j) {
+          if constexpr (VEC) {
+            ci[i][j] = cols[c0 + (ok ? lc + j : 0)];
+          } else {
+            ci[i][j] = ok ? cols[c0 + lc + j] : -1;
+          }
+        }
+      }
+    };
+    auto load_x = [&](const int32_t (&ci)[GPT][4], uint2 (&xr)[GPT][4]) {
+      for (int i = 0; i < GPT; ++i) {
+        for (int j = 0; j < 4; ++j) {
+          if constexpr (VEC) {
+            xr[i][j] = *reinterpret_cast<const uint2 *>(
+                Xs + (int64_t)ci[i][j] * nrhs + (rc < nrhs ? rc : 0));
+            continue;
+          }
           uint2 v = make_uint2(0u, 0u);
-          if (lc < nch * 16) {
-            const int64_t col = cols[c0 + lc + j];
-            const int64_t e = col * nrhs + rc;
+          if (ci[i][j] >= 0) {
+            const int64_t e = (int64_t)ci[i][j] * nrhs + rc;
             if (vec_x && rc + 3 < nrhs) {
               v = *reinterpret_cast<const uint2 *>(Xs + e);
             } else {
@@ -227,15 +257,22 @@ __global__ void __launch_bounds__(256) k_panel_mm_bf16(
           xr[i][j] = v;
         }
       }
+    };
+    auto load_p = [&](int64_t s, uint2 (&pf)[SC]) {
       for (int c = 0; c < SC; ++c) {
         const int64_t ch = s * SC + c;
+        const bool ok = ch < nch;
         // B fragment: row 16 w + li, columns 4 lg .. 4 lg + 3 of chunk ch
-        pf[c] = ch < nch ? P2[((c0 + ch * 16) * kPanelRows + (w * 16 + li) * 16 + lg * 4) / 4]
-                         : make_uint2(0u, 0u);
+        const uint2 v =
+            P2[((c0 + (ok ? ch : 0) * 16) * kPanelRows + (w * 16 + li) * 16 + lg * 4) / 4];
+        if constexpr (VEC)
+          pf[c] = v;
+        else
+          pf[c] = ok ? v : make_uint2(0u, 0u);
       }
     };
     // LDS store of the staged X groups: xs[kg][rhs][0..3] = 4 columns at one rhs
-    auto store = [&](int buf) {
+    auto store = [&](int buf, const uint2 (&xr)[GPT][4]) {
       for (int i = 0; i < GPT; ++i) {
         const int kg = sg + 16 * i;
         for (int r = 0; r < 4; ++r) {
@@ -251,20 +288,40 @@ __global__ void __launch_bounds__(256) k_panel_mm_bf16(
     v4f acc[4];
     for (int t = 0; t < 4; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
     if (nst > 0) {
-      load(0, pnext);
-      store(0);
+      load_ids(0, ci);
+      load_x(ci, xa);
+      load_ids(1, ci);
+      load_x(ci, xb);
+      load_ids(2, ci);
+      load_p(0, p0);
+      load_p(1, p1);
+      store(0, xa);
     }
     __syncthreads();
-    for (int64_t s = 0; s < nst; ++s) {
+    // One stage: compute from the panel fragments in pc while the loads of
+    // X(s+1), ids(s+2) and panel(s+2) (into pl) are in flight.  The stage
+    // loop is unrolled by the ring period (3) so each buffer keeps a fixed
+    // role: copying a register whose load is still pending would make the
+    // compiler wait for it and collapse the prefetch.
+    auto step = [&](int64_t s, const uint2 (&pc)[SC], uint2 (&pl)[SC], uint2 (&xl)[GPT][4],
+                    const uint2 (&xst)[GPT][4]) {
       const int buf = (int)(s & 1);
-      for (int c = 0; c < SC; ++c) pcur[c] = pnext[c];
-      if (s + 1 < nst) load(s + 1, pnext);
+      // VEC: loads for stages past the end are clamped and harmless, so
+      // they go out unconditionally (a join with a different number of
+      // pending loads would make the compiler wait conservatively)
+      if (VEC || s + 2 < nst) {
+        load_x(ci, xl);
+        load_ids(s + 3, ci);
+        load_p(s + 2, pl);
+      }
       for (int c = 0; c < SC; ++c) {
+        // chunks past the block's end hold clamped, unmasked loads: skip them
+        if (VEC && s * SC + c >= nch) break;
         v4s bfrag;
-        bfrag[0] = (short)(pcur[c].x & 0xffffu);
-        bfrag[1] = (short)(pcur[c].x >> 16);
-        bfrag[2] = (short)(pcur[c].y & 0xffffu);
-        bfrag[3] = (short)(pcur[c].y >> 16);
+        bfrag[0] = (short)(pc[c].x & 0xffffu);
+        bfrag[1] = (short)(pc[c].x >> 16);
+        bfrag[2] = (short)(pc[c].y & 0xffffu);
+        bfrag[3] = (short)(pc[c].y >> 16);
         for (int t = 0; t < 4; ++t) {
           // A fragment: X^T[rhs 16 t + li][columns 4 (4c + lg) .. +3]
           const uint2 av = xs[buf][(c * 4 + lg) * 64 + t * 16 + li];
@@ -276,9 +333,24 @@ __global__ void __launch_bounds__(256) k_panel_mm_bf16(
           acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(afrag, bfrag, acc[t], 0, 0, 0);
         }
       }
-      if (s + 1 < nst) store(buf ^ 1);
+      if (VEC || s + 1 < nst) store(buf ^ 1, xst);
       __syncthreads();
+    };
+    // unrolled by 6 = lcm of the panel ring (3) and the X ring (2)
+    int64_t s = 0;
+    for (; s + 6 <= nst; s += 6) {
+      step(s, p0, p2, xa, xb);
+      step(s + 1, p1, p0, xb, xa);
+      step(s + 2, p2, p1, xa, xb);
+      step(s + 3, p0, p2, xb, xa);
+      step(s + 4, p1, p0, xa, xb);
+      step(s + 5, p2, p1, xb, xa);
     }
+    if (s < nst) step(s, p0, p2, xa, xb);
+    if (s + 1 < nst) step(s + 1, p1, p0, xb, xa);
+    if (s + 2 < nst) step(s + 2, p2, p1, xa, xb);
+    if (s + 3 < nst) step(s + 3, p0, p2, xb, xa);
+    if (s + 4 < nst) step(s + 4, p1, p0, xa, xb);
     // epilogue: acc[t][q] = Y[row 16 w + li][rhs g0 + 16 t + 4 lg + q]
     const int64_t r = row_base + w * 16 + li;
     const bool row_ok = r < nrows;
@@ -348,7 +420,8 @@ __global__ void __launch_bounds__(256) k_panel_mm_bf16(
 // 16-byte load (4 consecutive FP32 values of its row).
 constexpr int kF64Stage = 2;
 
-__global__ void __launch_bounds__(256) k_panel_mm_f64(
+template <bool VEC>
+__global__ void __launch_bounds__(256, 3) k_panel_mm_f64(
     int64_t nrows, int64_t nblk, const int64_t *__restrict__ colptr,
     const int32_t *__restrict__ cols, const float *__restrict__ panel,
     const double *__restrict__ X, int64_t nrhs, const double *__restrict__ E,
@@ -363,23 +436,41 @@ __global__ void __launch_bounds__(256) k_panel_mm_f64(
   const int64_t nch = (colptr[b + 1] - c0) >> 4;
   const int64_t nst = (nch + SC - 1) / SC;
   const int64_t row_base = b * kPanelRows;
-  const float4 *P4 = reinterpret_cast<const float4 *>(panel);
+  const v4f *P4 = reinterpret_cast<const v4f *>(panel);
   const bool vec_x = (nrhs & 3) == 0;
   // staging role: columns sc + 16 i of the stage, right-hand sides 4 rq .. 4 rq + 3
   const int sc = tid >> 4, rq = tid & 15;
   for (int64_t g0 = 0; g0 < nrhs; g0 += 64) {
-    double2 xr[SC][2];
-    float4 pcur[SC], pnext[SC];
-    auto load = [&](int64_t s, float4 (&pf)[SC]) {
-      const int64_t rc = g0 + 4 * rq;
+    // same pipeline as k_panel_mm_bf16: X(s+2), ids(s+3), panel(s+2) in flight
+    v2d xa[SC][2], xb[SC][2];
+    int32_t ci[SC];
+    v4f p0[SC], p1[SC], p2[SC];
+    const int64_t rc = g0 + 4 * rq;
+    auto load_ids = [&](int64_t s, int32_t (&ci)[SC]) {
       for (int i = 0; i < SC; ++i) {
         const int64_t lc = s * NC + sc + 16 * i;
+        const bool ok = lc < nch * 16;
+        if constexpr (VEC) {
+          ci[i] = cols[c0 + (ok ? lc : 0)];
+        } else {
+          ci[i] = ok ? cols[c0 + lc] : -1;
+        }
+      }
+    };
+    auto load_x = [&](const int32_t (&ci)[SC], v2d (&xr)[SC][2]) {
+      for (int i = 0; i < SC; ++i) {
+        if constexpr (VEC) {
+          const double *src = X + (int64_t)ci[i] * nrhs + (rc < nrhs ? rc : 0);
+          xr[i][0] = *reinterpret_cast<const v2d *>(src);
+          xr[i][1] = *reinterpret_cast<const v2d *>(src + 2);
+          continue;
+        }
         double v[4] = {0.0, 0.0, 0.0, 0.0};
-        if (lc < nch * 16) {
-          const int64_t e = (int64_t)cols[c0 + lc] * nrhs + rc;
+        if (ci[i] >= 0) {
+          const int64_t e = (int64_t)ci[i] * nrhs + rc;
           if (vec_x && rc + 3 < nrhs) {
-            const double2 a0 = *reinterpret_cast<const double2 *>(X + e);
-            const double2 a1 = *reinterpret_cast<const double2 *>(X + e + 2);
+            const v2d a0 = *reinterpret_cast<const v2d *>(X + e);
+            const v2d a1 = *reinterpret_cast<const v2d *>(X + e + 2);
             v[0] = a0.x;
             v[1] = a0.y;
             v[2] = a1.x;
@@ -389,18 +480,25 @@ __global__ void __launch_bounds__(256) k_panel_mm_f64(
               if (rc + q < nrhs) v[q] = X[e + q];
           }
         }
-        xr[i][0] = double2{v[0], v[1]};
-        xr[i][1] = double2{v[2], v[3]};
-      }
-      for (int c = 0; c < SC; ++c) {
-        const int64_t ch = s * SC + c;
-        pf[c] = ch < nch ? P4[((c0 + ch * 16) * kPanelRows + (w * 16 + li) * 16 + lg * 4) / 4]
-                         : make_float4(0.f, 0.f, 0.f, 0.f);
+        xr[i][0] = v2d{v[0], v[1]};
+        xr[i][1] = v2d{v[2], v[3]};
       }
     };
-    auto store = [&](int buf) {
+    auto load_p = [&](int64_t s, v4f (&pf)[SC]) {
+      for (int c = 0; c < SC; ++c) {
+        const int64_t ch = s * SC + c;
+        const bool ok = ch < nch;
+        const v4f v =
+            P4[((c0 + (ok ? ch : 0) * 16) * kPanelRows + (w * 16 + li) * 16 + lg * 4) / 4];
+        if constexpr (VEC)
+          pf[c] = v;
+        else
+          pf[c] = ok ? v : v4f{0.f, 0.f, 0.f, 0.f};
+      }
+    };
+    auto store = [&](int buf, const v2d (&xr)[SC][2]) {
       for (int i = 0; i < SC; ++i) {
-        double2 *d = reinterpret_cast<double2 *>(&xs[buf][(sc + 16 * i) * 64 + 4 * rq]);
+        v2d *d = reinterpret_cast<v2d *>(&xs[buf][(sc + 16 * i) * 64 + 4 * rq]);
         d[0] = xr[i][0];
         d[1] = xr[i][1];
       }
@@ -408,16 +506,28 @@ __global__ void __launch_bounds__(256) k_panel_mm_f64(
     v4d acc[4];
     for (int t = 0; t < 4; ++t) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
     if (nst > 0) {
-      load(0, pnext);
-      store(0);
+      load_ids(0, ci);
+      load_x(ci, xa);
+      load_ids(1, ci);
+      load_x(ci, xb);
+      load_ids(2, ci);
+      load_p(0, p0);
+      load_p(1, p1);
+      store(0, xa);
     }
     __syncthreads();
-    for (int64_t s = 0; s < nst; ++s) {
+    // stage loop unrolled by the ring period, as in k_panel_mm_bf16
+    auto step = [&](int64_t s, const v4f (&pc)[SC], v4f (&pl)[SC], v2d (&xl)[SC][2],
+                    const v2d (&xst)[SC][2]) {
       const int buf = (int)(s & 1);
-      for (int c = 0; c < SC; ++c) pcur[c] = pnext[c];
-      if (s + 1 < nst) load(s + 1, pnext);
+      if (VEC || s + 2 < nst) {
+        load_x(ci, xl);
+        load_ids(s + 3, ci);
+        load_p(s + 2, pl);
+      }
       for (int c = 0; c < SC; ++c) {
-        const float pv[4] = {pcur[c].x, pcur[c].y, pcur[c].z, pcur[c].w};
+        if (VEC && s * SC + c >= nch) break;  // clamped, unmasked loads
+        const float pv[4] = {pc[c][0], pc[c][1], pc[c][2], pc[c][3]};
         for (int ks = 0; ks < 4; ++ks) {
           // B: P^T[column 4 lg + ks][row li];  A: X^T[rhs 16 t + li][column 4 lg + ks]
           const double bv = (double)pv[ks];
@@ -426,9 +536,24 @@ __global__ void __launch_bounds__(256) k_panel_mm_f64(
             acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(xrow[t * 16], bv, acc[t], 0, 0, 0);
         }
       }
-      if (s + 1 < nst) store(buf ^ 1);
+      if (VEC || s + 1 < nst) store(buf ^ 1, xst);
       __syncthreads();
+    };
+    // unrolled by 6 = lcm of the panel ring (3) and the X ring (2)
+    int64_t s = 0;
+    for (; s + 6 <= nst; s += 6) {
+      step(s, p0, p2, xa, xb);
+      step(s + 1, p1, p0, xb, xa);
+      step(s + 2, p2, p1, xa, xb);
+      step(s + 3, p0, p2, xb, xa);
+      step(s + 4, p1, p0, xa, xb);
+      step(s + 5, p2, p1, xb, xa);
     }
+    if (s < nst) step(s, p0, p2, xa, xb);
+    if (s + 1 < nst) step(s + 1, p1, p0, xb, xa);
+    if (s + 2 < nst) step(s + 2, p2, p1, xa, xb);
+    if (s + 3 < nst) step(s + 3, p0, p2, xb, xa);
+    if (s + 4 < nst) step(s + 4, p1, p0, xa, xb);
     // epilogue: 16x16x4 FP64 result layout, lane l item q = D[4 q + l/16][l%16]:
     // acc[t][q] = Y[row 16 w + li][rhs g0 + 16 t + 4 q + lg]
     const int64_t r = row_base + w * 16 + li;
@@ -483,6 +608,16 @@ extern "C" int flux_scan_i64(const int64_t *in, int64_t n, int64_t *out, int64_t
                              void *stream);
 
 static int64_t panel_lds_words(int64_t ncols) { return (ncols + 31) / 32; }
+
+// 16-column chunks per pipeline stage of the BF16 product: 8, or 4 when the
+// environment variable FLUX_PANEL_STAGE says 4 (a tuning knob).
+static int panel_stage() {
+  static const int v = [] {
+    const char *e = getenv("FLUX_PANEL_STAGE");
+    return (e && atoi(e) == 4) ? 4 : 8;
+  }();
+  return v;
+}
 
 extern "C" int flux_panel_count(int64_t nrows, int64_t ncols, const int64_t *indptr,
                                 const int32_t *indices, int64_t *counts, void *stream) {
@@ -551,9 +686,24 @@ extern "C" int flux_panel_mm_bf16(int64_t nrows, const int64_t *colptr, const in
   if (colmax)
     hipLaunchKernelGGL(k_zero_f64_p, dim3(ceil_div_u(nrhs, 256)), dim3(256), 0, st, colmax,
                        nrhs);
-  hipLaunchKernelGGL(k_panel_mm_bf16, dim3((unsigned)nblk), dim3(256), 0, st, nrows, nblk,
-                     colptr, cols, (const __hip_bfloat16 *)panel, (const __hip_bfloat16 *)X, nrhs,
-                     base, rho, rho_s, left, out, (__hip_bfloat16 *)xnext, prev, colmax);
+#define MMB(SCV, VECV)                                                                         \
+  hipLaunchKernelGGL((k_panel_mm_bf16<SCV, VECV>), dim3((unsigned)nblk), dim3(256), 0, st, nrows, \
+                     nblk, colptr, cols, (const __hip_bfloat16 *)panel,                            \
+                     (const __hip_bfloat16 *)X, nrhs, base, rho, rho_s, left, out,                 \
+                     (__hip_bfloat16 *)xnext, prev, colmax)
+  const bool vec = (nrhs & 3) == 0;
+  if (panel_stage() == 4) {
+    if (vec)
+      MMB(4, true);
+    else
+      MMB(4, false);
+  } else {
+    if (vec)
+      MMB(8, true);
+    else
+      MMB(8, false);
+  }
+#undef MMB
   FLUX_LAUNCH_CHECK();
   return 0;
 }
@@ -572,8 +722,12 @@ extern "C" int flux_panel_mm_f64(int64_t nrows, const int64_t *colptr, const int
   if (colmax)
     hipLaunchKernelGGL(k_zero_f64_p, dim3(ceil_div_u(nrhs, 256)), dim3(256), 0, st, colmax,
                        nrhs);
-  hipLaunchKernelGGL(k_panel_mm_f64, dim3((unsigned)nblk), dim3(256), 0, st, nrows, nblk, colptr,
-                     cols, panel, X, nrhs, E, B, rho, rho_s, left, out, colmax);
+  if ((nrhs & 3) == 0)
+    hipLaunchKernelGGL(k_panel_mm_f64<true>, dim3((unsigned)nblk), dim3(256), 0, st, nrows, nblk,
+                       colptr, cols, panel, X, nrhs, E, B, rho, rho_s, left, out, colmax);
+  else
+    hipLaunchKernelGGL(k_panel_mm_f64<false>, dim3((unsigned)nblk), dim3(256), 0, st, nrows,
+                       nblk, colptr, cols, panel, X, nrhs, E, B, rho, rho_s, left, out, colmax);
   FLUX_LAUNCH_CHECK();
   return 0;
 }
