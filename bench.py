@@ -49,7 +49,10 @@ def main():
     ap.add_argument("--fp64-suns", type=int, default=4)
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--min-size", type=int, default=4096)
-    ap.add_argument("--inner", type=int, default=4)
+    ap.add_argument("--inner", type=int, default=2)
+    ap.add_argument("--residual", choices=["int8", "fp64"], default="int8",
+                    help="refinement residual: int8-sliced panels on int8 MFMA "
+                         "(FP64-accurate) or FP32 panels on FP64 MFMA")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib", action="store_true",
                     help="end with a 512 MiB device copy (counter calibration)")
@@ -64,10 +67,9 @@ def main():
 
     import torch
     from fluxpy_amd import _kernels, _lib, meshes
-    from fluxpy_amd._panel import PanelMatrix, PanelRefinedSolver
     from fluxpy_amd.compressed_form_factors import CompressedFormFactorMatrix
     from fluxpy_amd.shape import CgalTrimeshShapeModel
-    from fluxpy_amd.solve import solve_radiosity, solve_radiosity_refined
+    from fluxpy_amd.solve import _panel_engine, solve_radiosity, solve_radiosity_refined
 
     torch.cuda.set_device(0)
     dev = torch.device("cuda", 0)
@@ -144,55 +146,90 @@ def main():
     # ---------------- BF16 + refinement, batched ----------------
     pure_near = op64.nfar == 0
     if pure_near:
-        P32 = PanelMatrix(near, "fp32")
-        Pbf = P32.sibling(near, "bf16")
-        Ei = op64.to_internal(E64)
-        rho_i = op64.to_internal(rho_d.reshape(-1, 1)).reshape(-1).contiguous()
-        solver = PanelRefinedSolver(Pbf, P32, nf, args.suns, rho_i)
-        log(f"[bench] panels: fill {P32.fill:.3f}, bf16 {Pbf.nbytes / 1e6:.0f} MB, "
-            f"fp32 {P32.nbytes / 1e6:.0f} MB")
+        # the panels are built on the first solve and kept on H
+        Pbf, Pres = _panel_engine(H, args.residual)
+        log(f"[bench] panels: fill {Pbf.fill:.3f}, bf16 {Pbf.nbytes / 1e6:.0f} MB, "
+            f"{args.residual} residual {Pres.nbytes / 1e6:.0f} MB")
 
-        def step():
-            Bi, info = solver.solve(Ei, rtol=1e-7, inner_iters=args.inner)
-            return op64.from_internal(Bi), info
+        def make_step(residual):
+            def step():
+                return solve_radiosity_refined(H, H, E64, rho_d, rtol=1e-7,
+                                               inner_iters=args.inner, return_info=True,
+                                               residual=residual)
+            return step
     else:
         opbf = H.device_operator("bf16")
         op32 = H.device_operator("fp32")
 
-        def step():
-            return solve_radiosity_refined(opbf, op32, E64, rho, rtol=1e-7,
-                                           inner_iters=args.inner, return_info=True)
+        def make_step(residual):
+            def step():
+                return solve_radiosity_refined(opbf, op32, E64, rho, rtol=1e-7,
+                                               inner_iters=args.inner, return_info=True)
+            return step
 
-    for _ in range(args.warmup):
-        B, info = step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        B, info = step()
-    torch.cuda.synchronize()
-    t_step = (time.perf_counter() - t0) / args.steps
+    def timed(step, steps, warmup):
+        for _ in range(warmup):
+            B, info = step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            B, info = step()
+        torch.cuda.synchronize()
+        t = (time.perf_counter() - t0) / steps
+        errs = [float((B[:, s] - B64[s]).abs().max() / B64[s].abs().max())
+                for s in range(ns64)]
+        return t, B, info, max(errs)
+
+    t_step, B, info, err = timed(make_step(args.residual), args.steps, args.warmup)
     sps = args.suns / t_step
-    errs = [float((B[:, s] - B64[s]).abs().max() / B64[s].abs().max()) for s in range(ns64)]
-    log(f"[bench] BF16+refine: {t_step * 1e3:.2f} ms/step, {sps:.0f} solves/s, "
-        f"rounds={info['rounds']} max rel err vs FP64 {max(errs):.2e}")
+    log(f"[bench] BF16+refine ({args.residual} residual): {t_step * 1e3:.2f} ms/step, "
+        f"{sps:.0f} solves/s, rounds={info['rounds']} max rel err vs FP64 {err:.2e}")
+    alt = None
+    if pure_near:
+        other = "fp64" if args.residual == "int8" else "int8"
+        t_alt, _, info_alt, err_alt = timed(make_step(other), 2, 1)
+        alt = {"residual": other, "solves_per_s": round(args.suns / t_alt, 2),
+               "rounds": info_alt["rounds"], "max_rel_err_vs_fp64": err_alt}
+        log(f"[bench] BF16+refine ({other} residual): {t_alt * 1e3:.2f} ms/step, "
+            f"{args.suns / t_alt:.0f} solves/s, rounds={info_alt['rounds']} "
+            f"max rel err vs FP64 {err_alt:.2e}")
 
     # ---------------- roofline of the dominant kernel ----------------
     k = args.suns
     reps = 20
+    res_us = None
     if pure_near:
         Xb = torch.zeros((nf, k), dtype=torch.bfloat16, device=dev)
         Xn = torch.empty_like(Xb)
         out = torch.empty((nf, k), dtype=torch.float32, device=dev)
         base = torch.zeros_like(out)
+        rho32 = rho_d.to(torch.float32)
         for _ in range(3):
-            Pbf.mm_bf16(Xb, out, base=base, rho=solver.rho32, xnext=Xn)
+            Pbf.mm_bf16(Xb, out, base=base, rho=rho32, xnext=Xn)
         torch.cuda.synchronize()
         ev0.record()
         for _ in range(reps):
-            Pbf.mm_bf16(Xb, out, base=base, rho=solver.rho32, xnext=Xn)
+            Pbf.mm_bf16(Xb, out, base=base, rho=rho32, xnext=Xn)
         ev1.record()
         torch.cuda.synchronize()
         kname = "k_panel_mm_bf16"
+        k_ms = ev0.elapsed_time(ev1) / reps
+        # the residual kernel beside it (X slicing included)
+        P8 = _panel_engine(H, "int8")[1]
+        X64 = torch.ones((nf, k), dtype=torch.float64, device=dev)
+        R64 = torch.empty_like(X64)
+        cmx = torch.empty(k, dtype=torch.float64, device=dev)
+        for _ in range(3):
+            P8.mm_f64(X64, R64, E=X64, B=X64, colmax=cmx, xscale=rho_d)
+        torch.cuda.synchronize()
+        ev0.record()
+        for _ in range(reps):
+            P8.mm_f64(X64, R64, E=X64, B=X64, colmax=cmx, xscale=rho_d)
+        ev1.record()
+        torch.cuda.synchronize()
+        res_us = ev0.elapsed_time(ev1) / reps * 1e3
+        log(f"[bench] int8-sliced residual (slice X + k_panel8_mm): {res_us:.1f} us")
+        del X64, R64
     else:
         ev0.record()
         for _ in range(reps):
@@ -200,7 +237,7 @@ def main():
         ev1.record()
         torch.cuda.synchronize()
         kname = "k_csr_rows<bf16> + far field"
-    k_ms = ev0.elapsed_time(ev1) / reps
+        k_ms = ev0.elapsed_time(ev1) / reps
     # algorithmic H-matvec bytes: near nnz (bf16 value + int32 index), indptr,
     # x in (bf16) and y out (fp32), plus the fused epilogue (base in, x next out)
     alg_bytes = nnz * (2 + 4) + 4 * (nf + 1) + k * nf * (2 + 4) + k * nf * (4 + 2)
@@ -248,7 +285,9 @@ def main():
         },
         "fp64_solves_per_s": round(fp64_sps, 2),
         "speedup_vs_fp64": round(sps / fp64_sps, 2),
-        "max_rel_err_vs_fp64": max(errs),
+        "max_rel_err_vs_fp64": err,
+        "residual": args.residual,
+        "other_residual": alt,
         "refine_rounds": info["rounds"],
         "mrays_per_s": round(mrays, 1),
         "rays": rays,
@@ -258,6 +297,7 @@ def main():
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
             "kernel": kname, "kernel_us": round(k_ms * 1e3, 2), "algorithmic_bytes": alg_bytes,
         },
+        "residual_kernel_us": None if res_us is None else round(res_us, 2),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -1,11 +1,14 @@
-"""Row-block panel form of a square near-field CSR (see csrc/panel.hip) and
-the batched mixed-precision radiosity solve built on it.
+"""Row-block panel form of a square near-field CSR (see csrc/panel.hip and
+csrc/panel8.hip) and the batched mixed-precision radiosity solve built on it.
 
 The panels hold the same nonzeros as the CSR (zeros fill the rest), so a
 panel product equals the CSR product up to summation order.  BF16 panels
-serve the inner sweeps; FP32 panels (the master values) serve the FP64
-residual of the refinement.
+serve the inner sweeps.  The residual of the refinement runs either on the
+int8-sliced panels (FP64-accurate products on int8 MFMA, the default) or on
+FP32 panels of the master values with FP64 MFMA.
 """
+import ctypes
+
 import numpy as np
 
 from . import _kernels
@@ -85,6 +88,69 @@ class PanelMatrix:
         return out
 
 
+class SlicedPanelMatrix:
+    """Int8-sliced panels of a square CSR: every value split into 5 signed
+    7-bit slices under a power-of-two scale per row, for FP64-accurate
+    products on int8 MFMA (see csrc/panel8.hip for the error bound).
+    Shares the column lists of ``like``, a PanelMatrix of the same CSR."""
+
+    ROWS = 64
+    QUAD_BYTES = 5 * 64 * 64
+
+    def __init__(self, csr, like):
+        t = _lib.require_gpu()
+        m, n = csr.shape
+        self.shape = (m, n)
+        self.values = "int8x5"
+        dev = csr.data.device
+        st = _lib.stream()
+        nblk = (m + self.ROWS - 1) // self.ROWS
+        self.colptr, self.cols = like.colptr, like.cols
+        self.qptr = t.empty(nblk + 1, dtype=t.int64, device=dev)
+        total, widest = ctypes.c_int64(), ctypes.c_int64()
+        _lib.call("flux_panel8_layout", nblk, self.colptr.data_ptr(), self.qptr.data_ptr(),
+                  ctypes.byref(total), ctypes.byref(widest), st)
+        self.quads = total.value
+        self.panel = t.empty(max(self.quads * self.QUAD_BYTES, 16), dtype=t.int8, device=dev)
+        self.rowexp = t.empty(max(m, 1), dtype=t.int32, device=dev)
+        _lib.call("flux_panel8_fill", m, n, csr.indptr.data_ptr(), csr.indices.data_ptr(),
+                  csr.data.data_ptr(), _lib.dtype_code(csr.data.dtype), self.qptr.data_ptr(),
+                  self.quads, self.rowexp.data_ptr(), self.panel.data_ptr(), st)
+        self.nnz = csr.nnz
+        self._work = None
+
+    @property
+    def nbytes(self):
+        return (self.panel.numel() + self.rowexp.numel() * 4 + self.qptr.numel() * 8
+                + self.cols.numel() * 4 + self.colptr.numel() * 8)
+
+    def workspace(self, k):
+        """X-slice scratch for k right-hand sides (grown on demand)."""
+        t = _lib.torch()
+        nb = int(_lib.load().flux_slice_x_bytes(self.shape[1], k))
+        if self._work is None or self._work.numel() < nb:
+            self._work = t.empty(nb, dtype=t.uint8, device=self.panel.device)
+        return self._work
+
+    def mm_f64(self, X, out, E=None, B=None, rho=None, rho_s=1.0, left=False, colmax=None,
+               xscale=None):
+        """out = P (s X), or with E: out = E + r (P s X) - B and colmax = max|out|
+        (r_i = rho_i if left; s_i = xscale_i, a tensor or scalar, or 1)."""
+        k = X.shape[1] if X.dim() == 2 else 1
+        work = self.workspace(k)
+        st = _lib.stream()
+        t = _lib.torch()
+        xs_t = xscale if isinstance(xscale, t.Tensor) else None
+        xs_s = 1.0 if xscale is None or xs_t is not None else float(xscale)
+        _lib.call("flux_slice_x", self.shape[1], k, X.data_ptr(), _lib.ptr(xs_t), xs_s,
+                  int(xscale is not None), work.data_ptr(), st)
+        _lib.call("flux_panel8_mm", self.shape[0], self.colptr.data_ptr(), self.cols.data_ptr(),
+                  self.qptr.data_ptr(), self.panel.data_ptr(), self.rowexp.data_ptr(), k,
+                  work.data_ptr(), _lib.ptr(E), _lib.ptr(B), _lib.ptr(rho), float(rho_s),
+                  int(left), out.data_ptr(), _lib.ptr(colmax), st)
+        return out
+
+
 def to_bf16_scaled(x32, rho=None, rho_s=1.0, scale=True, out=None):
     t = _lib.torch()
     out = t.empty(x32.shape, dtype=t.bfloat16, device=x32.device) if out is None else out
@@ -101,25 +167,20 @@ class PanelRefinedSolver:
 
     * inner: `inner_iters` Jacobi sweeps of the correction equation with the
       BF16 panels, FP32 vectors, FP32 accumulation, no host round trip;
-    * residual: r = E - (I - F rho) B with the FP32 master panels, FP64
-      vectors and FP64 accumulation (MFMA FP64);
+    * residual: r = E - (I - F rho) B with FP64 vectors, on ``P_hi``: a
+      SlicedPanelMatrix (int8 MFMA, FP64-accurate) or the FP32 master panels
+      (FP64 MFMA);
     * stop when max|r_c| <= rtol max|E_c| for every column c.
 
     All vectors live in the panels' (cluster) order.
     """
 
-    def __init__(self, P_lo, P_hi, n, k, rho, left=False):
+    def __init__(self, P_lo, P_hi, n, k, rho=1.0, left=False):
         t = _lib.require_gpu()
         dev = P_lo.panel.device
         self.P_lo, self.P_hi = P_lo, P_hi
         self.n, self.k, self.left = n, k, left
-        if isinstance(rho, t.Tensor):
-            self.rho64 = rho.to(t.float64).contiguous()
-            self.rho32 = _kernels.convert(self.rho64, t.float32)
-            self.rs = 1.0
-        else:
-            self.rho64 = self.rho32 = None
-            self.rs = float(rho)
+        self.set_rho(rho)
         f32, f64 = t.float32, t.float64
         self.B = t.empty((n, k), dtype=f64, device=dev)
         self.R = t.empty((n, k), dtype=f64, device=dev)
@@ -129,6 +190,17 @@ class PanelRefinedSolver:
         self.Xb = [t.empty((n, k), dtype=t.bfloat16, device=dev) for _ in range(2)]
         self.colmax = t.empty(k, dtype=f64, device=dev)
         self.vo = _kernels.VecOps(k)
+
+    def set_rho(self, rho):
+        """Albedo: an n-vector device tensor (panel order) or a scalar."""
+        t = _lib.torch()
+        if isinstance(rho, t.Tensor):
+            self.rho64 = rho.to(t.float64).reshape(-1).contiguous()
+            self.rho32 = _kernels.convert(self.rho64, t.float32)
+            self.rs = 1.0
+        else:
+            self.rho64 = self.rho32 = None
+            self.rs = float(rho)
 
     def solve(self, E, rtol=1e-7, inner_iters=4, max_rounds=30):
         """E: n x k FP64 device tensor in panel order.  Returns (B, info)."""
@@ -154,13 +226,14 @@ class PanelRefinedSolver:
             D = self.D[1 - cur]
             _kernels.axpby(None, _kernels.convert(D, t.float64), None, B)
             # residual R = E + F(rho B) - B  (right) / E + rho F B - B (left)
-            if self.left:
-                Xr = B
+            rs = self.rho64 if self.rho64 is not None else self.rs
+            if isinstance(self.P_hi, SlicedPanelMatrix):
+                self.P_hi.mm_f64(B, R, E=E, B=B, rho=self.rho64, rho_s=self.rs, left=self.left,
+                                 colmax=self.colmax, xscale=None if self.left else rs)
             else:
-                Xr = _kernels.rowscale(self.rho64 if self.rho64 is not None else self.rs, B,
-                                       out=self.Xr)
-            self.P_hi.mm_f64(Xr, R, E=E, B=B, rho=self.rho64, rho_s=self.rs, left=self.left,
-                             colmax=self.colmax)
+                Xr = B if self.left else _kernels.rowscale(rs, B, out=self.Xr)
+                self.P_hi.mm_f64(Xr, R, E=E, B=B, rho=self.rho64, rho_s=self.rs,
+                                 left=self.left, colmax=self.colmax)
             rmax = self.colmax.cpu().numpy()
             hist.append(float((rmax / np.where(emax > 0, emax, 1.0)).max()))
             if np.all(rmax <= thresh) or rounds >= max_rounds:

@@ -26,7 +26,7 @@
 // gather mostly the same X rows, share an L2.
 #include <hip/hip_bf16.h>
 
-#include "common.h"
+#include "panel.h"
 
 namespace flux {
 
@@ -34,8 +34,6 @@ typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef double v2d __attribute__((ext_vector_type(2)));
-
-constexpr int kPanelRows = 64;
 
 // ---------------------------------------------------------------------------
 // builder
@@ -85,41 +83,13 @@ __global__ void __launch_bounds__(256) k_panel_fill(int64_t nrows, int64_t ncols
                                                     int32_t *__restrict__ cols,
                                                     TP *__restrict__ panel) {
   extern __shared__ uint32_t sm[];
+  __shared__ uint32_t part[256];
   const int64_t b = blockIdx.x;
   const int64_t nw = (ncols + 31) / 32;
   uint32_t *bm = sm;
   uint32_t *pre = sm + nw;
-  for (int64_t w = threadIdx.x; w < nw; w += blockDim.x) bm[w] = 0u;
-  __syncthreads();
   const int64_t r0 = b * kPanelRows, r1 = min(nrows, r0 + kPanelRows);
-  for (int64_t k = indptr[r0] + threadIdx.x; k < indptr[r1]; k += blockDim.x) {
-    const int32_t j = indices[k];
-    atomicOr(&bm[j >> 5], 1u << (j & 31));
-  }
-  __syncthreads();
-  // exclusive prefix of word popcounts: each thread scans a contiguous run
-  __shared__ uint32_t part[256];
-  const int64_t per = (nw + blockDim.x - 1) / blockDim.x;
-  const int64_t w0 = threadIdx.x * per, w1 = min(nw, w0 + per);
-  uint32_t run = 0;
-  for (int64_t w = w0; w < w1; ++w) run += __popc(bm[w]);
-  part[threadIdx.x] = run;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t acc = 0;
-    for (int t = 0; t < (int)blockDim.x; ++t) {
-      const uint32_t v = part[t];
-      part[t] = acc;
-      acc += v;
-    }
-  }
-  __syncthreads();
-  run = part[threadIdx.x];
-  for (int64_t w = w0; w < w1; ++w) {
-    pre[w] = run;
-    run += __popc(bm[w]);
-  }
-  __syncthreads();
+  const int64_t ndist = block_columns(indptr, indices, r0, r1, nw, bm, pre, part);
   const int64_t base = colptr[b];
   const int64_t npad = colptr[b + 1] - base;
   // column list
@@ -133,14 +103,11 @@ __global__ void __launch_bounds__(256) k_panel_fill(int64_t nrows, int64_t ncols
       ++pos;
     }
   }
-  const int64_t ndist = nw > 0 ? (int64_t)pre[nw - 1] + __popc(bm[nw - 1]) : 0;
   for (int64_t p = ndist + threadIdx.x; p < npad; p += blockDim.x) cols[base + p] = 0;
   // values
   for (int64_t r = r0; r < r1; ++r) {
     for (int64_t k = indptr[r] + threadIdx.x; k < indptr[r + 1]; k += blockDim.x) {
-      const int32_t j = indices[k];
-      const uint32_t w = (uint32_t)j >> 5;
-      const uint32_t pos = pre[w] + __popc(bm[w] & ((1u << (j & 31)) - 1u));
+      const uint32_t pos = column_pos(bm, pre, indices[k]);
       const int64_t chunk = pos >> 4, kk = pos & 15;
       store_val(panel + (base + chunk * 16) * kPanelRows + (r - r0) * 16 + kk,
                 (double)data[k]);
@@ -162,19 +129,6 @@ __global__ void k_zero_any(T *p, int64_t n) {
 // base == NULL -> out = acc; xnext == NULL -> not written;
 // colmax != NULL -> colmax[c] = max_i |out[i, c] - prev[i, c]|.
 // ---------------------------------------------------------------------------
-// Row block of workgroup `wg`: XCD x = wg % 8 (the dispatch order) owns the
-// contiguous run of blocks [x q + min(x, r), ...), q = nblk / 8, r = nblk % 8.
-__device__ __forceinline__ int64_t xcd_block(int64_t wg, int64_t nblk) {
-  const int64_t x = wg & 7, j = wg >> 3, q = nblk >> 3, r = nblk & 7;
-  return x * q + min(x, r) + j;
-}
-
-// Max over the 16 lanes that share lane / 16 (bits 0-3 of the lane id).
-__device__ __forceinline__ double max16(double v) {
-  for (int m = 1; m < 16; m <<= 1) v = fmax(v, __shfl_xor(v, m, 64));
-  return v;
-}
-
 // SC chunks of 16 columns per stage.  LDS block per stage (bf16):
 // xs[kg][rhs][4], kg = group of 4 consecutive columns of the stage, so the
 // A fragment of a lane (4 consecutive columns at one right-hand side) is one
@@ -607,14 +561,13 @@ using namespace flux;
 extern "C" int flux_scan_i64(const int64_t *in, int64_t n, int64_t *out, int64_t *total_host,
                              void *stream);
 
-static int64_t panel_lds_words(int64_t ncols) { return (ncols + 31) / 32; }
-
-// 16-column chunks per pipeline stage of the BF16 product: 8, or 4 when the
-// environment variable FLUX_PANEL_STAGE says 4 (a tuning knob).
+// 16-column chunks per pipeline stage of the BF16 product: 4 (three waves
+// per SIMD; measured faster on the 41k-facet terrain), or 8 when the
+// environment variable FLUX_PANEL_STAGE says 8 (a tuning knob).
 static int panel_stage() {
   static const int v = [] {
     const char *e = getenv("FLUX_PANEL_STAGE");
-    return (e && atoi(e) == 4) ? 4 : 8;
+    return (e && atoi(e) == 8) ? 8 : 4;
   }();
   return v;
 }

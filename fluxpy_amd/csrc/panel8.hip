@@ -1,0 +1,523 @@
+// Int8-sliced panels: the refinement residual r = E + s o (P X) - B on int8
+// MFMA with FP64-level accuracy (the Ozaki scheme: Ozaki, Ogita, Oishi &
+// Rump, "Error-free transformations of matrix multiplication by using fast
+// routines of matrix multiplication and its applications", Numer. Algorithms
+// 59, 2012; here with fixed power-of-two slicing).
+//
+// Slicing.  A real v with |v| < 2^e is written as
+//   v = 2^(e-6) (a_0 + 2^-7 a_1 + 2^-14 a_2 + 2^-21 a_3 + 2^-28 a_4) + t,
+//   a_i = integers in [-64, 64],  |t| <= 2^(e-35),
+// by repeated round-to-nearest (every step is exact in FP64).  Panel values
+// take e = e_r, the exponent of their row's largest |value| (so entries of
+// at least 2^-11 of the row maximum keep all 24 bits of an FP32 value);
+// right-hand side k takes e = e_k, the exponent of its largest |x|.
+//
+// Product.  (P X)[r, k] = 2^(e_r + e_k - 12) sum_d 2^-7d S_d[r, k] with
+//   S_d = sum_{i + j = d, i, j < 5} sum_c a_i[r, c] b_j[c, k]   (d = 0..4),
+// the 15 slice pairs of weight >= 2^-28 relative to the leading pair.  Each
+// pair product is one v_mfma_i32_16x16x64_i8 per 64 columns, exact in int32
+// (|S_d| <= 5 * 64 * 64 * ncols_b < 2^31 for ncols_b <= 104,832 distinct
+// columns per row block, checked when the layout is built); the five sums
+// are combined in FP64.  Per (r, k) the error against the exact product is
+// below 2^(e_r + e_k) (2^-35 sum_c |x_c| / 2^e_k + 2^-35 sum_c |p_rc| / 2^e_r
+// + 2^-33 n_c) with n_c the block's column count: about 1e-9 of
+// max|p_r.| max|x_.k| per column, far below the 1e-7 stopping threshold of
+// the refinement.
+//
+// Layout (shares colptr / cols with the BF16 and FP32 panels).  Row block b
+// has nq_b = ceil(nch_b / 4) quads of 64 columns (4 chunks of 16, the MFMA
+// K); quad q of block b is kSlices consecutive 64 x 64 int8 tiles, slice i
+// of row rr at byte (qptr[b] + q) * 20480 + i * 4096 + rr * 64 + column in
+// quad.  Columns past the block's distinct columns are zero.
+// X slices (written per residual by flux_slice_x): right-hand sides in groups
+// of 64; facet c, group h: 320 bytes at (c * nh + h) * 320, slice j of rhs
+// 64 h + m at byte j * 64 + m.
+#include "panel.h"
+
+namespace flux {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+constexpr int kSlices = 5;
+constexpr int kQuadCols = 64;
+constexpr int kSliceTile = kPanelRows * kQuadCols;    // 4096 bytes
+constexpr int kQuadBytes = kSlices * kSliceTile;      // 20480 bytes
+constexpr int kGroup = 64;                            // right-hand sides per workgroup
+constexpr int kXRow = kSlices * kGroup;               // 320 bytes per (facet, group)
+// LDS tile of one stage, in 4-byte words: slice j, right-hand side row
+// (0..63), columns 4 cw .. 4 cw + 3 at
+//   j * kXsSlice + row * 16 + 4 ((cw >> 2) ^ swz(row)) + (cw & 3),
+//   swz(row) = ((row >> 1) ^ (row >> 3)) & 3:
+// the 16-byte groups of a row are XOR-swizzled so that the A-fragment reads
+// (16 consecutive rows, one group each) cover every bank exactly once per
+// phase, for 32 banks (8-lane phases) and for 64 banks (16-lane phases);
+// slices are skewed by 16 words, which keeps the transposed 4-byte stores
+// within 4 lanes per bank.
+constexpr int kXsSlice = kGroup * 16 + 16;            // words per slice (1040)
+
+__device__ __forceinline__ int xs_swz(int row) { return ((row >> 1) ^ (row >> 3)) & 3; }
+constexpr int64_t kMaxBlockCols = 104832;             // int32 bound of the sums S_d
+
+// a_0..a_4 of v with |v| < 2^e (see the header).
+__device__ __forceinline__ void slice5(double v, int e, int8_t a[kSlices]) {
+  double t = ldexp(v, 6 - e);
+  for (int i = 0; i < kSlices; ++i) {
+    const double q = rint(t);
+    a[i] = (int8_t)(int)q;
+    t = (t - q) * 128.0;
+  }
+}
+
+// frexp exponent of m > 0 (m < 2^e), 0 for m == 0.
+__device__ __forceinline__ int exp_of(double m) {
+  int e = 0;
+  if (m > 0.0) frexp(m, &e);
+  return e;
+}
+
+// ---------------------------------------------------------------------------
+// builder
+// ---------------------------------------------------------------------------
+
+// qptr[b] = sum of ceil(nch_b' / 4) over b' < b; info[0] = total quads,
+// info[1] = largest column count of one block (one workgroup of 1024).
+__global__ void __launch_bounds__(1024) k_panel8_layout(int64_t nblk,
+                                                        const int64_t *__restrict__ colptr,
+                                                        int64_t *__restrict__ qptr,
+                                                        int64_t *__restrict__ info) {
+  __shared__ int64_t part[1024], mx[1024];
+  const int64_t per = (nblk + 1023) / 1024;
+  const int64_t b0 = threadIdx.x * per, b1 = min(nblk, b0 + per);
+  int64_t run = 0, m = 0;
+  for (int64_t b = b0; b < b1; ++b) {
+    const int64_t nc = colptr[b + 1] - colptr[b];
+    run += ((nc >> 4) + 3) >> 2;
+    m = max(m, nc);
+  }
+  part[threadIdx.x] = run;
+  mx[threadIdx.x] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t acc = 0, mm = 0;
+    for (int t = 0; t < 1024; ++t) {
+      const int64_t v = part[t];
+      part[t] = acc;
+      acc += v;
+      mm = max(mm, mx[t]);
+    }
+    qptr[nblk] = acc;
+    info[0] = acc;
+    info[1] = mm;
+  }
+  __syncthreads();
+  run = part[threadIdx.x];
+  for (int64_t b = b0; b < b1; ++b) {
+    qptr[b] = run;
+    run += (((colptr[b + 1] - colptr[b]) >> 4) + 3) >> 2;
+  }
+}
+
+// rowexp[r] = exponent of max_c |P[r, c]| (one wave per row).
+template <typename TA>
+__global__ void __launch_bounds__(256) k_row_exp(int64_t nrows,
+                                                 const int64_t *__restrict__ indptr,
+                                                 const TA *__restrict__ data,
+                                                 int32_t *__restrict__ rowexp) {
+  const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (r >= nrows) return;  // uniform per wave
+  double m = 0.0;
+  for (int64_t k = indptr[r] + lane; k < indptr[r + 1]; k += 64)
+    m = fmax(m, fabs((double)data[k]));
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  if (lane == 0) rowexp[r] = exp_of(m);
+}
+
+// Slices of every value into its quad tile (panel8 zeroed beforehand).
+template <typename TA>
+__global__ void __launch_bounds__(256) k_panel8_fill(int64_t nrows, int64_t ncols,
+                                                     const int64_t *__restrict__ indptr,
+                                                     const int32_t *__restrict__ indices,
+                                                     const TA *__restrict__ data,
+                                                     const int64_t *__restrict__ qptr,
+                                                     const int32_t *__restrict__ rowexp,
+                                                     int8_t *__restrict__ panel8) {
+  extern __shared__ uint32_t sm[];
+  __shared__ uint32_t part[256];
+  const int64_t b = blockIdx.x;
+  const int64_t nw = (ncols + 31) / 32;
+  uint32_t *bm = sm;
+  uint32_t *pre = sm + nw;
+  const int64_t r0 = b * kPanelRows, r1 = min(nrows, r0 + kPanelRows);
+  block_columns(indptr, indices, r0, r1, nw, bm, pre, part);
+  int8_t *base = panel8 + qptr[b] * (int64_t)kQuadBytes;
+  for (int64_t r = r0; r < r1; ++r) {
+    const int e = rowexp[r];
+    for (int64_t k = indptr[r] + threadIdx.x; k < indptr[r + 1]; k += blockDim.x) {
+      const uint32_t pos = column_pos(bm, pre, indices[k]);
+      int8_t a[kSlices];
+      slice5((double)data[k], e, a);
+      int8_t *dst = base + (int64_t)(pos >> 6) * kQuadBytes + (r - r0) * kQuadCols + (pos & 63);
+      for (int i = 0; i < kSlices; ++i) dst[i * kSliceTile] = a[i];
+    }
+  }
+}
+
+__global__ void k_zero_u32(uint32_t *p, int64_t n) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// X slices
+// ---------------------------------------------------------------------------
+
+// cmax[k] = max_i |s_i x[i, k]| as the bit pattern of a non-negative double
+// (cmax zeroed beforehand).  Block (64, 4): rhs 64 blockIdx.y + threadIdx.x.
+__global__ void __launch_bounds__(256) k_colmax_scaled(int64_t n, int64_t nrhs,
+                                                       const double *__restrict__ x,
+                                                       const double *__restrict__ rho,
+                                                       double rho_s, int scale,
+                                                       unsigned long long *__restrict__ cmax) {
+  __shared__ double red[4][64];
+  const int64_t k = (int64_t)blockIdx.y * 64 + threadIdx.x;
+  double m = 0.0;
+  if (k < nrhs) {
+    for (int64_t i = (int64_t)blockIdx.x * 4 + threadIdx.y; i < n; i += (int64_t)gridDim.x * 4) {
+      const double s = scale ? (rho ? rho[i] : rho_s) : 1.0;
+      m = fmax(m, fabs(s * x[i * nrhs + k]));
+    }
+  }
+  red[threadIdx.y][threadIdx.x] = m;
+  __syncthreads();
+  if (threadIdx.y == 0 && k < nrhs) {
+    m = fmax(fmax(red[0][threadIdx.x], red[1][threadIdx.x]),
+             fmax(red[2][threadIdx.x], red[3][threadIdx.x]));
+    atomicMax(cmax + k, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+// x8 / xexp from s o x (rhs padded to nh * 32 with zeros).
+__global__ void __launch_bounds__(256) k_slice_x(int64_t n, int64_t nrhs, int nh,
+                                                 const double *__restrict__ x,
+                                                 const double *__restrict__ rho, double rho_s,
+                                                 int scale,
+                                                 const unsigned long long *__restrict__ cmax,
+                                                 int32_t *__restrict__ xexp,
+                                                 int8_t *__restrict__ x8) {
+  const int64_t np = (int64_t)nh * kGroup;
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= n * np) return;
+  const int64_t i = id / np, k = id - i * np;
+  int e = 0;
+  double v = 0.0;
+  if (k < nrhs) {
+    e = exp_of(__longlong_as_double((long long)cmax[k]));
+    const double s = scale ? (rho ? rho[i] : rho_s) : 1.0;
+    v = s * x[i * nrhs + k];
+  }
+  if (i == 0) xexp[k] = e;
+  int8_t a[kSlices];
+  slice5(v, e, a);
+  const int h = (int)(k / kGroup), m = (int)(k % kGroup);
+  int8_t *dst = x8 + (i * nh + h) * (int64_t)kXRow + m;
+  for (int j = 0; j < kSlices; ++j) dst[j * kGroup] = a[j];
+}
+
+// ---------------------------------------------------------------------------
+// product: out = E + s o (P X) - B, colmax[k] = max_i |out[i, k]|
+//          (E == NULL: out = P X)
+// ---------------------------------------------------------------------------
+// Workgroup = (row block b, right-hand-side group h of 64), 4 waves; wave w
+// owns rows 16 w .. 16 w + 15.  The MFMA computes S_d^T = X_j^T P_i^T: A = X
+// slice j (M = 16 right-hand sides, K = 64 columns of the quad, staged in
+// LDS), B = panel slice i (N = the wave's 16 rows, straight from HBM into
+// registers: lane (row li, group lg) holds the 16 columns 16 lg .. 16 lg + 15
+// of the quad, i.e. chunk 4 q + lg, which is also what lane group lg of the
+// A operand holds).
+// Pipeline per stage (one quad) s: issue X(s+1), ids(s+3), panel(s+2);
+// MFMAs of stage s; transpose X(s+1) into LDS; store ids(s+3); one barrier.
+// Loads return in issue order, so the wait for X(s+1) at the end of stage s
+// also covers panel(s+1), issued after X(s) a stage earlier: the panel
+// stream (HBM) gets about two stages of latency, the X gathers (mostly
+// cache hits) one.  Registers: X of one stage, panel fragments of three
+// (p0 / p1 / p2); the stage loop is unrolled by the ring period.
+// X staging: 1280 units of (4 columns x 4 rhs x 1 slice) per stage, unit
+// u = tid + 256 m (m < 5): column group u / 80, slice (u % 80) / 16, rhs
+// group u % 16 (consecutive lanes read the 320 contiguous bytes of a column).
+__device__ __forceinline__ void transpose4x4(const uint32_t (&wi)[4], uint32_t (&wo)[4]) {
+  // wi[c] = bytes (rhs 0..3) of column c  ->  wo[r] = bytes (column 0..3) of rhs r
+  const uint32_t t0 = __builtin_amdgcn_perm(wi[1], wi[0], 0x05010400u);  // c0r0 c1r0 c0r1 c1r1
+  const uint32_t t1 = __builtin_amdgcn_perm(wi[1], wi[0], 0x07030602u);  // c0r2 c1r2 c0r3 c1r3
+  const uint32_t t2 = __builtin_amdgcn_perm(wi[3], wi[2], 0x05010400u);
+  const uint32_t t3 = __builtin_amdgcn_perm(wi[3], wi[2], 0x07030602u);
+  wo[0] = __builtin_amdgcn_perm(t2, t0, 0x05040100u);
+  wo[1] = __builtin_amdgcn_perm(t2, t0, 0x07060302u);
+  wo[2] = __builtin_amdgcn_perm(t3, t1, 0x05040100u);
+  wo[3] = __builtin_amdgcn_perm(t3, t1, 0x07060302u);
+}
+
+constexpr int kUnits = 5;  // X staging units per thread and stage
+
+__global__ void __launch_bounds__(256, 2) k_panel8_mm(
+    int64_t nrows, int64_t nblk, int nh, const int64_t *__restrict__ colptr,
+    const int32_t *__restrict__ cols, const int64_t *__restrict__ qptr,
+    const int8_t *__restrict__ panel8, const int32_t *__restrict__ rowexp,
+    const int8_t *__restrict__ x8, const int32_t *__restrict__ xexp, int64_t nrhs,
+    const double *__restrict__ E, const double *__restrict__ B, const double *__restrict__ rho,
+    double rho_s, int left, double *__restrict__ out, double *__restrict__ colmax) {
+  __shared__ __align__(16) uint32_t xs[2][kSlices * kXsSlice];
+  __shared__ __align__(16) int32_t ids[4][kQuadCols];
+  const int64_t item = xcd_block(blockIdx.x, nblk * nh);
+  const int64_t b = item / nh;
+  const int h = (int)(item - b * nh);
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int li = lane & 15, lg = lane >> 4;
+  const int64_t c0 = colptr[b];
+  const int64_t ncl = colptr[b + 1] - c0;  // columns (multiple of 16)
+  const int64_t nq = ((ncl >> 4) + 3) >> 2;
+  const int64_t row_base = b * kPanelRows;
+  const v4i *pb = reinterpret_cast<const v4i *>(panel8 + qptr[b] * (int64_t)kQuadBytes +
+                                                (16 * w + li) * kQuadCols + 16 * lg);
+  const int8_t *xg = x8 + h * (int64_t)kXRow;
+  const int64_t xstride = (int64_t)nh * kXRow;
+  v4i acc[kSlices][4];
+  for (int d = 0; d < kSlices; ++d)
+    for (int t = 0; t < 4; ++t) acc[d][t] = v4i{0, 0, 0, 0};
+  if (nq > 0) {
+    // every load address is clamped into the block, so nothing is masked
+    auto load_id = [&](int64_t s) -> int32_t {
+      return cols[c0 + min(s * kQuadCols + lane, ncl - 1)];
+    };
+    auto load_x = [&](int64_t s, uint32_t (&xr)[kUnits][4]) {
+      const int32_t *id = ids[s & 3];
+      for (int m = 0; m < kUnits; ++m) {
+        const int u = tid + 256 * m;
+        const int cg = u / 80, jr = u - cg * 80;
+        const int j = jr >> 4, rg = jr & 15;
+        const v4i cid = *reinterpret_cast<const v4i *>(id + 4 * cg);
+        for (int c = 0; c < 4; ++c)
+          xr[m][c] = *reinterpret_cast<const uint32_t *>(xg + (int64_t)cid[c] * xstride +
+                                                         j * kGroup + 4 * rg);
+      }
+    };
+    auto store_x = [&](int buf, const uint32_t (&xr)[kUnits][4]) {
+      for (int m = 0; m < kUnits; ++m) {
+        const int u = tid + 256 * m;
+        const int cg = u / 80, jr = u - cg * 80;
+        const int j = jr >> 4, rg = jr & 15;
+        uint32_t wo[4];
+        transpose4x4(xr[m], wo);
+        uint32_t *d = &xs[buf][j * kXsSlice + (cg & 3)];
+        for (int r = 0; r < 4; ++r) {
+          const int row = 4 * rg + r;
+          d[row * 16 + 4 * ((cg >> 2) ^ xs_swz(row))] = wo[r];
+        }
+      }
+    };
+    auto load_p = [&](int64_t s, v4i (&pf)[kSlices]) {
+      const v4i *p = pb + min(s, nq - 1) * (kQuadBytes / 16);
+      for (int i = 0; i < kSlices; ++i) pf[i] = p[i * (kSliceTile / 16)];
+    };
+    uint32_t xr[kUnits][4];
+    v4i p0[kSlices], p1[kSlices], p2[kSlices];
+    // prologue: ids(0..2) to LDS (every wave stores the same ids, so no
+    // branch sits next to a load), X(0) to LDS, panel(0), panel(1) in flight
+    ids[0][lane] = load_id(0);
+    ids[1][lane] = load_id(1);
+    ids[2][lane] = load_id(2);
+    __syncthreads();
+    load_x(0, xr);
+    load_p(0, p0);
+    load_p(1, p1);
+    store_x(0, xr);
+    __syncthreads();
+    auto step = [&](int64_t s, const v4i (&pc)[kSlices], v4i (&pl)[kSlices]) {
+      const int buf = (int)(s & 1);
+      load_x(s + 1, xr);
+      const int32_t idn = load_id(s + 3);
+      load_p(s + 2, pl);
+      // keep every load of the stage ahead of its MFMAs
+      __builtin_amdgcn_sched_barrier(0);
+      const uint32_t *xb_ = xs[buf];
+      for (int t = 0; t < 4; ++t) {
+        const int row = 16 * t + li;
+        const uint32_t *xq = xb_ + row * 16 + 4 * (lg ^ xs_swz(row));
+        for (int j = 0; j < kSlices; ++j) {
+          const v4i a = *reinterpret_cast<const v4i *>(xq + j * kXsSlice);
+          for (int i = 0; i + j < kSlices; ++i)
+            acc[i + j][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, pc[i], acc[i + j][t], 0, 0, 0);
+        }
+      }
+      store_x(buf ^ 1, xr);
+      ids[(s + 3) & 3][lane] = idn;
+      __syncthreads();
+    };
+    int64_t s = 0;
+    for (; s + 3 <= nq; s += 3) {
+      step(s, p0, p2);
+      step(s + 1, p1, p0);
+      step(s + 2, p2, p1);
+    }
+    if (s < nq) step(s, p0, p2);
+    if (s + 1 < nq) step(s + 1, p1, p0);
+  }
+  // epilogue: lane l item q of tile t = S^T[rhs 16 t + 4 lg + q][row li]
+  const int64_t r = row_base + 16 * w + li;
+  const bool row_ok = r < nrows;
+  const int er = row_ok ? rowexp[r] : 0;
+  const double rr = row_ok ? (rho ? rho[r] : rho_s) : 0.0;
+  for (int t = 0; t < 4; ++t) {
+    const int64_t k0 = (int64_t)h * kGroup + 16 * t + 4 * lg;
+    double y[4], cm[4];
+    for (int q = 0; q < 4; ++q) {
+      double v = 0.0;
+      for (int d = kSlices - 1; d >= 0; --d) v += ldexp((double)acc[d][t][q], -7 * d);
+      const int64_t k = k0 + q;
+      y[q] = k < nrhs ? ldexp(v, er + xexp[k] - 12) : 0.0;
+      cm[q] = 0.0;
+    }
+    if (row_ok) {
+      const int64_t o = r * nrhs + k0;
+      if (E) {
+        const double sc = left ? rr : 1.0;
+        for (int q = 0; q < 4; ++q) {
+          if (k0 + q < nrhs) {
+            y[q] = E[o + q] + sc * y[q] - B[o + q];
+            cm[q] = fabs(y[q]);
+          }
+        }
+      }
+      for (int q = 0; q < 4; ++q)
+        if (k0 + q < nrhs) out[o + q] = y[q];
+    }
+    if (colmax && E) {
+      for (int q = 0; q < 4; ++q) {
+        const double m = max16(cm[q]);
+        if (li == 0 && k0 + q < nrhs)
+          atomicMax((unsigned long long *)(colmax + k0 + q),
+                    (unsigned long long)__double_as_longlong(m));
+      }
+    }
+  }
+}
+
+}  // namespace flux
+
+using namespace flux;
+
+extern "C" int flux_panel8_layout(int64_t nblk, const int64_t *colptr, int64_t *qptr,
+                                  int64_t *total_quads, int64_t *max_cols, void *stream) {
+  FLUX_CHECK(nblk >= 0, "flux_panel8_layout: negative block count");
+  FLUX_CHECK(colptr && qptr && total_quads && max_cols, "flux_panel8_layout: NULL pointer");
+  hipStream_t st = as_stream(stream);
+  int64_t *info = nullptr;
+  FLUX_TRY_HIP(hipMallocAsync((void **)&info, 2 * sizeof(int64_t), st));
+  hipLaunchKernelGGL(k_panel8_layout, dim3(1), dim3(1024), 0, st, nblk, colptr, qptr, info);
+  int64_t h[2] = {0, 0};
+  hipError_t e1 = hipGetLastError();
+  hipError_t e2 = e1 == hipSuccess ? hipMemcpyAsync(h, info, sizeof h, hipMemcpyDeviceToHost, st)
+                                   : e1;
+  hipError_t e3 = hipFreeAsync(info, st);
+  hipError_t e4 = hipStreamSynchronize(st);
+  FLUX_TRY_HIP(e2);
+  FLUX_TRY_HIP(e3);
+  FLUX_TRY_HIP(e4);
+  *total_quads = h[0];
+  *max_cols = h[1];
+  FLUX_CHECK(h[1] <= kMaxBlockCols,
+             "flux_panel8_layout: a row block touches %lld columns; int8-sliced panels hold "
+             "at most %lld per block",
+             (long long)h[1], (long long)kMaxBlockCols);
+  return 0;
+}
+
+extern "C" int flux_panel8_fill(int64_t nrows, int64_t ncols, const int64_t *indptr,
+                                const int32_t *indices, const void *data, int dtype,
+                                const int64_t *qptr, int64_t total_quads, int32_t *rowexp,
+                                int8_t *panel8, void *stream) {
+  FLUX_CHECK(nrows >= 0 && ncols >= 0 && total_quads >= 0, "flux_panel8_fill: bad sizes");
+  FLUX_CHECK(2 * panel_lds_words(ncols) * 4 <= 150 * 1024,
+             "flux_panel8_fill: %lld columns exceed the LDS bitmap limit", (long long)ncols);
+  const int64_t nblk = (nrows + kPanelRows - 1) / kPanelRows;
+  if (nblk == 0) return 0;
+  FLUX_CHECK(indptr && indices && data && qptr && rowexp && panel8,
+             "flux_panel8_fill: NULL array");
+  FLUX_CHECK(dtype == FLUX_F32 || dtype == FLUX_F64, "flux_panel8_fill: values must be F32/F64");
+  hipStream_t st = as_stream(stream);
+  const int64_t nword = total_quads * (kQuadBytes / 4);
+  if (nword > 0)
+    hipLaunchKernelGGL(k_zero_u32, dim3(ceil_div_u(nword, 256)), dim3(256), 0, st,
+                       (uint32_t *)panel8, nword);
+  const size_t lds = 2 * panel_lds_words(ncols) * 4;
+  if (dtype == FLUX_F32) {
+    hipLaunchKernelGGL(k_row_exp<float>, dim3(ceil_div_u(nrows, 4)), dim3(256), 0, st, nrows,
+                       indptr, (const float *)data, rowexp);
+    hipLaunchKernelGGL(k_panel8_fill<float>, dim3((unsigned)nblk), dim3(256), lds, st, nrows,
+                       ncols, indptr, indices, (const float *)data, qptr, rowexp, panel8);
+  } else {
+    hipLaunchKernelGGL(k_row_exp<double>, dim3(ceil_div_u(nrows, 4)), dim3(256), 0, st, nrows,
+                       indptr, (const double *)data, rowexp);
+    hipLaunchKernelGGL(k_panel8_fill<double>, dim3((unsigned)nblk), dim3(256), lds, st, nrows,
+                       ncols, indptr, indices, (const double *)data, qptr, rowexp, panel8);
+  }
+  FLUX_LAUNCH_CHECK();
+  return 0;
+}
+
+static int64_t x8_groups(int64_t nrhs) { return (nrhs + kGroup - 1) / kGroup; }
+
+// workspace: xexp (nh * 64 int32), cmax (nh * 64 uint64), x8 (n * nh * 320 bytes)
+extern "C" int64_t flux_slice_x_bytes(int64_t n, int64_t nrhs) {
+  if (n < 0 || nrhs < 0) return -1;
+  const int64_t nh = x8_groups(nrhs);
+  return nh * kGroup * (4 + 8) + n * nh * kXRow;
+}
+
+extern "C" int flux_slice_x(int64_t n, int64_t nrhs, const double *X, const double *rho,
+                            double rho_s, int scale, void *work, void *stream) {
+  FLUX_CHECK(n >= 0 && nrhs >= 0, "flux_slice_x: bad sizes");
+  if (n == 0 || nrhs == 0) return 0;
+  FLUX_CHECK(X && work, "flux_slice_x: NULL array");
+  FLUX_CHECK(((uintptr_t)work & 15) == 0, "flux_slice_x: workspace must be 16-byte aligned");
+  hipStream_t st = as_stream(stream);
+  const int64_t nh = x8_groups(nrhs), np = nh * kGroup;
+  int32_t *xexp = (int32_t *)work;
+  unsigned long long *cmax = (unsigned long long *)((char *)work + np * 4);
+  int8_t *x8 = (int8_t *)((char *)work + np * 12);
+  hipLaunchKernelGGL(k_zero_u32, dim3(ceil_div_u(np * 2, 256)), dim3(256), 0, st,
+                     (uint32_t *)cmax, np * 2);
+  const unsigned gx = (unsigned)((n + 3) / 4 < 1024 ? (n + 3) / 4 : 1024);
+  hipLaunchKernelGGL(k_colmax_scaled, dim3(gx, ceil_div_u(nrhs, 64)), dim3(64, 4), 0, st, n,
+                     nrhs, X, rho, rho_s, scale, cmax);
+  hipLaunchKernelGGL(k_slice_x, dim3(ceil_div_u(n * np, 256)), dim3(256), 0, st, n, nrhs,
+                     (int)nh, X, rho, rho_s, scale, cmax, xexp, x8);
+  FLUX_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int flux_panel8_mm(int64_t nrows, const int64_t *colptr, const int32_t *cols,
+                              const int64_t *qptr, const int8_t *panel8, const int32_t *rowexp,
+                              int64_t nrhs, const void *work, const double *E, const double *B,
+                              const double *rho, double rho_s, int left, double *out,
+                              double *colmax, void *stream) {
+  FLUX_CHECK(nrows >= 0 && nrhs >= 0, "flux_panel8_mm: bad sizes");
+  const int64_t nblk = (nrows + kPanelRows - 1) / kPanelRows;
+  if (nblk == 0 || nrhs == 0) return 0;
+  FLUX_CHECK(colptr && cols && qptr && panel8 && rowexp && work && out,
+             "flux_panel8_mm: NULL array");
+  FLUX_CHECK(E == nullptr || B != nullptr, "flux_panel8_mm: E needs B");
+  hipStream_t st = as_stream(stream);
+  const int64_t nh = x8_groups(nrhs), np = nh * kGroup;
+  const int32_t *xexp = (const int32_t *)work;
+  const int8_t *x8 = (const int8_t *)work + np * 12;
+  if (colmax)
+    hipLaunchKernelGGL(k_zero_u32, dim3(ceil_div_u(nrhs * 2, 256)), dim3(256), 0, st,
+                       (uint32_t *)colmax, nrhs * 2);
+  FLUX_CHECK(nblk * nh < (1ll << 31), "flux_panel8_mm: too many workgroups");
+  hipLaunchKernelGGL(k_panel8_mm, dim3((unsigned)(nblk * nh)), dim3(256), 0, st, nrows, nblk,
+                     (int)nh, colptr, cols, qptr, panel8, rowexp, x8, xexp, nrhs, E, B, rho,
+                     rho_s, left, out, colmax);
+  FLUX_LAUNCH_CHECK();
+  return 0;
+}

@@ -290,8 +290,49 @@ def _gmres(op, E, rho, left, rtol, maxiter, restart, x0):
             _kernels.axpby(t.tensor(y[i], dtype=t.float64, device=dev), Vs[i], None, X)
 
 
+def _panel_ready(FF):
+    """True for a compressed matrix whose operator is all near field (no
+    low-rank leaves): its batched solves run on the row-block panels."""
+    from .compressed_form_factors import CompressedFormFactorMatrix
+    return (isinstance(FF, CompressedFormFactorMatrix) and not getattr(FF, "_derived", False)
+            and getattr(FF, "_near", None) is not None
+            and not getattr(FF, "_far_used", [None]))
+
+
+def _panel_engine(FF, residual):
+    """(BF16 panels, residual panels) of FF, built once and kept on FF."""
+    from ._panel import PanelMatrix, SlicedPanelMatrix
+    cache = FF.__dict__.setdefault("_panel_cache", {})
+    if "bf16" not in cache:
+        cache["bf16"] = PanelMatrix(FF._near, "bf16")
+    if residual not in cache:
+        lo = cache["bf16"]
+        cache[residual] = (SlicedPanelMatrix(FF._near, lo) if residual == "int8"
+                           else lo.sibling(FF._near, "fp32"))
+    return cache["bf16"], cache[residual]
+
+
+def _solve_panels(FF, E2, rho, left, rtol, inner_iters, max_refine, residual):
+    from ._panel import PanelRefinedSolver
+    t = _lib.torch()
+    n, k = E2.shape
+    op = device_operator(FF)
+    rho64 = _rho_internal(op, _prep_rho(rho, n, t.float64))
+    Ei = _to_internal(op, E2)
+    P_lo, P_hi = _panel_engine(FF, residual)
+    solvers = FF.__dict__.setdefault("_panel_solvers", {})
+    key = (residual, k, bool(left))
+    S = solvers.get(key)
+    if S is None:
+        S = solvers[key] = PanelRefinedSolver(P_lo, P_hi, n, k, rho64, left=left)
+    else:
+        S.set_rho(rho64)
+    B, info = S.solve(Ei, rtol=rtol, inner_iters=inner_iters, max_rounds=max_refine)
+    return _from_internal(op, B), info
+
+
 def solve_radiosity_refined(FF_lo, FF_hi, E, rho=1, albedo_placement='right', rtol=1e-7,
-                            inner_iters=4, max_refine=30, return_info=False):
+                            inner_iters=4, max_refine=30, return_info=False, residual="int8"):
     """Mixed-precision radiosity solve by iterative refinement.
 
     Inner solves use ``FF_lo`` (low-precision values, e.g. BF16 with FP32
@@ -303,17 +344,31 @@ def solve_radiosity_refined(FF_lo, FF_hi, E, rho=1, albedo_placement='right', rt
     exact solution of the master system is then below about
     rtol / (1 - max rho * rowsum F) relative to max|E_c|.
 
+    When ``FF_lo is FF_hi`` is a compressed matrix with no low-rank leaves,
+    the solve runs on its row-block panels (built on first use and kept):
+    BF16 panels on MFMA for the inner sweeps, and for the residual either
+    int8-sliced panels on int8 MFMA (``residual='int8'``, FP64-accurate:
+    about 1e-9 of max|F_r.| max|x| per column) or the FP32 master panels on
+    FP64 MFMA (``residual='fp64'``).
+
     E: N x k (NumPy or device).  Returns B (FP64) or (B, info).
     """
     t = _lib.require_gpu()
+    if residual not in ("int8", "fp64"):
+        raise ValueError("residual must be 'int8' or 'fp64'")
     host = not isinstance(E, t.Tensor)
     Ed = _lib.to_dev(np.asarray(E, dtype=np.float64) if host else E.to(t.float64))
     vec = Ed.dim() == 1
     E2 = Ed.reshape(Ed.shape[0], -1).contiguous()
     n, k = E2.shape
+    left = albedo_placement == 'left'
+    if FF_lo is FF_hi and _panel_ready(FF_lo):
+        B, info = _solve_panels(FF_lo, E2, rho, left, rtol, inner_iters, max_refine, residual)
+        out = B.reshape(-1) if vec else B
+        out = out.cpu().numpy() if host else out
+        return (out, info) if return_info else out
     op_lo = device_operator(FF_lo)
     op_hi = device_operator(FF_hi)
-    left = albedo_placement == 'left'
     # both operators must share one ordering (they are views of one matrix)
     rho64 = _rho_internal(op_hi, _prep_rho(rho, n, t.float64))
     E2 = _to_internal(op_hi, E2)

@@ -330,6 +330,48 @@ int flux_scale_to_bf16(int64_t n, int64_t nrhs, const float *x,
                        void *stream);
 
 /* ---------------------------------------------------------------------- */
+/* Int8-sliced panels: the FP64-accurate refinement residual on int8 MFMA  */
+/* (Ozaki scheme; no reference counterpart: they replace the FP64 H-matvec */
+/* inside the residual E - (I - F rho) B of the refined solve).  Values   */
+/* are split into 5 signed 7-bit slices under a power-of-two scale per     */
+/* panel row and per right-hand side; slice products are exact in int32   */
+/* and combined in FP64 (error ~1e-9 of max|F_r.| max|x_.k| per column).   */
+/* The panels share colptr / cols with the BF16 / FP32 panels of the same  */
+/* CSR; block b holds ceil(nch_b / 4) quads of 64 columns.                 */
+/* ---------------------------------------------------------------------- */
+
+/* qptr[b] = quads before block b (nblk + 1 entries); returns the total
+ * quads and the widest block's column count on the host (synchronises).
+ * Fails when a block has more than 104,832 columns (int32 bound). */
+int flux_panel8_layout(int64_t nblk, const int64_t *colptr, int64_t *qptr,
+                       int64_t *total_quads, int64_t *max_cols, void *stream);
+
+/* rowexp[r] (nrows int32) and the sliced panels (total_quads * 20480
+ * bytes) from the CSR values (F32 or F64). */
+int flux_panel8_fill(int64_t nrows, int64_t ncols, const int64_t *indptr,
+                     const int32_t *indices, const void *data, int dtype,
+                     const int64_t *qptr, int64_t total_quads, int32_t *rowexp,
+                     int8_t *panel8, void *stream);
+
+/* Bytes of the X-slice workspace for n rows and nrhs right-hand sides. */
+int64_t flux_slice_x_bytes(int64_t n, int64_t nrhs);
+
+/* Slices of s o X (n x nrhs FP64; s_i = rho_i or rho_s if scale, else 1)
+ * into `work` (16-byte aligned, flux_slice_x_bytes long). */
+int flux_slice_x(int64_t n, int64_t nrhs, const double *X, const double *rho,
+                 double rho_s, int scale, void *work, void *stream);
+
+/* acc = P X from the slices in `work`;  with E: out = E + s o acc - B
+ * (s_i = rho_i if left, else 1) and colmax[c] = max_i |out| (the
+ * refinement residual); without E: out = acc. */
+int flux_panel8_mm(int64_t nrows, const int64_t *colptr, const int32_t *cols,
+                   const int64_t *qptr, const int8_t *panel8,
+                   const int32_t *rowexp, int64_t nrhs, const void *work,
+                   const double *E, const double *B, const double *rho,
+                   double rho_s, int left, double *out, double *colmax,
+                   void *stream);
+
+/* ---------------------------------------------------------------------- */
 /* Column-blocked vector kernels (solver building blocks; blocks are      */
 /* n x nrhs row-major, reductions per column in FP64, deterministic)       */
 /* ---------------------------------------------------------------------- */

@@ -1,0 +1,203 @@
+"""GPU tests of the int8-sliced panels (the FP64-accurate refinement residual
+on int8 MFMA): the slices decode to the CSR values within the slicing bound,
+products and residuals stay within the a-priori error bound against FP64,
+and refined solves with this residual meet 1e-6 against the dense FP64
+solution (through the solver and through the public solve API)."""
+import numpy as np
+import pytest
+
+from fluxpy_amd import meshes
+
+pytestmark = pytest.mark.gpu
+
+SLICES = 5
+
+
+def _dev():
+    from fluxpy_amd import _lib
+    return _lib.device()
+
+
+def _exp(m):
+    """frexp exponent (m < 2^e), 0 where m == 0."""
+    e = np.frexp(m)[1]
+    return np.where(m > 0, e, 0)
+
+
+@pytest.fixture(scope="module")
+def setup():
+    import torch
+    torch.cuda.set_device(0)
+    from fluxpy_amd.compressed_form_factors import _ClusterTree
+    from fluxpy_amd import _kernels
+    from fluxpy_amd._panel import PanelMatrix, SlicedPanelMatrix
+    from fluxpy_amd.shape import CgalTrimeshShapeModel
+    V, F = meshes.fractal_terrain(grid=49, levels=6, seed=11)
+    sm = CgalTrimeshShapeModel(V.astype(np.float32), F)
+    ct = _ClusterTree(np.asarray(sm.P, dtype=np.float64), 2)
+    csr = _kernels.ff_csr(sm, ct.perm, ct.perm)
+    Pb = PanelMatrix(csr, "bf16")
+    P8 = SlicedPanelMatrix(csr, Pb)
+    return torch, sm, csr, Pb, P8, ct
+
+
+def _bound(sp, X):
+    """A-priori error bound of the sliced product per (row, rhs):
+    6.2 * 2^(e_r + e_k - 35) * nnz_r plus FP64 rounding."""
+    absp = abs(sp)
+    er = _exp(np.asarray(absp.max(axis=1).todense()).ravel())
+    ek = _exp(abs(X).max(axis=0))
+    nnz = np.diff(sp.indptr)
+    b = 6.2 * np.ldexp(1.0, er[:, None] + ek[None, :] - 35) * nnz[:, None]
+    return b + 4e-16 * (absp @ abs(X))
+
+
+def test_panel8_layout_and_slices(setup):
+    torch, sm, csr, Pb, P8, ct = setup
+    sp = csr.to_scipy().tocsr()
+    sp.sort_indices()
+    n = sp.shape[0]
+    colptr = P8.colptr.cpu().numpy()
+    cols = P8.cols.cpu().numpy()
+    qptr = P8.qptr.cpu().numpy()
+    nq = (np.diff(colptr) // 16 + 3) // 4
+    assert qptr[0] == 0 and np.array_equal(np.diff(qptr), nq)
+    assert P8.quads == qptr[-1]
+    rowmax = np.asarray(abs(sp).max(axis=1).todense()).ravel()
+    er = P8.rowexp.cpu().numpy()[:n]
+    assert np.array_equal(er, _exp(rowmax))
+    panel = P8.panel.cpu().numpy()[:P8.quads * 20480].reshape(P8.quads, SLICES, 64, 64)
+    assert abs(panel.astype(np.int32)).max() <= 64
+    w = np.ldexp(1.0, -6 - 7 * np.arange(SLICES))
+    for b in range(len(colptr) - 1):
+        r0, r1 = 64 * b, min(n, 64 * (b + 1))
+        q = panel[qptr[b]:qptr[b + 1]].astype(np.float64)       # (nq, 5, 64, 64)
+        dec = np.einsum("qirc,i->rqc", q, w).reshape(64, -1)     # rows x positions
+        dec = dec[:r1 - r0] * np.ldexp(1.0, er[r0:r1])[:, None]
+        ncl = colptr[b + 1] - colptr[b]
+        nd = np.unique(sp[r0:r1].indices).size
+        want = np.zeros_like(dec)
+        want[:, :nd] = sp[r0:r1][:, cols[colptr[b]:colptr[b] + nd]].toarray()
+        # |v - decoded| <= 2^(e_r - 35); exact zeros elsewhere (padding included)
+        tol = np.ldexp(1.0, er[r0:r1] - 35)[:, None]
+        assert (abs(dec - want) <= tol).all(), b
+        assert (dec[:, ncl:] == 0).all() and (dec[want == 0] == 0).all()
+
+
+@pytest.mark.parametrize("k", [64, 16, 5, 70, 33])
+def test_panel8_product_within_bound(setup, k):
+    torch, sm, csr, Pb, P8, ct = setup
+    n = csr.shape[0]
+    sp = csr.to_scipy().astype(np.float64)
+    rng = np.random.default_rng(100 + k)
+    X = rng.standard_normal((n, k))
+    X[:, 0] = 0.0                                    # an all-zero right-hand side
+    if k > 2:
+        X[:, 1] *= np.exp(rng.uniform(-20, 0, n))    # wide dynamic range
+    Xd = torch.tensor(X, device=_dev())
+    Y = torch.empty_like(Xd)
+    P8.mm_f64(Xd, Y)
+    ref = sp @ X
+    err = abs(Y.cpu().numpy() - ref)
+    bound = _bound(sp, X)
+    assert (err <= bound).all(), (err / np.maximum(bound, 1e-300)).max()
+    assert (Y.cpu().numpy()[:, 0] == 0).all()
+    # the bound is absolute (relative to max|p_r.| max|x_.k|); typical
+    # entries of the ordinary columns are far below FP32 accuracy relative to
+    # |P| |X| (column 1 spans e^-20 and is left out)
+    ordinary = np.arange(k) >= 2
+    scale = (abs(sp) @ abs(X))[:, ordinary]
+    rel = err[:, ordinary][scale > 0] / scale[scale > 0]
+    assert rel.size == 0 or np.median(rel) < 1e-9, np.median(rel)
+
+
+@pytest.mark.parametrize("left", [False, True])
+def test_panel8_residual(setup, left):
+    torch, sm, csr, Pb, P8, ct = setup
+    n = csr.shape[0]
+    k = 64
+    sp = csr.to_scipy().astype(np.float64)
+    rng = np.random.default_rng(7 + int(left))
+    E = rng.uniform(0, 1000, (n, k))
+    B = E * rng.uniform(1.0, 1.2, (n, k))
+    rho = rng.uniform(0.1, 0.14, n)
+    t = lambda a: torch.tensor(a, device=_dev())
+    R = torch.empty((n, k), dtype=torch.float64, device=_dev())
+    cm = torch.empty(k, dtype=torch.float64, device=_dev())
+    if left:
+        P8.mm_f64(t(B), R, E=t(E), B=t(B), rho=t(rho), left=True, colmax=cm)
+        Rr = E + rho[:, None] * (sp @ B) - B
+        bound = rho[:, None] * _bound(sp, B)
+    else:
+        P8.mm_f64(t(B), R, E=t(E), B=t(B), colmax=cm, xscale=t(rho))
+        X = rho[:, None] * B
+        Rr = E + sp @ X - B
+        bound = _bound(sp, X)
+    err = abs(R.cpu().numpy() - Rr)
+    assert (err <= bound + 1e-13 * abs(E).max()).all()
+    assert np.allclose(cm.cpu().numpy(), abs(R.cpu().numpy()).max(0), rtol=0, atol=0)
+    assert np.allclose(cm.cpu().numpy(), abs(Rr).max(0), rtol=1e-9)
+    # a scalar albedo on the right
+    P8.mm_f64(t(B), R, E=t(E), B=t(B), xscale=0.12)
+    Rs = E + sp @ (0.12 * B) - B
+    assert (abs(R.cpu().numpy() - Rs) <= _bound(sp, 0.12 * B) + 1e-13 * abs(E).max()).all()
+
+
+@pytest.mark.parametrize("left", [False, True])
+def test_refined_solve_int8_residual(setup, oracle_mod, left):
+    from fluxpy_amd._panel import PanelRefinedSolver
+    torch, sm, csr, Pb, P8, ct = setup
+    n = csr.shape[0]
+    sp = csr.to_scipy().astype(np.float64)
+    D = meshes.sun_directions(64, seed=3)
+    E = sm.get_direct_irradiance_multi(1365.0, D).astype(np.float64)
+    Ep = E[ct.perm]
+    rho = np.random.default_rng(0).uniform(0.1, 0.14, n)
+    S = PanelRefinedSolver(Pb, P8, n, 64, torch.tensor(rho, device=_dev()), left=left)
+    B, info = S.solve(torch.tensor(Ep, device=_dev()), rtol=1e-7, inner_iters=2)
+    Bd = oracle_mod.solve_radiosity_dense(sp, Ep, rho, "left" if left else "right")
+    err = abs(B.cpu().numpy() - Bd).max(0) / abs(Bd).max(0)
+    assert err.max() <= 1e-6, err.max()
+    assert info["rounds"] <= 6
+    # the same solver object reused with a scalar albedo
+    S.set_rho(0.12)
+    B2, _ = S.solve(torch.tensor(Ep, device=_dev()), rtol=1e-7, inner_iters=2)
+    Bd2 = oracle_mod.solve_radiosity_dense(sp, Ep, np.full(n, 0.12), "left" if left else "right")
+    assert (abs(B2.cpu().numpy() - Bd2).max(0) / abs(Bd2).max(0)).max() <= 1e-6
+
+
+@pytest.mark.parametrize("residual", ["int8", "fp64"])
+def test_public_refined_solve_on_panels(oracle_mod, residual):
+    """solve_radiosity_refined(H, H, ...) on a compressed matrix with no
+    low-rank leaves runs on the panels and meets 1e-6 in facet order."""
+    import torch
+    from fluxpy_amd import _hbuild
+    from fluxpy_amd.compressed_form_factors import CompressedFormFactorMatrix
+    from fluxpy_amd.shape import CgalTrimeshShapeModel
+    from fluxpy_amd.solve import _panel_ready, solve_radiosity_refined
+    V, F = meshes.fractal_terrain(grid=33, levels=5, seed=5)
+    sm = CgalTrimeshShapeModel(V.astype(np.float32), F)
+    old = _hbuild.LOWRANK
+    _hbuild.LOWRANK = False
+    try:
+        H = CompressedFormFactorMatrix(sm, tol=1e-6, min_size=1024)
+    finally:
+        _hbuild.LOWRANK = old
+    assert _panel_ready(H)
+    n = sm.num_faces
+    D = meshes.sun_directions(8, seed=9)
+    E = sm.get_direct_irradiance_multi(1365.0, D).astype(np.float64)
+    rho = np.random.default_rng(2).uniform(0.1, 0.14, n)
+    B, info = solve_radiosity_refined(H, H, E, rho, rtol=1e-7, inner_iters=2,
+                                      return_info=True, residual=residual)
+    assert isinstance(B, np.ndarray) and B.shape == E.shape
+    Fd = H.toarray().astype(np.float64)
+    import scipy.sparse
+    Bd = oracle_mod.solve_radiosity_dense(scipy.sparse.csr_matrix(Fd), E, rho, "right")
+    err = abs(B - Bd).max(0) / abs(Bd).max(0)
+    assert err.max() <= 1e-6, err.max()
+    # device input stays on the device; a second call reuses the cached panels
+    Bt = solve_radiosity_refined(H, H, torch.tensor(E, device=_dev()), rho, rtol=1e-7,
+                                 inner_iters=2, residual=residual)
+    assert isinstance(Bt, torch.Tensor)
+    assert abs(Bt.cpu().numpy() - Bd).max() / abs(Bd).max() <= 1e-6
