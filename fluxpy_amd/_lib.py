@@ -69,8 +69,10 @@ SIGNATURES = {
     "flux_panel_count": (_c_int, [_c_i64, _c_i64, _vp, _vp, _vp, _vp]),
     "flux_panel_fill": (_c_int, [_c_i64, _c_i64, _vp, _vp, _vp, _c_int, _vp, _c_i64, _vp, _vp,
                                  _c_int, _vp]),
+    "flux_panel_plan": (_c_int, [_c_i64, _vp, _c_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "flux_panel_mm_bf16": (_c_int, [_c_i64, _vp, _vp, _vp, _vp, _c_i64, _vp, _vp,
-                                    ctypes.c_float, _c_int, _vp, _vp, _vp, _vp, _vp]),
+                                    ctypes.c_float, _c_int, _vp, _vp, _vp, _vp, _vp, _c_i64,
+                                    _vp, _vp, _vp, _c_i64, _vp, _vp]),
     "flux_panel_mm_f64": (_c_int, [_c_i64, _vp, _vp, _vp, _vp, _c_i64, _vp, _vp, _vp, _c_dbl,
                                    _c_int, _vp, _vp, _vp]),
     "flux_scale_to_bf16": (_c_int, [_c_i64, _c_i64, _vp, _vp, ctypes.c_float, _c_int, _vp,
@@ -81,7 +83,8 @@ SIGNATURES = {
     "flux_slice_x_bytes": (_c_i64, [_c_i64, _c_i64]),
     "flux_slice_x": (_c_int, [_c_i64, _c_i64, _vp, _vp, _c_dbl, _c_int, _vp, _vp]),
     "flux_panel8_mm": (_c_int, [_c_i64, _vp, _vp, _vp, _vp, _vp, _c_i64, _vp, _vp, _vp, _vp,
-                                _c_dbl, _c_int, _vp, _vp, _vp]),
+                                _c_dbl, _c_int, _vp, _vp, _vp, _c_i64, _vp, _vp, _vp, _c_i64,
+                                _vp, _vp]),
     "flux_vec_reduce":(_c_int, [_c_i64, _c_i64, _vp, _vp, _c_int, _c_int, _vp, _vp, _vp]),
     "flux_vec_axpby": (_c_int, [_c_i64, _c_i64, _vp, _vp, _vp, _vp, _c_int, _vp]),
     "flux_vec_rowscale": (_c_int, [_c_i64, _c_i64, _vp, _c_dbl, _c_int, _c_dbl, _vp, _vp,

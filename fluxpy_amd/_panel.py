@@ -15,12 +15,60 @@ from . import _kernels
 from . import _lib
 
 
+class PanelPlan:
+    """Balanced work items of the panel products (flux_panel_plan): row
+    blocks cut into pieces of at most ``cap`` quads (64 columns), in block
+    order with each XCD's run sorted longest first.  Blocks cut into several
+    pieces sum their partials (slots) in a second launch.  One plan serves
+    every product on the same column lists; the slots are kept per product
+    kind, and calls that share them must not overlap (one stream does
+    that)."""
+
+    CAP = 32
+
+    def __init__(self, colptr, nblk, cap=None):
+        t = _lib.require_gpu()
+        dev = colptr.device
+        st = _lib.stream()
+        self.cap = int(cap or self.CAP)
+        self.nblk = nblk
+        n_items, n_slots, n_split = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        _lib.call("flux_panel_plan", nblk, colptr.data_ptr(), self.cap, None, None, None, None,
+                  ctypes.byref(n_items), ctypes.byref(n_slots), ctypes.byref(n_split), st)
+        self.n_items, self.n_slots = n_items.value, n_slots.value
+        self.n_split = n_split.value
+        self.items = t.empty(max(4 * self.n_items, 4), dtype=t.int32, device=dev)
+        self.nsplit = t.empty(max(nblk, 1), dtype=t.int32, device=dev)
+        self.slot0 = t.empty(max(nblk, 1), dtype=t.int32, device=dev)
+        self.split_blocks = t.empty(max(self.n_split, 1), dtype=t.int32, device=dev)
+        _lib.call("flux_panel_plan", nblk, colptr.data_ptr(), self.cap, self.items.data_ptr(),
+                  self.nsplit.data_ptr(), self.slot0.data_ptr(), self.split_blocks.data_ptr(),
+                  ctypes.byref(n_items), ctypes.byref(n_slots), ctypes.byref(n_split), st)
+        self._ws = {}
+
+    def args(self, kind, nrhs, elem_bytes):
+        """(items, n_items, nsplit, slot0, split_blocks, n_split, ws) for a
+        product of kind ``kind`` with nrhs right-hand sides."""
+        t = _lib.torch()
+        nh = (nrhs + 63) // 64
+        key = (kind, nh)
+        if key not in self._ws:
+            self._ws[key] = t.empty(max(self.n_slots * nh * 4096 * elem_bytes, 16),
+                                    dtype=t.uint8, device=self.items.device)
+        return (self.items.data_ptr(), self.n_items, self.nsplit.data_ptr(),
+                self.slot0.data_ptr(), self.split_blocks.data_ptr(), self.n_split,
+                self._ws[key].data_ptr())
+
+
+_NO_PLAN = (None, 0, None, None, None, 0, None)
+
+
 class PanelMatrix:
     """Panels of a square CSR in one value precision ('bf16' or 'fp32')."""
 
     ROWS = 64
 
-    def __init__(self, csr, values="bf16", colptr=None, cols=None):
+    def __init__(self, csr, values="bf16", colptr=None, cols=None, plan=None):
         t = _lib.require_gpu()
         m, n = csr.shape
         self.shape = (m, n)
@@ -52,10 +100,12 @@ class PanelMatrix:
                   total_cols, scratch_cols.data_ptr(), self.panel.data_ptr(),
                   _lib.dtype_code(pdt), st)
         self.nnz = csr.nnz
+        # balanced work items (shared with siblings); None -> one per block
+        self.plan = plan if plan is not None else PanelPlan(colptr, nblk)
 
     def sibling(self, csr, values):
         """Panels of the same pattern in another precision (shares cols)."""
-        return PanelMatrix(csr, values, colptr=self.colptr, cols=self.cols)
+        return PanelMatrix(csr, values, colptr=self.colptr, cols=self.cols, plan=self.plan)
 
     @property
     def nbytes(self):
@@ -72,10 +122,11 @@ class PanelMatrix:
                 prev=None, colmax=None):
         """out = base + s (P Xb) with the Jacobi epilogue (see panel.hip)."""
         k = Xb.shape[1]
+        plan = self.plan.args("bf16", k, 4) if self.plan else _NO_PLAN
         _lib.call("flux_panel_mm_bf16", self.shape[0], self.colptr.data_ptr(),
                   self.cols.data_ptr(), self.panel.data_ptr(), Xb.data_ptr(), k, _lib.ptr(base),
                   _lib.ptr(rho), float(rho_s), int(left), out.data_ptr(), _lib.ptr(xnext),
-                  _lib.ptr(prev), _lib.ptr(colmax), _lib.stream())
+                  _lib.ptr(prev), _lib.ptr(colmax), *plan, _lib.stream())
         return out
 
     def mm_f64(self, X, out, E=None, B=None, rho=None, rho_s=1.0, left=False, colmax=None):
@@ -117,6 +168,7 @@ class SlicedPanelMatrix:
                   csr.data.data_ptr(), _lib.dtype_code(csr.data.dtype), self.qptr.data_ptr(),
                   self.quads, self.rowexp.data_ptr(), self.panel.data_ptr(), st)
         self.nnz = csr.nnz
+        self.plan = like.plan
         self._work = None
 
     @property
@@ -144,10 +196,11 @@ class SlicedPanelMatrix:
         xs_s = 1.0 if xscale is None or xs_t is not None else float(xscale)
         _lib.call("flux_slice_x", self.shape[1], k, X.data_ptr(), _lib.ptr(xs_t), xs_s,
                   int(xscale is not None), work.data_ptr(), st)
+        plan = self.plan.args("int8", k, 8) if self.plan else _NO_PLAN
         _lib.call("flux_panel8_mm", self.shape[0], self.colptr.data_ptr(), self.cols.data_ptr(),
                   self.qptr.data_ptr(), self.panel.data_ptr(), self.rowexp.data_ptr(), k,
                   work.data_ptr(), _lib.ptr(E), _lib.ptr(B), _lib.ptr(rho), float(rho_s),
-                  int(left), out.data_ptr(), _lib.ptr(colmax), st)
+                  int(left), out.data_ptr(), _lib.ptr(colmax), *plan, st)
         return out
 
 

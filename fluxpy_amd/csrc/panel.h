@@ -69,6 +69,42 @@ __device__ __forceinline__ uint32_t column_pos(const uint32_t *bm, const uint32_
   return pre[w] + __popc(bm[w] & ((1u << (j & 31)) - 1u));
 }
 
+// Work item of a panel product launch.  With a plan (flux_panel_plan) item
+// `it` covers quads [q0, q1) of row block b; a block cut into nsplit > 1
+// items owns the partial-sum slots slot0[b] .. slot0[b] + nsplit - 1 (item
+// `piece` writes slot slot0[b] + piece).  Without a plan (items == NULL) item
+// `it` is the whole block `it`.  Launches map workgroups to items with
+// xcd_block, so every XCD works through a contiguous run of items.
+struct PanelItem {
+  int64_t b, q0, q1;
+  int nsplit, slot0, piece;
+};
+
+__device__ __forceinline__ PanelItem panel_item(const int32_t *__restrict__ items,
+                                                const int32_t *__restrict__ nsplit,
+                                                const int32_t *__restrict__ slot0,
+                                                const int64_t *__restrict__ colptr,
+                                                int64_t it) {
+  PanelItem p;
+  if (items) {
+    const int4 r = reinterpret_cast<const int4 *>(items)[it];
+    p.b = r.x;
+    p.q0 = r.y;
+    p.q1 = r.z;
+    p.piece = r.w;
+    p.nsplit = nsplit[p.b];
+    p.slot0 = slot0[p.b];
+  } else {
+    p.b = it;
+    p.q0 = 0;
+    p.q1 = (((colptr[p.b + 1] - colptr[p.b]) >> 4) + 3) >> 2;
+    p.nsplit = 1;
+    p.slot0 = -1;
+    p.piece = 0;
+  }
+  return p;
+}
+
 // LDS words of the column bitmap of a matrix with ncols columns.
 static inline int64_t panel_lds_words(int64_t ncols) { return (ncols + 31) / 32; }
 

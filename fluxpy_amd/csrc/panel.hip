@@ -26,6 +26,9 @@
 // gather mostly the same X rows, share an L2.
 #include <hip/hip_bf16.h>
 
+#include <algorithm>
+#include <vector>
+
 #include "panel.h"
 
 namespace flux {
@@ -128,7 +131,78 @@ __global__ void k_zero_any(T *p, int64_t n) {
 //   xnext[i, c] = bf16(t_i out[i, c])               (t_i = rho_i if right)
 // base == NULL -> out = acc; xnext == NULL -> not written;
 // colmax != NULL -> colmax[c] = max_i |out[i, c] - prev[i, c]|.
+// With a plan, the items of a block cut into several pieces write their FP32
+// partials to slots (slot (s, g) holds 16 x 256 floats at ws + (s nh + g)
+// 4096) and k_panel_reduce_bf16 sums them in slot order (so the result does
+// not depend on scheduling) and runs the epilogue.
 // ---------------------------------------------------------------------------
+// Jacobi-sweep epilogue of one 64-row block and one group of 64 right-hand
+// sides (g0 ..), from the product acc in the MFMA layout: acc[t][q] =
+// Y[row 16 w + li][rhs g0 + 16 t + 4 lg + q] (w = wave, li = lane % 16,
+// lg = lane / 16).
+__device__ __forceinline__ void bf16_epilogue(
+    const v4f (&acc)[4], int64_t b, int64_t g0, int64_t nrows, int64_t nrhs,
+    const float *__restrict__ base, const float *__restrict__ rho, float rho_s, int left,
+    float *__restrict__ out, __hip_bfloat16 *__restrict__ xnext, const float *__restrict__ prev,
+    double *__restrict__ colmax) {
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int li = lane & 15, lg = lane >> 4;
+  const bool vec_x = (nrhs & 3) == 0;
+  const int64_t row_base = b * kPanelRows;
+  const int64_t r = row_base + w * 16 + li;
+  const bool row_ok = r < nrows;
+  const float rr = row_ok ? (rho ? rho[r] : rho_s) : 0.f;
+  for (int t = 0; t < 4; ++t) {
+    const int64_t cb = g0 + t * 16 + lg * 4;
+    double cm[4] = {0.0, 0.0, 0.0, 0.0};
+    if (row_ok && cb < nrhs) {
+      const int64_t o = r * nrhs + cb;
+      float v[4];
+      for (int q = 0; q < 4; ++q) v[q] = left ? acc[t][q] * rr : acc[t][q];
+      if (vec_x && cb + 3 < nrhs) {
+        if (base) {
+          const float4 bv = *reinterpret_cast<const float4 *>(base + o);
+          v[0] += bv.x; v[1] += bv.y; v[2] += bv.z; v[3] += bv.w;
+        }
+        *reinterpret_cast<float4 *>(out + o) = make_float4(v[0], v[1], v[2], v[3]);
+        if (xnext) {
+          const float s0 = left ? 1.f : rr;
+          unsigned short h[4];
+          for (int q = 0; q < 4; ++q) {
+            const __hip_bfloat16 hb = __float2bfloat16(s0 * v[q]);
+            h[q] = *reinterpret_cast<const unsigned short *>(&hb);
+          }
+          *reinterpret_cast<uint2 *>(xnext + o) =
+              make_uint2(h[0] | ((unsigned)h[1] << 16), h[2] | ((unsigned)h[3] << 16));
+        }
+        if (colmax) {
+          const float4 pv = *reinterpret_cast<const float4 *>(prev + o);
+          cm[0] = fabs((double)v[0] - (double)pv.x);
+          cm[1] = fabs((double)v[1] - (double)pv.y);
+          cm[2] = fabs((double)v[2] - (double)pv.z);
+          cm[3] = fabs((double)v[3] - (double)pv.w);
+        }
+      } else {
+        for (int q = 0; q < 4; ++q) {
+          if (cb + q >= nrhs) break;
+          if (base) v[q] += base[o + q];
+          out[o + q] = v[q];
+          if (xnext) xnext[o + q] = __float2bfloat16(left ? v[q] : rr * v[q]);
+          if (colmax) cm[q] = fabs((double)v[q] - (double)prev[o + q]);
+        }
+      }
+    }
+    if (colmax) {
+      for (int q = 0; q < 4; ++q) {
+        const double m = max16(cm[q]);
+        if (li == 0 && cb + q < nrhs)
+          atomicMax((unsigned long long *)(colmax + cb + q),
+                    (unsigned long long)__double_as_longlong(m));
+      }
+    }
+  }
+}
+
 // SC chunks of 16 columns per stage.  LDS block per stage (bf16):
 // xs[kg][rhs][4], kg = group of 4 consecutive columns of the stage, so the
 // A fragment of a lane (4 consecutive columns at one right-hand side) is one
@@ -142,16 +216,25 @@ __global__ void __launch_bounds__(256, SC == 4 ? 3 : 2) k_panel_mm_bf16(
     const __hip_bfloat16 *__restrict__ X, int64_t nrhs, const float *__restrict__ base,
     const float *__restrict__ rho, float rho_s, int left, float *__restrict__ out,
     __hip_bfloat16 *__restrict__ xnext, const float *__restrict__ prev,
-    double *__restrict__ colmax) {
+    double *__restrict__ colmax, const int32_t *__restrict__ items,
+    const int32_t *__restrict__ nsplit, const int32_t *__restrict__ slot0,
+    float *__restrict__ ws, int64_t nitems) {
   static_assert(SC % 4 == 0, "a stage holds a multiple of 4 chunks");
   constexpr int KG = SC * 4, GPT = KG / 16;  // column groups per stage / per thread
   __shared__ __align__(16) uint2 xs[2][KG * 64];
-  const int64_t b = xcd_block(blockIdx.x, nblk);
+  const PanelItem itm =
+      panel_item(items, nsplit, slot0, colptr, xcd_block(blockIdx.x, nitems));
+  const int64_t b = itm.b;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int li = lane & 15, lg = lane >> 4;
-  const int64_t c0 = colptr[b];
-  const int64_t nch = (colptr[b + 1] - c0) >> 4;
+  // the item's chunks [4 q0, min(4 q1, chunks of the block)), as a block of
+  // its own: its first column and its chunk count
+  const int64_t nch_b = (colptr[b + 1] - colptr[b]) >> 4;
+  const int64_t ch0 = 4 * itm.q0;
+  const int64_t c0 = colptr[b] + 16 * ch0;
+  const int64_t nch = max((int64_t)0, min(4 * itm.q1, nch_b) - ch0);
   const int64_t nst = (nch + SC - 1) / SC;
+  const int64_t nh = (nrhs + 63) / 64;
   const int64_t row_base = b * kPanelRows;
   const unsigned short *Xs = reinterpret_cast<const unsigned short *>(X);
   const uint2 *P2 = reinterpret_cast<const uint2 *>(panel);
@@ -305,59 +388,37 @@ __global__ void __launch_bounds__(256, SC == 4 ? 3 : 2) k_panel_mm_bf16(
     if (s + 2 < nst) step(s + 2, p2, p1, xa, xb);
     if (s + 3 < nst) step(s + 3, p0, p2, xb, xa);
     if (s + 4 < nst) step(s + 4, p1, p0, xa, xb);
-    // epilogue: acc[t][q] = Y[row 16 w + li][rhs g0 + 16 t + 4 lg + q]
-    const int64_t r = row_base + w * 16 + li;
-    const bool row_ok = r < nrows;
-    const float rr = row_ok ? (rho ? rho[r] : rho_s) : 0.f;
-    for (int t = 0; t < 4; ++t) {
-      const int64_t cb = g0 + t * 16 + lg * 4;
-      double cm[4] = {0.0, 0.0, 0.0, 0.0};
-      if (row_ok && cb < nrhs) {
-        const int64_t o = r * nrhs + cb;
-        float v[4];
-        for (int q = 0; q < 4; ++q) v[q] = left ? acc[t][q] * rr : acc[t][q];
-        if (vec_x && cb + 3 < nrhs) {
-          if (base) {
-            const float4 bv = *reinterpret_cast<const float4 *>(base + o);
-            v[0] += bv.x; v[1] += bv.y; v[2] += bv.z; v[3] += bv.w;
-          }
-          *reinterpret_cast<float4 *>(out + o) = make_float4(v[0], v[1], v[2], v[3]);
-          if (xnext) {
-            const float s0 = left ? 1.f : rr;
-            unsigned short h[4];
-            for (int q = 0; q < 4; ++q) {
-              const __hip_bfloat16 hb = __float2bfloat16(s0 * v[q]);
-              h[q] = *reinterpret_cast<const unsigned short *>(&hb);
-            }
-            *reinterpret_cast<uint2 *>(xnext + o) =
-                make_uint2(h[0] | ((unsigned)h[1] << 16), h[2] | ((unsigned)h[3] << 16));
-          }
-          if (colmax) {
-            const float4 pv = *reinterpret_cast<const float4 *>(prev + o);
-            cm[0] = fabs((double)v[0] - (double)pv.x);
-            cm[1] = fabs((double)v[1] - (double)pv.y);
-            cm[2] = fabs((double)v[2] - (double)pv.z);
-            cm[3] = fabs((double)v[3] - (double)pv.w);
-          }
-        } else {
-          for (int q = 0; q < 4; ++q) {
-            if (cb + q >= nrhs) break;
-            if (base) v[q] += base[o + q];
-            out[o + q] = v[q];
-            if (xnext) xnext[o + q] = __float2bfloat16(left ? v[q] : rr * v[q]);
-            if (colmax) cm[q] = fabs((double)v[q] - (double)prev[o + q]);
-          }
-        }
-      }
-      if (colmax) {
-        for (int q = 0; q < 4; ++q) {
-          const double m = max16(cm[q]);
-          if (li == 0 && cb + q < nrhs)
-            atomicMax((unsigned long long *)(colmax + cb + q),
-                      (unsigned long long)__double_as_longlong(m));
-        }
-      }
+    if (itm.nsplit > 1) {
+      // partial of a split block: to its slot (k_panel_reduce_bf16 finishes)
+      float *slot = ws + ((int64_t)(itm.slot0 + itm.piece) * nh + g0 / 64) * 4096 + tid;
+      for (int t = 0; t < 4; ++t)
+        for (int q = 0; q < 4; ++q) slot[(4 * t + q) * 256] = acc[t][q];
+      continue;
     }
+    bf16_epilogue(acc, b, g0, nrows, nrhs, base, rho, rho_s, left, out, xnext, prev, colmax);
+  }
+}
+
+// Split blocks of a planned product: the sum of the block's partials, in
+// slot order, and the epilogue (one workgroup per split block).
+__global__ void __launch_bounds__(256) k_panel_reduce_bf16(
+    const int32_t *__restrict__ split_blocks, const int32_t *__restrict__ nsplit,
+    const int32_t *__restrict__ slot0, const float *__restrict__ ws, int64_t nrows,
+    int64_t nrhs, const float *__restrict__ base, const float *__restrict__ rho, float rho_s,
+    int left, float *__restrict__ out, __hip_bfloat16 *__restrict__ xnext,
+    const float *__restrict__ prev, double *__restrict__ colmax) {
+  const int64_t b = split_blocks[blockIdx.x];
+  const int ns = nsplit[b], s0 = slot0[b];
+  const int64_t nh = (nrhs + 63) / 64;
+  for (int64_t g0 = 0; g0 < nrhs; g0 += 64) {
+    v4f acc[4];
+    for (int t = 0; t < 4; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
+    for (int k = 0; k < ns; ++k) {
+      const float *src = ws + ((int64_t)(s0 + k) * nh + g0 / 64) * 4096 + threadIdx.x;
+      for (int t = 0; t < 4; ++t)
+        for (int q = 0; q < 4; ++q) acc[t][q] += src[(4 * t + q) * 256];
+    }
+    bf16_epilogue(acc, b, g0, nrows, nrhs, base, rho, rho_s, left, out, xnext, prev, colmax);
   }
 }
 
@@ -625,25 +686,99 @@ extern "C" int flux_panel_fill(int64_t nrows, int64_t ncols, const int64_t *indp
   return 0;
 }
 
+// Work plan of the panel products: every row block is cut into
+// ceil(quads / cap) items of balanced size (at least one, also for an empty
+// block), kept in block order so that xcd_block gives every XCD a
+// contiguous run of blocks (neighbouring blocks gather mostly the same X
+// rows, which then stay in that XCD's L2), and sorted longest first within
+// each XCD's run so the widest pieces start first and the short ones fill
+// the tail.  Host-side planning from colptr (read back once).
+extern "C" int flux_panel_plan(int64_t nblk, const int64_t *colptr, int64_t cap_quads,
+                               int32_t *items, int32_t *nsplit, int32_t *slot0,
+                               int32_t *split_blocks, int64_t *n_items, int64_t *n_slots,
+                               int64_t *n_split, void *stream) {
+  FLUX_CHECK(nblk >= 0 && cap_quads >= 1, "flux_panel_plan: bad sizes");
+  FLUX_CHECK(colptr && n_items && n_slots && n_split, "flux_panel_plan: NULL pointer");
+  hipStream_t st = as_stream(stream);
+  std::vector<int64_t> cp(nblk + 1);
+  FLUX_TRY_HIP(hipMemcpyAsync(cp.data(), colptr, (nblk + 1) * sizeof(int64_t),
+                              hipMemcpyDeviceToHost, st));
+  FLUX_TRY_HIP(hipStreamSynchronize(st));
+  std::vector<int32_t> its, ns(nblk), s0(nblk), sb;
+  int64_t slots = 0;
+  for (int64_t b = 0; b < nblk; ++b) {
+    const int64_t nq = (((cp[b + 1] - cp[b]) >> 4) + 3) >> 2;
+    const int64_t k = std::max<int64_t>(1, (nq + cap_quads - 1) / cap_quads);
+    FLUX_CHECK(nq < (1ll << 30) && slots + k < (1ll << 30), "flux_panel_plan: too many quads");
+    ns[b] = (int32_t)k;
+    s0[b] = k > 1 ? (int32_t)slots : -1;
+    if (k > 1) {
+      slots += k;
+      sb.push_back((int32_t)b);
+    }
+    const int64_t base = nq / k, rem = nq % k;
+    int64_t q = 0;
+    for (int64_t i = 0; i < k; ++i) {
+      const int64_t len = base + (i < rem ? 1 : 0);
+      its.insert(its.end(), {(int32_t)b, (int32_t)q, (int32_t)(q + len), (int32_t)i});
+      q += len;
+    }
+  }
+  const int64_t n = (int64_t)its.size() / 4;
+  std::vector<int64_t> order(n);
+  for (int64_t i = 0; i < n; ++i) order[i] = i;
+  const int64_t q = n / 8, r = n % 8;  // the runs of xcd_block
+  for (int64_t x = 0, lo = 0; x < 8; ++x) {
+    const int64_t hi = lo + q + (x < r ? 1 : 0);
+    std::stable_sort(order.begin() + lo, order.begin() + hi, [&](int64_t a, int64_t c) {
+      return its[4 * a + 2] - its[4 * a + 1] > its[4 * c + 2] - its[4 * c + 1];
+    });
+    lo = hi;
+  }
+  std::vector<int32_t> sorted(4 * n);
+  for (int64_t i = 0; i < n; ++i)
+    for (int c = 0; c < 4; ++c) sorted[4 * i + c] = its[4 * order[i] + c];
+  *n_items = n;
+  *n_slots = slots;
+  *n_split = (int64_t)sb.size();
+  if (items)
+    FLUX_TRY_HIP(hipMemcpyAsync(items, sorted.data(), sorted.size() * 4, hipMemcpyHostToDevice, st));
+  if (nsplit)
+    FLUX_TRY_HIP(hipMemcpyAsync(nsplit, ns.data(), ns.size() * 4, hipMemcpyHostToDevice, st));
+  if (slot0)
+    FLUX_TRY_HIP(hipMemcpyAsync(slot0, s0.data(), s0.size() * 4, hipMemcpyHostToDevice, st));
+  if (split_blocks && !sb.empty())
+    FLUX_TRY_HIP(
+        hipMemcpyAsync(split_blocks, sb.data(), sb.size() * 4, hipMemcpyHostToDevice, st));
+  FLUX_TRY_HIP(hipStreamSynchronize(st));
+  return 0;
+}
+
 extern "C" int flux_panel_mm_bf16(int64_t nrows, const int64_t *colptr, const int32_t *cols,
                                   const void *panel, const void *X, int64_t nrhs,
                                   const float *base, const float *rho, float rho_s, int left,
                                   float *out, void *xnext, const float *prev, double *colmax,
-                                  void *stream) {
+                                  const int32_t *items, int64_t n_items, const int32_t *nsplit,
+                                  const int32_t *slot0, const int32_t *split_blocks,
+                                  int64_t n_split, float *ws, void *stream) {
   FLUX_CHECK(nrows >= 0 && nrhs >= 0, "flux_panel_mm_bf16: bad sizes");
   const int64_t nblk = (nrows + kPanelRows - 1) / kPanelRows;
   if (nblk == 0 || nrhs == 0) return 0;
   FLUX_CHECK(colptr && cols && panel && X && out, "flux_panel_mm_bf16: NULL array");
   FLUX_CHECK(colmax == nullptr || prev != nullptr, "flux_panel_mm_bf16: colmax needs prev");
+  FLUX_CHECK(items == nullptr ||
+                 (n_items >= nblk && nsplit && slot0 && (n_split == 0 || (split_blocks && ws))),
+             "flux_panel_mm_bf16: a plan needs nsplit, slot0, split_blocks and ws");
+  const int64_t ngrid = items ? n_items : nblk;
   hipStream_t st = as_stream(stream);
   if (colmax)
     hipLaunchKernelGGL(k_zero_f64_p, dim3(ceil_div_u(nrhs, 256)), dim3(256), 0, st, colmax,
                        nrhs);
 #define MMB(SCV, VECV)                                                                         \
-  hipLaunchKernelGGL((k_panel_mm_bf16<SCV, VECV>), dim3((unsigned)nblk), dim3(256), 0, st, nrows, \
-                     nblk, colptr, cols, (const __hip_bfloat16 *)panel,                            \
+  hipLaunchKernelGGL((k_panel_mm_bf16<SCV, VECV>), dim3((unsigned)ngrid), dim3(256), 0, st,      \
+                     nrows, nblk, colptr, cols, (const __hip_bfloat16 *)panel,                     \
                      (const __hip_bfloat16 *)X, nrhs, base, rho, rho_s, left, out,                 \
-                     (__hip_bfloat16 *)xnext, prev, colmax)
+                     (__hip_bfloat16 *)xnext, prev, colmax, items, nsplit, slot0, ws, ngrid)
   const bool vec = (nrhs & 3) == 0;
   if (panel_stage() == 4) {
     if (vec)
@@ -657,6 +792,10 @@ extern "C" int flux_panel_mm_bf16(int64_t nrows, const int64_t *colptr, const in
       MMB(8, false);
   }
 #undef MMB
+  if (items && n_split > 0)
+    hipLaunchKernelGGL(k_panel_reduce_bf16, dim3((unsigned)n_split), dim3(256), 0, st,
+                       split_blocks, nsplit, slot0, ws, nrows, nrhs, base, rho, rho_s, left, out,
+                       (__hip_bfloat16 *)xnext, prev, colmax);
   FLUX_LAUNCH_CHECK();
   return 0;
 }

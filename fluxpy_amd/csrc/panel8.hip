@@ -235,13 +235,13 @@ __global__ void __launch_bounds__(256) k_slice_x(int64_t n, int64_t nrhs, int nh
 // registers: lane (row li, group lg) holds the 16 columns 16 lg .. 16 lg + 15
 // of the quad, i.e. chunk 4 q + lg, which is also what lane group lg of the
 // A operand holds).
-// Pipeline per stage (one quad) s: issue X(s+1), ids(s+3), panel(s+2);
+// Pipeline per stage (one quad) s: issue X(s+1), ids(s+3), panel(s+1);
 // MFMAs of stage s; transpose X(s+1) into LDS; store ids(s+3); one barrier.
-// Loads return in issue order, so the wait for X(s+1) at the end of stage s
-// also covers panel(s+1), issued after X(s) a stage earlier: the panel
-// stream (HBM) gets about two stages of latency, the X gathers (mostly
-// cache hits) one.  Registers: X of one stage, panel fragments of three
-// (p0 / p1 / p2); the stage loop is unrolled by the ring period.
+// Loads return in issue order; the wait for X(s+1) at the end of stage s
+// only covers loads issued before it, and panel(s+1) has the whole stage
+// (about 0.8 us at two waves per SIMD, above the loaded HBM latency of
+// ~0.6 us measured on MI355X) before stage s+1 uses it.  Registers: X of
+// one stage, panel fragments of two (p0 / p1).
 // X staging: 1280 units of (4 columns x 4 rhs x 1 slice) per stage, unit
 // u = tid + 256 m (m < 5): column group u / 80, slice (u % 80) / 16, rhs
 // group u % 16 (consecutive lanes read the 320 contiguous bytes of a column).
@@ -257,6 +257,51 @@ __device__ __forceinline__ void transpose4x4(const uint32_t (&wi)[4], uint32_t (
   wo[3] = __builtin_amdgcn_perm(t3, t1, 0x07060302u);
 }
 
+// Residual epilogue of tile t (right-hand sides 64 h + 16 t ..) of a
+// 64-row block: y[q] = the combined sums sum_d 2^-7d S_d for row 16 w + li
+// and rhs 64 h + 16 t + 4 lg + q (w = wave, li = lane % 16, lg = lane / 16).
+__device__ __forceinline__ void panel8_epilogue(
+    int t, double (&y)[4], int64_t b, int h, int64_t nrows, int64_t nrhs,
+    const int32_t *__restrict__ rowexp, const int32_t *__restrict__ xexp,
+    const double *__restrict__ E, const double *__restrict__ B, const double *__restrict__ rho,
+    double rho_s, int left, double *__restrict__ out, double *__restrict__ colmax) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int li = lane & 15, lg = lane >> 4;
+  const int64_t r = b * kPanelRows + 16 * w + li;
+  const bool row_ok = r < nrows;
+  const int er = row_ok ? rowexp[r] : 0;
+  const double rr = row_ok ? (rho ? rho[r] : rho_s) : 0.0;
+  const int64_t k0 = (int64_t)h * kGroup + 16 * t + 4 * lg;
+  double cm[4];
+  for (int q = 0; q < 4; ++q) {
+    const int64_t k = k0 + q;
+    y[q] = k < nrhs ? ldexp(y[q], er + xexp[k] - 12) : 0.0;
+    cm[q] = 0.0;
+  }
+  if (row_ok) {
+    const int64_t o = r * nrhs + k0;
+    if (E) {
+      const double sc = left ? rr : 1.0;
+      for (int q = 0; q < 4; ++q) {
+        if (k0 + q < nrhs) {
+          y[q] = E[o + q] + sc * y[q] - B[o + q];
+          cm[q] = fabs(y[q]);
+        }
+      }
+    }
+    for (int q = 0; q < 4; ++q)
+      if (k0 + q < nrhs) out[o + q] = y[q];
+  }
+  if (colmax && E) {
+    for (int q = 0; q < 4; ++q) {
+      const double m = max16(cm[q]);
+      if (li == 0 && k0 + q < nrhs)
+        atomicMax((unsigned long long *)(colmax + k0 + q),
+                  (unsigned long long)__double_as_longlong(m));
+    }
+  }
+}
+
 constexpr int kUnits = 5;  // X staging units per thread and stage
 
 __global__ void __launch_bounds__(256, 2) k_panel8_mm(
@@ -265,19 +310,23 @@ __global__ void __launch_bounds__(256, 2) k_panel8_mm(
     const int8_t *__restrict__ panel8, const int32_t *__restrict__ rowexp,
     const int8_t *__restrict__ x8, const int32_t *__restrict__ xexp, int64_t nrhs,
     const double *__restrict__ E, const double *__restrict__ B, const double *__restrict__ rho,
-    double rho_s, int left, double *__restrict__ out, double *__restrict__ colmax) {
+    double rho_s, int left, double *__restrict__ out, double *__restrict__ colmax,
+    const int32_t *__restrict__ items, const int32_t *__restrict__ nsplit,
+    const int32_t *__restrict__ slot0, double *__restrict__ ws, int64_t nitems) {
   __shared__ __align__(16) uint32_t xs[2][kSlices * kXsSlice];
   __shared__ __align__(16) int32_t ids[4][kQuadCols];
-  const int64_t item = xcd_block(blockIdx.x, nblk * nh);
-  const int64_t b = item / nh;
-  const int h = (int)(item - b * nh);
+  const int64_t pair = xcd_block(blockIdx.x, nitems * nh);
+  const int64_t it = pair / nh;
+  const int h = (int)(pair - it * nh);
+  const PanelItem itm = panel_item(items, nsplit, slot0, colptr, it);
+  const int64_t b = itm.b;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int li = lane & 15, lg = lane >> 4;
-  const int64_t c0 = colptr[b];
-  const int64_t ncl = colptr[b + 1] - c0;  // columns (multiple of 16)
-  const int64_t nq = ((ncl >> 4) + 3) >> 2;
-  const int64_t row_base = b * kPanelRows;
-  const v4i *pb = reinterpret_cast<const v4i *>(panel8 + qptr[b] * (int64_t)kQuadBytes +
+  // the item's quads [q0, q1) as a block of its own: first column, columns
+  const int64_t c0 = colptr[b] + kQuadCols * itm.q0;
+  const int64_t ncl = max((int64_t)0, min(colptr[b + 1], colptr[b] + kQuadCols * itm.q1) - c0);
+  const int64_t nq = itm.q1 - itm.q0;
+  const v4i *pb = reinterpret_cast<const v4i *>(panel8 + (qptr[b] + itm.q0) * kQuadBytes +
                                                 (16 * w + li) * kQuadCols + 16 * lg);
   const int8_t *xg = x8 + h * (int64_t)kXRow;
   const int64_t xstride = (int64_t)nh * kXRow;
@@ -320,23 +369,22 @@ __global__ void __launch_bounds__(256, 2) k_panel8_mm(
       for (int i = 0; i < kSlices; ++i) pf[i] = p[i * (kSliceTile / 16)];
     };
     uint32_t xr[kUnits][4];
-    v4i p0[kSlices], p1[kSlices], p2[kSlices];
+    v4i p0[kSlices], p1[kSlices];
     // prologue: ids(0..2) to LDS (every wave stores the same ids, so no
-    // branch sits next to a load), X(0) to LDS, panel(0), panel(1) in flight
+    // branch sits next to a load), X(0) to LDS, panel(0) in flight
     ids[0][lane] = load_id(0);
     ids[1][lane] = load_id(1);
     ids[2][lane] = load_id(2);
     __syncthreads();
     load_x(0, xr);
     load_p(0, p0);
-    load_p(1, p1);
     store_x(0, xr);
     __syncthreads();
     auto step = [&](int64_t s, const v4i (&pc)[kSlices], v4i (&pl)[kSlices]) {
       const int buf = (int)(s & 1);
       load_x(s + 1, xr);
       const int32_t idn = load_id(s + 3);
-      load_p(s + 2, pl);
+      load_p(s + 1, pl);
       // keep every load of the stage ahead of its MFMAs
       __builtin_amdgcn_sched_barrier(0);
       const uint32_t *xb_ = xs[buf];
@@ -354,51 +402,52 @@ __global__ void __launch_bounds__(256, 2) k_panel8_mm(
       __syncthreads();
     };
     int64_t s = 0;
-    for (; s + 3 <= nq; s += 3) {
-      step(s, p0, p2);
+    for (; s + 2 <= nq; s += 2) {
+      step(s, p0, p1);
       step(s + 1, p1, p0);
-      step(s + 2, p2, p1);
     }
-    if (s < nq) step(s, p0, p2);
-    if (s + 1 < nq) step(s + 1, p1, p0);
+    if (s < nq) step(s, p0, p1);
   }
-  // epilogue: lane l item q of tile t = S^T[rhs 16 t + 4 lg + q][row li]
-  const int64_t r = row_base + 16 * w + li;
-  const bool row_ok = r < nrows;
-  const int er = row_ok ? rowexp[r] : 0;
-  const double rr = row_ok ? (rho ? rho[r] : rho_s) : 0.0;
+  // lane l item q of tile t = S^T[rhs 16 t + 4 lg + q][row li]; the five
+  // sums combine exactly term by term in FP64
+  auto combine = [&](int t, int q) -> double {
+    double v = 0.0;
+    for (int d = kSlices - 1; d >= 0; --d) v += ldexp((double)acc[d][t][q], -7 * d);
+    return v;
+  };
+  if (itm.nsplit > 1) {
+    // partial of a split block: to its slot (k_panel8_reduce finishes)
+    double *slot = ws + ((int64_t)(itm.slot0 + itm.piece) * nh + h) * 4096 + tid;
+    for (int t = 0; t < 4; ++t)
+      for (int q = 0; q < 4; ++q) slot[(4 * t + q) * 256] = combine(t, q);
+    return;
+  }
   for (int t = 0; t < 4; ++t) {
-    const int64_t k0 = (int64_t)h * kGroup + 16 * t + 4 * lg;
-    double y[4], cm[4];
-    for (int q = 0; q < 4; ++q) {
-      double v = 0.0;
-      for (int d = kSlices - 1; d >= 0; --d) v += ldexp((double)acc[d][t][q], -7 * d);
-      const int64_t k = k0 + q;
-      y[q] = k < nrhs ? ldexp(v, er + xexp[k] - 12) : 0.0;
-      cm[q] = 0.0;
+    double y[4];
+    for (int q = 0; q < 4; ++q) y[q] = combine(t, q);
+    panel8_epilogue(t, y, b, h, nrows, nrhs, rowexp, xexp, E, B, rho, rho_s, left, out, colmax);
+  }
+}
+
+// Split blocks of a planned product: the sum of the block's partials, in
+// slot order, and the epilogue (one workgroup per split block and group of
+// 64 right-hand sides).
+__global__ void __launch_bounds__(256) k_panel8_reduce(
+    const int32_t *__restrict__ split_blocks, const int32_t *__restrict__ nsplit,
+    const int32_t *__restrict__ slot0, const double *__restrict__ ws, int nh, int64_t nrows,
+    const int32_t *__restrict__ rowexp, const int32_t *__restrict__ xexp, int64_t nrhs,
+    const double *__restrict__ E, const double *__restrict__ B, const double *__restrict__ rho,
+    double rho_s, int left, double *__restrict__ out, double *__restrict__ colmax) {
+  const int64_t b = split_blocks[blockIdx.x / nh];
+  const int h = (int)(blockIdx.x % nh);
+  const int ns = nsplit[b], s0 = slot0[b];
+  for (int t = 0; t < 4; ++t) {
+    double y[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int k = 0; k < ns; ++k) {
+      const double *src = ws + ((int64_t)(s0 + k) * nh + h) * 4096 + threadIdx.x;
+      for (int q = 0; q < 4; ++q) y[q] += src[(4 * t + q) * 256];
     }
-    if (row_ok) {
-      const int64_t o = r * nrhs + k0;
-      if (E) {
-        const double sc = left ? rr : 1.0;
-        for (int q = 0; q < 4; ++q) {
-          if (k0 + q < nrhs) {
-            y[q] = E[o + q] + sc * y[q] - B[o + q];
-            cm[q] = fabs(y[q]);
-          }
-        }
-      }
-      for (int q = 0; q < 4; ++q)
-        if (k0 + q < nrhs) out[o + q] = y[q];
-    }
-    if (colmax && E) {
-      for (int q = 0; q < 4; ++q) {
-        const double m = max16(cm[q]);
-        if (li == 0 && k0 + q < nrhs)
-          atomicMax((unsigned long long *)(colmax + k0 + q),
-                    (unsigned long long)__double_as_longlong(m));
-      }
-    }
+    panel8_epilogue(t, y, b, h, nrows, nrhs, rowexp, xexp, E, B, rho, rho_s, left, out, colmax);
   }
 }
 
@@ -500,13 +549,20 @@ extern "C" int flux_panel8_mm(int64_t nrows, const int64_t *colptr, const int32_
                               const int64_t *qptr, const int8_t *panel8, const int32_t *rowexp,
                               int64_t nrhs, const void *work, const double *E, const double *B,
                               const double *rho, double rho_s, int left, double *out,
-                              double *colmax, void *stream) {
+                              double *colmax, const int32_t *items, int64_t n_items,
+                              const int32_t *nsplit, const int32_t *slot0,
+                              const int32_t *split_blocks, int64_t n_split, double *ws,
+                              void *stream) {
   FLUX_CHECK(nrows >= 0 && nrhs >= 0, "flux_panel8_mm: bad sizes");
   const int64_t nblk = (nrows + kPanelRows - 1) / kPanelRows;
   if (nblk == 0 || nrhs == 0) return 0;
   FLUX_CHECK(colptr && cols && qptr && panel8 && rowexp && work && out,
              "flux_panel8_mm: NULL array");
   FLUX_CHECK(E == nullptr || B != nullptr, "flux_panel8_mm: E needs B");
+  FLUX_CHECK(items == nullptr ||
+                 (n_items >= nblk && nsplit && slot0 && (n_split == 0 || (split_blocks && ws))),
+             "flux_panel8_mm: a plan needs nsplit, slot0, split_blocks and ws");
+  const int64_t nit = items ? n_items : nblk;
   hipStream_t st = as_stream(stream);
   const int64_t nh = x8_groups(nrhs), np = nh * kGroup;
   const int32_t *xexp = (const int32_t *)work;
@@ -514,10 +570,14 @@ extern "C" int flux_panel8_mm(int64_t nrows, const int64_t *colptr, const int32_
   if (colmax)
     hipLaunchKernelGGL(k_zero_u32, dim3(ceil_div_u(nrhs * 2, 256)), dim3(256), 0, st,
                        (uint32_t *)colmax, nrhs * 2);
-  FLUX_CHECK(nblk * nh < (1ll << 31), "flux_panel8_mm: too many workgroups");
-  hipLaunchKernelGGL(k_panel8_mm, dim3((unsigned)(nblk * nh)), dim3(256), 0, st, nrows, nblk,
+  FLUX_CHECK(nit * nh < (1ll << 31), "flux_panel8_mm: too many workgroups");
+  hipLaunchKernelGGL(k_panel8_mm, dim3((unsigned)(nit * nh)), dim3(256), 0, st, nrows, nblk,
                      (int)nh, colptr, cols, qptr, panel8, rowexp, x8, xexp, nrhs, E, B, rho,
-                     rho_s, left, out, colmax);
+                     rho_s, left, out, colmax, items, nsplit, slot0, ws, nit);
+  if (items && n_split > 0)
+    hipLaunchKernelGGL(k_panel8_reduce, dim3((unsigned)(n_split * nh)), dim3(256), 0, st,
+                       split_blocks, nsplit, slot0, ws, (int)nh, nrows, rowexp, xexp, nrhs, E, B,
+                       rho, rho_s, left, out, colmax);
   FLUX_LAUNCH_CHECK();
   return 0;
 }

@@ -304,16 +304,37 @@ int flux_panel_fill(int64_t nrows, int64_t ncols, const int64_t *indptr,
                     const int64_t *colptr, int64_t total_cols, int32_t *cols,
                     void *panel, int pdtype, void *stream);
 
+/* Work plan of the panel products (load balance): every row block is cut
+ * into ceil(quads / cap_quads) items of balanced size, in block order with
+ * each eighth of the list (one XCD's contiguous run of workgroups) sorted
+ * longest first.  The items of a block cut into several pieces write
+ * partial sums to slots; a second launch sums them in slot order (so
+ * results do not depend on scheduling) and runs the epilogue.
+ * Outputs (device, may be NULL to query the sizes first): items (4 int32
+ * per item: block, first quad, end quad, piece), nsplit[nblk] (items of the
+ * block), slot0[nblk] (first slot of a split block, else -1), split_blocks
+ * (the split blocks).  Returns the item, slot and split-block counts on the
+ * host (synchronises).  Calls sharing a slot workspace must not overlap. */
+int flux_panel_plan(int64_t nblk, const int64_t *colptr, int64_t cap_quads,
+                    int32_t *items, int32_t *nsplit, int32_t *slot0,
+                    int32_t *split_blocks, int64_t *n_items, int64_t *n_slots,
+                    int64_t *n_split, void *stream);
+
 /* BF16 panels x BF16 X (ncols x nrhs) on v_mfma_f32_16x16x16_bf16:
  *   acc = P X;  out = base + s o acc  (s_i = rho_i if left, else 1;
  *   base NULL -> out = acc);  xnext = bf16(t o out) (t_i = rho_i if right,
  *   else 1; xnext may be NULL);  colmax[c] = max_i |out - prev| if colmax.
- * One Jacobi sweep of the correction equation per call. */
+ * One Jacobi sweep of the correction equation per call.  items == NULL: one
+ * workgroup per row block; with a plan, ws holds n_slots * ceil(nrhs / 64)
+ * * 4096 floats. */
 int flux_panel_mm_bf16(int64_t nrows, const int64_t *colptr, const int32_t *cols,
                        const void *panel, const void *X, int64_t nrhs,
                        const float *base, const float *rho, float rho_s,
                        int left, float *out, void *xnext, const float *prev,
-                       double *colmax, void *stream);
+                       double *colmax, const int32_t *items, int64_t n_items,
+                       const int32_t *nsplit, const int32_t *slot0,
+                       const int32_t *split_blocks, int64_t n_split, float *ws,
+                       void *stream);
 
 /* FP32 panels x FP64 X on v_mfma_f64_16x16x4_f64 (values widened):
  *   acc = P X; with E: out = E + s o acc - B and colmax[c] = max_i |out|
@@ -363,13 +384,17 @@ int flux_slice_x(int64_t n, int64_t nrhs, const double *X, const double *rho,
 
 /* acc = P X from the slices in `work`;  with E: out = E + s o acc - B
  * (s_i = rho_i if left, else 1) and colmax[c] = max_i |out| (the
- * refinement residual); without E: out = acc. */
+ * refinement residual); without E: out = acc.  Plan arguments as for
+ * flux_panel_mm_bf16 (items == NULL: one workgroup per row block); with a
+ * plan, ws holds n_slots * ceil(nrhs / 64) * 4096 doubles. */
 int flux_panel8_mm(int64_t nrows, const int64_t *colptr, const int32_t *cols,
                    const int64_t *qptr, const int8_t *panel8,
                    const int32_t *rowexp, int64_t nrhs, const void *work,
                    const double *E, const double *B, const double *rho,
                    double rho_s, int left, double *out, double *colmax,
-                   void *stream);
+                   const int32_t *items, int64_t n_items, const int32_t *nsplit,
+                   const int32_t *slot0, const int32_t *split_blocks,
+                   int64_t n_split, double *ws, void *stream);
 
 /* ---------------------------------------------------------------------- */
 /* Column-blocked vector kernels (solver building blocks; blocks are      */

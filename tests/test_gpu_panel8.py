@@ -38,6 +38,7 @@ def setup():
     csr = _kernels.ff_csr(sm, ct.perm, ct.perm)
     Pb = PanelMatrix(csr, "bf16")
     P8 = SlicedPanelMatrix(csr, Pb)
+    Pb._default_plan = Pb.plan
     return torch, sm, csr, Pb, P8, ct
 
 
@@ -201,3 +202,79 @@ def test_public_refined_solve_on_panels(oracle_mod, residual):
                                  inner_iters=2, residual=residual)
     assert isinstance(Bt, torch.Tensor)
     assert abs(Bt.cpu().numpy() - Bd).max() / abs(Bd).max() <= 1e-6
+
+
+def test_plan_covers_blocks(setup):
+    """The balanced plan tiles every block's quads exactly once, longest
+    pieces first, with consistent split counts and slot ranges."""
+    from fluxpy_amd._panel import PanelPlan
+    torch, sm, csr, Pb, P8, ct = setup
+    colptr = Pb.colptr.cpu().numpy()
+    nblk = len(colptr) - 1
+    nq = (np.diff(colptr) // 16 + 3) // 4
+    for cap in (1, 3, 48, 10 ** 6):
+        pl = PanelPlan(Pb.colptr, nblk, cap=cap)
+        it = pl.items.cpu().numpy()[:4 * pl.n_items].reshape(-1, 4)
+        ns = pl.nsplit.cpu().numpy()[:nblk]
+        s0 = pl.slot0.cpu().numpy()[:nblk]
+        size = it[:, 2] - it[:, 1]
+        assert size.max() <= max(cap, 1)
+        # each eighth (one XCD's run) is sorted longest first and holds a
+        # contiguous range of blocks
+        q, r = divmod(pl.n_items, 8)
+        lo = 0
+        for x in range(8):
+            hi = lo + q + (1 if x < r else 0)
+            assert (np.diff(size[lo:hi]) <= 0).all()
+            if hi > lo:
+                blk = np.unique(it[lo:hi, 0])
+                assert np.array_equal(blk, np.arange(blk.min(), blk.max() + 1))
+            lo = hi
+        assert np.array_equal(ns, np.maximum(1, -(-nq // cap)))
+        assert pl.n_items == ns.sum() and pl.n_slots == ns[ns > 1].sum()
+        assert pl.n_split == (ns > 1).sum()
+        if pl.n_split:
+            assert np.array_equal(pl.split_blocks.cpu().numpy()[:pl.n_split],
+                                  np.flatnonzero(ns > 1))
+        for b in range(nblk):
+            mine = it[it[:, 0] == b]
+            mine = mine[np.argsort(mine[:, 3])]
+            assert np.array_equal(mine[:, 3], np.arange(ns[b]))
+            assert mine[0, 1] == 0 and mine[-1, 2] == nq[b]
+            assert (mine[1:, 1] == mine[:-1, 2]).all()
+            assert (s0[b] == -1) == (ns[b] == 1)
+        used = np.concatenate([s0[b] + np.arange(ns[b]) for b in range(nblk) if ns[b] > 1] or
+                              [np.zeros(0, int)])
+        assert np.array_equal(np.sort(used), np.arange(pl.n_slots))
+
+
+@pytest.mark.parametrize("k", [64, 70])
+def test_planned_products_match_unplanned(setup, k):
+    """Split blocks (cap of 2 quads, so nearly every block is cut) give the
+    same products as one workgroup per block, and the same bits on a second
+    call."""
+    from fluxpy_amd._panel import PanelPlan
+    torch, sm, csr, Pb, P8, ct = setup
+    n = csr.shape[0]
+    rng = np.random.default_rng(50 + k)
+    X = torch.tensor(rng.standard_normal((n, k)), device=_dev())
+    Xb = X.to(torch.float32).to(torch.bfloat16)
+    nblk = (n + 63) // 64
+    tiny = PanelPlan(Pb.colptr, nblk, cap=2)
+    assert tiny.n_slots > 0
+    outs = {}
+    for name, plan in (("none", False), ("tiny", tiny), ("tiny2", tiny)):
+        Pb.plan = P8.plan = plan
+        y32 = torch.empty((n, k), dtype=torch.float32, device=_dev())
+        Pb.mm_bf16(Xb, y32)
+        y64 = torch.empty((n, k), dtype=torch.float64, device=_dev())
+        P8.mm_f64(X, y64)
+        torch.cuda.synchronize()
+        outs[name] = (y32.cpu().numpy(), y64.cpu().numpy())
+    Pb.plan = P8.plan = Pb.__dict__.get("_default_plan", tiny)
+    a32, a64 = outs["none"]
+    b32, b64 = outs["tiny"]
+    c32, c64 = outs["tiny2"]
+    assert np.array_equal(b32, c32) and np.array_equal(b64, c64)      # deterministic
+    assert np.allclose(b32, a32, rtol=1e-5, atol=1e-5 * abs(a32).max())
+    assert np.allclose(b64, a64, rtol=1e-14, atol=1e-14 * abs(a64).max())
