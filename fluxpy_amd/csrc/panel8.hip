@@ -260,11 +260,12 @@ __device__ __forceinline__ void transpose4x4(const uint32_t (&wi)[4], uint32_t (
 // Residual epilogue of tile t (right-hand sides 64 h + 16 t ..) of a
 // 64-row block: y[q] = the combined sums sum_d 2^-7d S_d for row 16 w + li
 // and rhs 64 h + 16 t + 4 lg + q (w = wave, li = lane % 16, lg = lane / 16).
+// With E, the wave's column maxima of |out| go to s_cm[w][16 t + 4 lg + q].
 __device__ __forceinline__ void panel8_epilogue(
     int t, double (&y)[4], int64_t b, int h, int64_t nrows, int64_t nrhs,
     const int32_t *__restrict__ rowexp, const int32_t *__restrict__ xexp,
     const double *__restrict__ E, const double *__restrict__ B, const double *__restrict__ rho,
-    double rho_s, int left, double *__restrict__ out, double *__restrict__ colmax) {
+    double rho_s, int left, double *__restrict__ out, double *s_cm) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int li = lane & 15, lg = lane >> 4;
   const int64_t r = b * kPanelRows + 16 * w + li;
@@ -292,14 +293,43 @@ __device__ __forceinline__ void panel8_epilogue(
     for (int q = 0; q < 4; ++q)
       if (k0 + q < nrhs) out[o + q] = y[q];
   }
-  if (colmax && E) {
+  if (E) {
     for (int q = 0; q < 4; ++q) {
       const double m = max16(cm[q]);
-      if (li == 0 && k0 + q < nrhs)
-        atomicMax((unsigned long long *)(colmax + k0 + q),
-                  (unsigned long long)__double_as_longlong(m));
+      if (li == 0) s_cm[w * kGroup + 16 * t + 4 * lg + q] = m;
     }
   }
+}
+
+// Column maxima of the block (64 rows) from the waves' s_cm: one row of
+// bcm per (block, group), no atomics; k_colmax_blocks reduces over blocks.
+__device__ __forceinline__ void panel8_block_colmax(const double *s_cm, int64_t b, int h, int nh,
+                                                    double *__restrict__ bcm) {
+  __syncthreads();
+  if (threadIdx.x < kGroup) {
+    const int c = threadIdx.x;
+    const double m = fmax(fmax(s_cm[c], s_cm[kGroup + c]),
+                          fmax(s_cm[2 * kGroup + c], s_cm[3 * kGroup + c]));
+    bcm[(b * nh + h) * kGroup + c] = m;
+  }
+}
+
+// colmax[k] = max over blocks of bcm[(b nh + k / 64) 64 + k % 64].
+__global__ void __launch_bounds__(256) k_colmax_blocks(const double *__restrict__ bcm,
+                                                       int64_t nblk, int nh, int64_t nrhs,
+                                                       double *__restrict__ colmax) {
+  __shared__ double red[256];
+  const int64_t k = blockIdx.x;
+  const int64_t h = k / kGroup, c = k % kGroup;
+  double m = 0.0;
+  for (int64_t b = threadIdx.x; b < nblk; b += 256) m = fmax(m, bcm[(b * nh + h) * kGroup + c]);
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && k < nrhs) colmax[k] = red[0];
 }
 
 constexpr int kUnits = 5;  // X staging units per thread and stage
@@ -310,7 +340,7 @@ __global__ void __launch_bounds__(256, 2) k_panel8_mm(
     const int8_t *__restrict__ panel8, const int32_t *__restrict__ rowexp,
     const int8_t *__restrict__ x8, const int32_t *__restrict__ xexp, int64_t nrhs,
     const double *__restrict__ E, const double *__restrict__ B, const double *__restrict__ rho,
-    double rho_s, int left, double *__restrict__ out, double *__restrict__ colmax,
+    double rho_s, int left, double *__restrict__ out, double *__restrict__ bcm,
     const int32_t *__restrict__ items, const int32_t *__restrict__ nsplit,
     const int32_t *__restrict__ slot0, double *__restrict__ ws, int64_t nitems) {
   __shared__ __align__(16) uint32_t xs[2][kSlices * kXsSlice];
@@ -422,11 +452,13 @@ __global__ void __launch_bounds__(256, 2) k_panel8_mm(
       for (int q = 0; q < 4; ++q) slot[(4 * t + q) * 256] = combine(t, q);
     return;
   }
+  __shared__ double s_cm[4 * kGroup];
   for (int t = 0; t < 4; ++t) {
     double y[4];
     for (int q = 0; q < 4; ++q) y[q] = combine(t, q);
-    panel8_epilogue(t, y, b, h, nrows, nrhs, rowexp, xexp, E, B, rho, rho_s, left, out, colmax);
+    panel8_epilogue(t, y, b, h, nrows, nrhs, rowexp, xexp, E, B, rho, rho_s, left, out, s_cm);
   }
+  if (bcm && E) panel8_block_colmax(s_cm, b, h, nh, bcm);
 }
 
 // Split blocks of a planned product: the sum of the block's partials, in
@@ -437,7 +469,8 @@ __global__ void __launch_bounds__(256) k_panel8_reduce(
     const int32_t *__restrict__ slot0, const double *__restrict__ ws, int nh, int64_t nrows,
     const int32_t *__restrict__ rowexp, const int32_t *__restrict__ xexp, int64_t nrhs,
     const double *__restrict__ E, const double *__restrict__ B, const double *__restrict__ rho,
-    double rho_s, int left, double *__restrict__ out, double *__restrict__ colmax) {
+    double rho_s, int left, double *__restrict__ out, double *__restrict__ bcm) {
+  __shared__ double s_cm[4 * kGroup];
   const int64_t b = split_blocks[blockIdx.x / nh];
   const int h = (int)(blockIdx.x % nh);
   const int ns = nsplit[b], s0 = slot0[b];
@@ -447,8 +480,9 @@ __global__ void __launch_bounds__(256) k_panel8_reduce(
       const double *src = ws + ((int64_t)(s0 + k) * nh + h) * 4096 + threadIdx.x;
       for (int q = 0; q < 4; ++q) y[q] += src[(4 * t + q) * 256];
     }
-    panel8_epilogue(t, y, b, h, nrows, nrhs, rowexp, xexp, E, B, rho, rho_s, left, out, colmax);
+    panel8_epilogue(t, y, b, h, nrows, nrhs, rowexp, xexp, E, B, rho, rho_s, left, out, s_cm);
   }
+  if (bcm && E) panel8_block_colmax(s_cm, b, h, nh, bcm);
 }
 
 }  // namespace flux
@@ -516,11 +550,14 @@ extern "C" int flux_panel8_fill(int64_t nrows, int64_t ncols, const int64_t *ind
 
 static int64_t x8_groups(int64_t nrhs) { return (nrhs + kGroup - 1) / kGroup; }
 
-// workspace: xexp (nh * 64 int32), cmax (nh * 64 uint64), x8 (n * nh * 320 bytes)
+// workspace: xexp (nh * 64 int32), cmax (nh * 64 uint64), x8 (n * nh * 320
+// bytes), then the block column maxima of the product (ceil(n / 64) * nh *
+// 64 doubles; the product of a square matrix of n rows)
 extern "C" int64_t flux_slice_x_bytes(int64_t n, int64_t nrhs) {
   if (n < 0 || nrhs < 0) return -1;
   const int64_t nh = x8_groups(nrhs);
-  return nh * kGroup * (4 + 8) + n * nh * kXRow;
+  const int64_t nblk = (n + kPanelRows - 1) / kPanelRows;
+  return nh * kGroup * (4 + 8) + n * nh * kXRow + nblk * nh * kGroup * 8;
 }
 
 extern "C" int flux_slice_x(int64_t n, int64_t nrhs, const double *X, const double *rho,
@@ -567,17 +604,20 @@ extern "C" int flux_panel8_mm(int64_t nrows, const int64_t *colptr, const int32_
   const int64_t nh = x8_groups(nrhs), np = nh * kGroup;
   const int32_t *xexp = (const int32_t *)work;
   const int8_t *x8 = (const int8_t *)work + np * 12;
-  if (colmax)
-    hipLaunchKernelGGL(k_zero_u32, dim3(ceil_div_u(nrhs * 2, 256)), dim3(256), 0, st,
-                       (uint32_t *)colmax, nrhs * 2);
+  // block column maxima after x8 (the matrix is square: X has nrows rows)
+  double *bcm = (colmax && E) ? (double *)((char *)work + np * 12 + nrows * nh * kXRow)
+                              : nullptr;
   FLUX_CHECK(nit * nh < (1ll << 31), "flux_panel8_mm: too many workgroups");
   hipLaunchKernelGGL(k_panel8_mm, dim3((unsigned)(nit * nh)), dim3(256), 0, st, nrows, nblk,
                      (int)nh, colptr, cols, qptr, panel8, rowexp, x8, xexp, nrhs, E, B, rho,
-                     rho_s, left, out, colmax, items, nsplit, slot0, ws, nit);
+                     rho_s, left, out, bcm, items, nsplit, slot0, ws, nit);
   if (items && n_split > 0)
     hipLaunchKernelGGL(k_panel8_reduce, dim3((unsigned)(n_split * nh)), dim3(256), 0, st,
                        split_blocks, nsplit, slot0, ws, (int)nh, nrows, rowexp, xexp, nrhs, E, B,
-                       rho, rho_s, left, out, colmax);
+                       rho, rho_s, left, out, bcm);
+  if (bcm)
+    hipLaunchKernelGGL(k_colmax_blocks, dim3((unsigned)nrhs), dim3(256), 0, st, bcm, nblk,
+                       (int)nh, nrhs, colmax);
   FLUX_LAUNCH_CHECK();
   return 0;
 }

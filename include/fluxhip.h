@@ -374,7 +374,9 @@ int flux_panel8_fill(int64_t nrows, int64_t ncols, const int64_t *indptr,
                      const int64_t *qptr, int64_t total_quads, int32_t *rowexp,
                      int8_t *panel8, void *stream);
 
-/* Bytes of the X-slice workspace for n rows and nrhs right-hand sides. */
+/* Bytes of the X-slice workspace for n rows and nrhs right-hand sides (it
+ * also holds the column maxima of flux_panel8_mm on the square n x n
+ * panels). */
 int64_t flux_slice_x_bytes(int64_t n, int64_t nrhs);
 
 /* Slices of s o X (n x nrhs FP64; s_i = rho_i or rho_s if scale, else 1)
@@ -382,9 +384,10 @@ int64_t flux_slice_x_bytes(int64_t n, int64_t nrhs);
 int flux_slice_x(int64_t n, int64_t nrhs, const double *X, const double *rho,
                  double rho_s, int scale, void *work, void *stream);
 
-/* acc = P X from the slices in `work`;  with E: out = E + s o acc - B
- * (s_i = rho_i if left, else 1) and colmax[c] = max_i |out| (the
- * refinement residual); without E: out = acc.  Plan arguments as for
+/* acc = P X from the slices in `work` (P square: nrows x nrows);  with E:
+ * out = E + s o acc - B (s_i = rho_i if left, else 1) and colmax[c] =
+ * max_i |out| (the refinement residual, column maxima without atomics);
+ * without E: out = acc.  Plan arguments as for
  * flux_panel_mm_bf16 (items == NULL: one workgroup per row block); with a
  * plan, ws holds n_slots * ceil(nrhs / 64) * 4096 doubles. */
 int flux_panel8_mm(int64_t nrows, const int64_t *colptr, const int32_t *cols,
