@@ -247,6 +247,9 @@ class PanelRefinedSolver:
     def set_rho(self, rho):
         """Albedo: an n-vector device tensor (panel order) or a scalar."""
         t = _lib.torch()
+        if isinstance(rho, t.Tensor) and rho is getattr(self, "_rho_src", None):
+            return
+        self._rho_src = rho
         if isinstance(rho, t.Tensor):
             self.rho64 = rho.to(t.float64).reshape(-1).contiguous()
             self.rho32 = _kernels.convert(self.rho64, t.float32)
@@ -258,26 +261,29 @@ class PanelRefinedSolver:
     def solve(self, E, rtol=1e-7, inner_iters=4, max_rounds=30):
         """E: n x k FP64 device tensor in panel order.  Returns (B, info)."""
         t = _lib.torch()
-        emax = self.vo.maxabs(E).cpu().numpy()
-        thresh = rtol * np.where(emax > 0, emax, 1.0)
+        st = _lib.stream()
+        n, k = self.n, self.k
+        # max|E_c| stays on the device until the first residual is read back
+        emax_d = self.vo.maxabs(E)
+        emax = None
         B, R = self.B, self.R
-        B.zero_()
-        R.copy_(E)
+        src = E              # the residual of B = 0
         rounds = 0
         hist = []
         while True:
             rounds += 1
-            _lib.call("flux_vec_convert", R.numel(), R.data_ptr(), _lib.F64, self.R32.data_ptr(),
-                      _lib.F32, _lib.stream())
-            # D0 = R32; X0 = bf16(rho D0) (right) or bf16(D0) (left)
-            to_bf16_scaled(self.R32, self.rho32, self.rs, scale=not self.left, out=self.Xb[0])
+            # D0 = fp32(src); X0 = bf16(rho D0) (right) or bf16(D0) (left)
+            _lib.call("flux_refine_prepare", n, k, src.data_ptr(), _lib.ptr(self.rho32),
+                      float(self.rs), int(not self.left), self.R32.data_ptr(),
+                      self.Xb[0].data_ptr(), st)
             cur = 0
             for _ in range(inner_iters):
                 self.P_lo.mm_bf16(self.Xb[cur], self.D[cur], base=self.R32, rho=self.rho32,
                                   rho_s=self.rs, left=self.left, xnext=self.Xb[1 - cur])
                 cur = 1 - cur
             D = self.D[1 - cur]
-            _kernels.axpby(None, _kernels.convert(D, t.float64), None, B)
+            _lib.call("flux_refine_accumulate", n, k, D.data_ptr(), B.data_ptr(),
+                      int(rounds == 1), st)
             # residual R = E + F(rho B) - B  (right) / E + rho F B - B (left)
             rs = self.rho64 if self.rho64 is not None else self.rs
             if isinstance(self.P_hi, SlicedPanelMatrix):
@@ -287,9 +293,15 @@ class PanelRefinedSolver:
                 Xr = B if self.left else _kernels.rowscale(rs, B, out=self.Xr)
                 self.P_hi.mm_f64(Xr, R, E=E, B=B, rho=self.rho64, rho_s=self.rs,
                                  left=self.left, colmax=self.colmax)
-            rmax = self.colmax.cpu().numpy()
+            if emax is None:
+                both = t.cat([self.colmax, emax_d.reshape(-1)]).cpu().numpy()
+                rmax, emax = both[:k], both[k:]
+                thresh = rtol * np.where(emax > 0, emax, 1.0)
+            else:
+                rmax = self.colmax.cpu().numpy()
             hist.append(float((rmax / np.where(emax > 0, emax, 1.0)).max()))
             if np.all(rmax <= thresh) or rounds >= max_rounds:
                 break
+            src = R
         return B, {"rounds": rounds, "inner_iters": rounds * inner_iters,
                    "residual_history": hist}

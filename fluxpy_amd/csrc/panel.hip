@@ -413,10 +413,14 @@ __global__ void __launch_bounds__(256) k_panel_reduce_bf16(
   for (int64_t g0 = 0; g0 < nrhs; g0 += 64) {
     v4f acc[4];
     for (int t = 0; t < 4; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
+    // a slot's 16 values are loaded together; slots are summed in order
+#pragma unroll 2
     for (int k = 0; k < ns; ++k) {
       const float *src = ws + ((int64_t)(s0 + k) * nh + g0 / 64) * 4096 + threadIdx.x;
+      float v[16];
+      for (int e = 0; e < 16; ++e) v[e] = src[e * 256];
       for (int t = 0; t < 4; ++t)
-        for (int q = 0; q < 4; ++q) acc[t][q] += src[(4 * t + q) * 256];
+        for (int q = 0; q < 4; ++q) acc[t][q] += v[4 * t + q];
     }
     bf16_epilogue(acc, b, g0, nrows, nrhs, base, rho, rho_s, left, out, xnext, prev, colmax);
   }
@@ -613,6 +617,29 @@ __global__ void k_scale_to_bf16(int64_t n, int64_t nrhs, const float *__restrict
   const int64_t i = k / nrhs;
   const float s = scale ? (rho ? rho[i] : rho_s) : 1.0f;
   xb[k] = __float2bfloat16(s * x[k]);
+}
+
+// Start of a refinement round: r32 = fp32(r) (the correction's right-hand
+// side) and xb = bf16(s_i r32) (s_i = rho_i or rho_s if scale, else 1), the
+// first BF16 iterate.  Four consecutive entries per thread.
+__global__ void k_refine_prepare(int64_t n, int64_t nrhs, const double *__restrict__ r,
+                                 const float *__restrict__ rho, float rho_s, int scale,
+                                 float *__restrict__ r32, __hip_bfloat16 *__restrict__ xb) {
+  const int64_t total = n * nrhs;
+  const int64_t e0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
+  for (int64_t e = e0; e < total && e < e0 + 4; ++e) {
+    const float v = (float)r[e];
+    const float s = scale ? (rho ? rho[e / nrhs] : rho_s) : 1.0f;
+    r32[e] = v;
+    xb[e] = __float2bfloat16(s * v);
+  }
+}
+
+// End of the inner sweeps: b = d (assign) or b += d, FP32 d into FP64 b.
+__global__ void k_refine_accumulate(int64_t total, const float *__restrict__ d,
+                                    double *__restrict__ b, int assign) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < total) b[e] = assign ? (double)d[e] : b[e] + (double)d[e];
 }
 
 }  // namespace flux
@@ -820,6 +847,29 @@ extern "C" int flux_panel_mm_f64(int64_t nrows, const int64_t *colptr, const int
   else
     hipLaunchKernelGGL(k_panel_mm_f64<false>, dim3((unsigned)nblk), dim3(256), 0, st, nrows,
                        nblk, colptr, cols, panel, X, nrhs, E, B, rho, rho_s, left, out, colmax);
+  FLUX_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int flux_refine_prepare(int64_t n, int64_t nrhs, const double *r, const float *rho,
+                                   float rho_s, int scale, float *r32, void *xb, void *stream) {
+  FLUX_CHECK(n >= 0 && nrhs >= 0, "flux_refine_prepare: bad sizes");
+  if (n * nrhs == 0) return 0;
+  FLUX_CHECK(r && r32 && xb, "flux_refine_prepare: NULL array");
+  hipLaunchKernelGGL(k_refine_prepare, dim3(ceil_div_u(n * nrhs, 1024)), dim3(256), 0,
+                     as_stream(stream), n, nrhs, r, rho, rho_s, scale, r32,
+                     (__hip_bfloat16 *)xb);
+  FLUX_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int flux_refine_accumulate(int64_t n, int64_t nrhs, const float *d, double *b,
+                                      int assign, void *stream) {
+  FLUX_CHECK(n >= 0 && nrhs >= 0, "flux_refine_accumulate: bad sizes");
+  if (n * nrhs == 0) return 0;
+  FLUX_CHECK(d && b, "flux_refine_accumulate: NULL array");
+  hipLaunchKernelGGL(k_refine_accumulate, dim3(ceil_div_u(n * nrhs, 256)), dim3(256), 0,
+                     as_stream(stream), n * nrhs, d, b, assign);
   FLUX_LAUNCH_CHECK();
   return 0;
 }

@@ -172,37 +172,61 @@ __global__ void k_zero_u32(uint32_t *p, int64_t n) {
 // X slices
 // ---------------------------------------------------------------------------
 
-// cmax[k] = max_i |s_i x[i, k]| as the bit pattern of a non-negative double
-// (cmax zeroed beforehand).  Block (64, 4): rhs 64 blockIdx.y + threadIdx.x.
-__global__ void __launch_bounds__(256) k_colmax_scaled(int64_t n, int64_t nrhs,
+// pmax[blockIdx.x][k] = max over the block's rows of |s_i x[i, k]|.  Block
+// (64, 4): rhs 64 blockIdx.y + threadIdx.x; rows interleaved over the grid,
+// four independent running maxima per thread so the loads overlap.
+__global__ void __launch_bounds__(256) k_colmax_scaled(int64_t n, int64_t nrhs, int64_t np,
                                                        const double *__restrict__ x,
                                                        const double *__restrict__ rho,
                                                        double rho_s, int scale,
-                                                       unsigned long long *__restrict__ cmax) {
+                                                       double *__restrict__ pmax) {
   __shared__ double red[4][64];
   const int64_t k = (int64_t)blockIdx.y * 64 + threadIdx.x;
-  double m = 0.0;
+  const int64_t step = (int64_t)gridDim.x * 4;
+  double m[4] = {0.0, 0.0, 0.0, 0.0};
   if (k < nrhs) {
-    for (int64_t i = (int64_t)blockIdx.x * 4 + threadIdx.y; i < n; i += (int64_t)gridDim.x * 4) {
+    int64_t i = (int64_t)blockIdx.x * 4 + threadIdx.y;
+    for (; i + 3 * step < n; i += 4 * step)
+      for (int u = 0; u < 4; ++u) {
+        const int64_t r = i + u * step;
+        const double s = scale ? (rho ? rho[r] : rho_s) : 1.0;
+        m[u] = fmax(m[u], fabs(s * x[r * nrhs + k]));
+      }
+    for (; i < n; i += step) {
       const double s = scale ? (rho ? rho[i] : rho_s) : 1.0;
-      m = fmax(m, fabs(s * x[i * nrhs + k]));
+      m[0] = fmax(m[0], fabs(s * x[i * nrhs + k]));
     }
   }
-  red[threadIdx.y][threadIdx.x] = m;
+  red[threadIdx.y][threadIdx.x] = fmax(fmax(m[0], m[1]), fmax(m[2], m[3]));
   __syncthreads();
-  if (threadIdx.y == 0 && k < nrhs) {
-    m = fmax(fmax(red[0][threadIdx.x], red[1][threadIdx.x]),
-             fmax(red[2][threadIdx.x], red[3][threadIdx.x]));
-    atomicMax(cmax + k, (unsigned long long)__double_as_longlong(m));
+  if (threadIdx.y == 0 && k < np) {
+    const double v = fmax(fmax(red[0][threadIdx.x], red[1][threadIdx.x]),
+                          fmax(red[2][threadIdx.x], red[3][threadIdx.x]));
+    pmax[(int64_t)blockIdx.x * np + k] = v;
   }
+}
+
+// cmax[k] = max over the nb rows of pmax[.][k] (one workgroup per column).
+__global__ void __launch_bounds__(256) k_colmax_final(const double *__restrict__ pmax, int64_t nb,
+                                                      int64_t np, double *__restrict__ cmax) {
+  __shared__ double red[256];
+  const int64_t k = blockIdx.x;
+  double m = 0.0;
+  for (int64_t b = threadIdx.x; b < nb; b += 256) m = fmax(m, pmax[b * np + k]);
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) cmax[k] = red[0];
 }
 
 // x8 / xexp from s o x (rhs padded to nh * 32 with zeros).
 __global__ void __launch_bounds__(256) k_slice_x(int64_t n, int64_t nrhs, int nh,
                                                  const double *__restrict__ x,
                                                  const double *__restrict__ rho, double rho_s,
-                                                 int scale,
-                                                 const unsigned long long *__restrict__ cmax,
+                                                 int scale, const double *__restrict__ cmax,
                                                  int32_t *__restrict__ xexp,
                                                  int8_t *__restrict__ x8) {
   const int64_t np = (int64_t)nh * kGroup;
@@ -212,7 +236,7 @@ __global__ void __launch_bounds__(256) k_slice_x(int64_t n, int64_t nrhs, int nh
   int e = 0;
   double v = 0.0;
   if (k < nrhs) {
-    e = exp_of(__longlong_as_double((long long)cmax[k]));
+    e = exp_of(cmax[k]);
     const double s = scale ? (rho ? rho[i] : rho_s) : 1.0;
     v = s * x[i * nrhs + k];
   }
@@ -474,14 +498,21 @@ __global__ void __launch_bounds__(256) k_panel8_reduce(
   const int64_t b = split_blocks[blockIdx.x / nh];
   const int h = (int)(blockIdx.x % nh);
   const int ns = nsplit[b], s0 = slot0[b];
-  for (int t = 0; t < 4; ++t) {
-    double y[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int k = 0; k < ns; ++k) {
-      const double *src = ws + ((int64_t)(s0 + k) * nh + h) * 4096 + threadIdx.x;
-      for (int q = 0; q < 4; ++q) y[q] += src[(4 * t + q) * 256];
-    }
-    panel8_epilogue(t, y, b, h, nrows, nrhs, rowexp, xexp, E, B, rho, rho_s, left, out, s_cm);
+  // all 16 values of a slot are loaded before any is added, so a slot's
+  // loads are in flight together; the sums run over the slots in order
+  double y[4][4];
+  for (int t = 0; t < 4; ++t)
+    for (int q = 0; q < 4; ++q) y[t][q] = 0.0;
+#pragma unroll 2
+  for (int k = 0; k < ns; ++k) {
+    const double *src = ws + ((int64_t)(s0 + k) * nh + h) * 4096 + threadIdx.x;
+    double v[16];
+    for (int e = 0; e < 16; ++e) v[e] = src[e * 256];
+    for (int t = 0; t < 4; ++t)
+      for (int q = 0; q < 4; ++q) y[t][q] += v[4 * t + q];
   }
+  for (int t = 0; t < 4; ++t)
+    panel8_epilogue(t, y[t], b, h, nrows, nrhs, rowexp, xexp, E, B, rho, rho_s, left, out, s_cm);
   if (bcm && E) panel8_block_colmax(s_cm, b, h, nh, bcm);
 }
 
@@ -550,14 +581,17 @@ extern "C" int flux_panel8_fill(int64_t nrows, int64_t ncols, const int64_t *ind
 
 static int64_t x8_groups(int64_t nrhs) { return (nrhs + kGroup - 1) / kGroup; }
 
-// workspace: xexp (nh * 64 int32), cmax (nh * 64 uint64), x8 (n * nh * 320
+// workspace: xexp (nh * 64 int32), cmax (nh * 64 double), x8 (n * nh * 320
 // bytes), then the block column maxima of the product (ceil(n / 64) * nh *
-// 64 doubles; the product of a square matrix of n rows)
+// 64 doubles; the product of a square matrix of n rows), then the
+// per-workgroup maxima of the slicer (kColmaxGrid * nh * 64 doubles)
+constexpr int64_t kColmaxGrid = 512;
 extern "C" int64_t flux_slice_x_bytes(int64_t n, int64_t nrhs) {
   if (n < 0 || nrhs < 0) return -1;
   const int64_t nh = x8_groups(nrhs);
   const int64_t nblk = (n + kPanelRows - 1) / kPanelRows;
-  return nh * kGroup * (4 + 8) + n * nh * kXRow + nblk * nh * kGroup * 8;
+  return nh * kGroup * (4 + 8) + n * nh * kXRow + nblk * nh * kGroup * 8 +
+         kColmaxGrid * nh * kGroup * 8;
 }
 
 extern "C" int flux_slice_x(int64_t n, int64_t nrhs, const double *X, const double *rho,
@@ -569,13 +603,14 @@ extern "C" int flux_slice_x(int64_t n, int64_t nrhs, const double *X, const doub
   hipStream_t st = as_stream(stream);
   const int64_t nh = x8_groups(nrhs), np = nh * kGroup;
   int32_t *xexp = (int32_t *)work;
-  unsigned long long *cmax = (unsigned long long *)((char *)work + np * 4);
+  double *cmax = (double *)((char *)work + np * 4);
   int8_t *x8 = (int8_t *)((char *)work + np * 12);
-  hipLaunchKernelGGL(k_zero_u32, dim3(ceil_div_u(np * 2, 256)), dim3(256), 0, st,
-                     (uint32_t *)cmax, np * 2);
-  const unsigned gx = (unsigned)((n + 3) / 4 < 1024 ? (n + 3) / 4 : 1024);
-  hipLaunchKernelGGL(k_colmax_scaled, dim3(gx, ceil_div_u(nrhs, 64)), dim3(64, 4), 0, st, n,
-                     nrhs, X, rho, rho_s, scale, cmax);
+  const int64_t nblk = (n + kPanelRows - 1) / kPanelRows;
+  double *pmax = (double *)((char *)work + np * 12 + n * nh * kXRow + nblk * np * 8);
+  const int64_t gx = (n + 3) / 4 < kColmaxGrid ? (n + 3) / 4 : kColmaxGrid;
+  hipLaunchKernelGGL(k_colmax_scaled, dim3((unsigned)gx, (unsigned)nh), dim3(64, 4), 0, st, n,
+                     nrhs, np, X, rho, rho_s, scale, pmax);
+  hipLaunchKernelGGL(k_colmax_final, dim3((unsigned)np), dim3(256), 0, st, pmax, gx, np, cmax);
   hipLaunchKernelGGL(k_slice_x, dim3(ceil_div_u(n * np, 256)), dim3(256), 0, st, n, nrhs,
                      (int)nh, X, rho, rho_s, scale, cmax, xexp, x8);
   FLUX_LAUNCH_CHECK();

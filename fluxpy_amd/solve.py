@@ -317,7 +317,18 @@ def _solve_panels(FF, E2, rho, left, rtol, inner_iters, max_refine, residual):
     t = _lib.torch()
     n, k = E2.shape
     op = device_operator(FF)
-    rho64 = _rho_internal(op, _prep_rho(rho, n, t.float64))
+    # the albedo in panel order is kept while the caller's tensor is unchanged
+    # (same storage, same version counter)
+    cache = FF.__dict__.setdefault("_panel_rho", {})
+    rkey = ((rho.data_ptr(), rho._version, tuple(rho.shape), rho.dtype)
+            if isinstance(rho, t.Tensor) else None)
+    if rkey is not None and cache.get("key") == rkey:
+        rho64 = cache["rho64"]
+    else:
+        rho64 = _rho_internal(op, _prep_rho(rho, n, t.float64))
+        cache.clear()
+        if rkey is not None:
+            cache.update(key=rkey, rho64=rho64)
     Ei = _to_internal(op, E2)
     P_lo, P_hi = _panel_engine(FF, residual)
     solvers = FF.__dict__.setdefault("_panel_solvers", {})
