@@ -37,6 +37,7 @@
 namespace flux {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v2i __attribute__((ext_vector_type(2)));
 
 constexpr int kSlices = 5;
 constexpr int kQuadCols = 64;
@@ -44,18 +45,22 @@ constexpr int kSliceTile = kPanelRows * kQuadCols;    // 4096 bytes
 constexpr int kQuadBytes = kSlices * kSliceTile;      // 20480 bytes
 constexpr int kGroup = 64;                            // right-hand sides per workgroup
 constexpr int kXRow = kSlices * kGroup;               // 320 bytes per (facet, group)
-// LDS tile of one stage, in 4-byte words: slice j, right-hand side row
-// (0..63), columns 4 cw .. 4 cw + 3 at
-//   j * kXsSlice + row * 16 + 4 ((cw >> 2) ^ swz(row)) + (cw & 3),
-//   swz(row) = ((row >> 1) ^ (row >> 3)) & 3:
-// the 16-byte groups of a row are XOR-swizzled so that the A-fragment reads
-// (16 consecutive rows, one group each) cover every bank exactly once per
-// phase, for 32 banks (8-lane phases) and for 64 banks (16-lane phases);
-// slices are skewed by 16 words, which keeps the transposed 4-byte stores
-// within 4 lanes per bank.
-constexpr int kXsSlice = kGroup * 16 + 16;            // words per slice (1040)
+// LDS tile of one stage, as gathered (no transposition on the way in):
+// slice j, column c of the quad, the 64 right-hand-side bytes at
+//   j * kXsSlice + c * kXsRow  (bytes; rows padded to 80 bytes and slices
+// skewed by 64, so that the transposing reads below hit each bank once per
+// 16-lane group and the 16-byte stores at most twice per phase).
+constexpr int kXsRow = 80;
+constexpr int kXsSlice = kQuadCols * kXsRow + 64;     // bytes per slice (5184)
 
-__device__ __forceinline__ int xs_swz(int row) { return ((row >> 1) ^ (row >> 3)) & 3; }
+// ds_read_b64_tr_b8 (gfx950): within each group of 16 lanes, byte e of lane
+// i's result is byte (i & 7) of the 8 bytes lane 2 e + (i >> 3) read.  With
+// lane j reading column 8 h + (j >> 1) at right-hand sides 16 t + 8 (j & 1)
+// .. + 7, lane i receives columns 8 h .. 8 h + 7 of right-hand side 16 t + i:
+// the 8 K values an int8 MFMA A operand needs, from a column-major tile.
+__device__ __forceinline__ v2i lds_read_tr8(const uint8_t *p) {
+  return __builtin_amdgcn_ds_read_tr8_b64_v2i32((__attribute__((address_space(3))) v2i *)p);
+}
 constexpr int64_t kMaxBlockCols = 104832;             // int32 bound of the sums S_d
 
 // a_0..a_4 of v with |v| < 2^e (see the header).
@@ -266,20 +271,10 @@ __global__ void __launch_bounds__(256) k_slice_x(int64_t n, int64_t nrhs, int nh
 // (about 0.8 us at two waves per SIMD, above the loaded HBM latency of
 // ~0.6 us measured on MI355X) before stage s+1 uses it.  Registers: X of
 // one stage, panel fragments of two (p0 / p1).
-// X staging: 1280 units of (4 columns x 4 rhs x 1 slice) per stage, unit
-// u = tid + 256 m (m < 5): column group u / 80, slice (u % 80) / 16, rhs
-// group u % 16 (consecutive lanes read the 320 contiguous bytes of a column).
-__device__ __forceinline__ void transpose4x4(const uint32_t (&wi)[4], uint32_t (&wo)[4]) {
-  // wi[c] = bytes (rhs 0..3) of column c  ->  wo[r] = bytes (column 0..3) of rhs r
-  const uint32_t t0 = __builtin_amdgcn_perm(wi[1], wi[0], 0x05010400u);  // c0r0 c1r0 c0r1 c1r1
-  const uint32_t t1 = __builtin_amdgcn_perm(wi[1], wi[0], 0x07030602u);  // c0r2 c1r2 c0r3 c1r3
-  const uint32_t t2 = __builtin_amdgcn_perm(wi[3], wi[2], 0x05010400u);
-  const uint32_t t3 = __builtin_amdgcn_perm(wi[3], wi[2], 0x07030602u);
-  wo[0] = __builtin_amdgcn_perm(t2, t0, 0x05040100u);
-  wo[1] = __builtin_amdgcn_perm(t2, t0, 0x07060302u);
-  wo[2] = __builtin_amdgcn_perm(t3, t1, 0x05040100u);
-  wo[3] = __builtin_amdgcn_perm(t3, t1, 0x07060302u);
-}
+// X staging: 1280 chunks of 16 bytes (16 rhs of one slice of one column)
+// per stage, chunk u = tid + 256 m (m < 5): column u / 20, slice
+// (u % 20) / 4, quarter u % 4 (consecutive lanes read the 320 contiguous
+// bytes of a column and write them unchanged to LDS).
 
 // Residual epilogue of tile t (right-hand sides 64 h + 16 t ..) of a
 // 64-row block: y[q] = the combined sums sum_d 2^-7d S_d for row 16 w + li
@@ -367,7 +362,7 @@ __global__ void __launch_bounds__(256, 2) k_panel8_mm(
     double rho_s, int left, double *__restrict__ out, double *__restrict__ bcm,
     const int32_t *__restrict__ items, const int32_t *__restrict__ nsplit,
     const int32_t *__restrict__ slot0, double *__restrict__ ws, int64_t nitems) {
-  __shared__ __align__(16) uint32_t xs[2][kSlices * kXsSlice];
+  __shared__ __align__(16) uint8_t xs[2][kSlices * kXsSlice];
   __shared__ __align__(16) int32_t ids[4][kQuadCols];
   const int64_t pair = xcd_block(blockIdx.x, nitems * nh);
   const int64_t it = pair / nh;
@@ -392,37 +387,27 @@ __global__ void __launch_bounds__(256, 2) k_panel8_mm(
     auto load_id = [&](int64_t s) -> int32_t {
       return cols[c0 + min(s * kQuadCols + lane, ncl - 1)];
     };
-    auto load_x = [&](int64_t s, uint32_t (&xr)[kUnits][4]) {
+    auto load_x = [&](int64_t s, v4i (&xr)[kUnits]) {
       const int32_t *id = ids[s & 3];
       for (int m = 0; m < kUnits; ++m) {
         const int u = tid + 256 * m;
-        const int cg = u / 80, jr = u - cg * 80;
-        const int j = jr >> 4, rg = jr & 15;
-        const v4i cid = *reinterpret_cast<const v4i *>(id + 4 * cg);
-        for (int c = 0; c < 4; ++c)
-          xr[m][c] = *reinterpret_cast<const uint32_t *>(xg + (int64_t)cid[c] * xstride +
-                                                         j * kGroup + 4 * rg);
+        const int c = u / 20, ch = u - c * 20;
+        xr[m] = *reinterpret_cast<const v4i *>(xg + (int64_t)id[c] * xstride + 16 * ch);
       }
     };
-    auto store_x = [&](int buf, const uint32_t (&xr)[kUnits][4]) {
+    auto store_x = [&](int buf, const v4i (&xr)[kUnits]) {
       for (int m = 0; m < kUnits; ++m) {
         const int u = tid + 256 * m;
-        const int cg = u / 80, jr = u - cg * 80;
-        const int j = jr >> 4, rg = jr & 15;
-        uint32_t wo[4];
-        transpose4x4(xr[m], wo);
-        uint32_t *d = &xs[buf][j * kXsSlice + (cg & 3)];
-        for (int r = 0; r < 4; ++r) {
-          const int row = 4 * rg + r;
-          d[row * 16 + 4 * ((cg >> 2) ^ xs_swz(row))] = wo[r];
-        }
+        const int c = u / 20, ch = u - c * 20;
+        *reinterpret_cast<v4i *>(&xs[buf][(ch >> 2) * kXsSlice + c * kXsRow + 16 * (ch & 3)]) =
+            xr[m];
       }
     };
     auto load_p = [&](int64_t s, v4i (&pf)[kSlices]) {
       const v4i *p = pb + min(s, nq - 1) * (kQuadBytes / 16);
       for (int i = 0; i < kSlices; ++i) pf[i] = p[i * (kSliceTile / 16)];
     };
-    uint32_t xr[kUnits][4];
+    v4i xr[kUnits];
     v4i p0[kSlices], p1[kSlices];
     // prologue: ids(0..2) to LDS (every wave stores the same ids, so no
     // branch sits next to a load), X(0) to LDS, panel(0) in flight
@@ -441,12 +426,14 @@ __global__ void __launch_bounds__(256, 2) k_panel8_mm(
       load_p(s + 1, pl);
       // keep every load of the stage ahead of its MFMAs
       __builtin_amdgcn_sched_barrier(0);
-      const uint32_t *xb_ = xs[buf];
+      // A operand of tile t, slice j: columns 16 lg + 8 h + e (h = 0, 1) of
+      // right-hand side 16 t + li, two transposing reads
+      const uint8_t *xq = xs[buf] + (16 * lg + (li >> 1)) * kXsRow + 8 * (li & 1);
       for (int t = 0; t < 4; ++t) {
-        const int row = 16 * t + li;
-        const uint32_t *xq = xb_ + row * 16 + 4 * (lg ^ xs_swz(row));
         for (int j = 0; j < kSlices; ++j) {
-          const v4i a = *reinterpret_cast<const v4i *>(xq + j * kXsSlice);
+          const uint8_t *pa = xq + j * kXsSlice + 16 * t;
+          const v2i a0 = lds_read_tr8(pa), a1 = lds_read_tr8(pa + 8 * kXsRow);
+          const v4i a = v4i{a0[0], a0[1], a1[0], a1[1]};
           for (int i = 0; i + j < kSlices; ++i)
             acc[i + j][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, pc[i], acc[i + j][t], 0, 0, 0);
         }
