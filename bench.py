@@ -33,6 +33,14 @@ sys.path.insert(0, ROOT)
 
 METRIC = "radiosity solves/s (40k tris, 64 suns) BF16+refine vs FP64 @1e-6; Mrays/s"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes
+# on this workload (each scaled by the factor a plain 512 MiB copy shows:
+# FETCH_SIZE x 2, WRITE_SIZE x 1); the numbers and the commands are in
+# profiles/panel_kernels_fractal41k_64suns.md.  Kernels not measured: None.
+MEASURED_TRAFFIC = {
+    "k_panel8_mm + k_panel8_reduce + k_colmax_blocks": 2114790246,
+    "k_panel_mm_bf16 + k_panel_reduce_bf16": 737875218,
+}
 
 
 def log(*a):
@@ -195,54 +203,66 @@ def main():
             f"max rel err vs FP64 {err_alt:.2e}")
 
     # ---------------- roofline of the dominant kernel ----------------
+    # Both product kernels of a round are timed with HIP events on the stream
+    # they run on; the roofline reports the one with the larger share of a
+    # solve step (launch time x launches per step), the other goes beside it.
+    # Algorithmic bytes per launch (stored element sizes; panel zeros and
+    # cache re-reads not counted):
+    #   BF16 sweep:    nnz (2 value + 4 index) + 4 (n + 1)
+    #                  + k n (2 X in + 4 out + 4 base in + 2 next X out)
+    #   int8 residual: nnz (5 slice bytes + 4 index) + 4 (n + 1)
+    #                  + k n (5 X slices + 8 E + 8 B + 8 out)
     k = args.suns
     reps = 20
-    res_us = None
+    kernels = {}
+
+    def time_launch(fn):
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        ev0.record()
+        for _ in range(reps):
+            fn()
+        ev1.record()
+        torch.cuda.synchronize()
+        return ev0.elapsed_time(ev1) / reps * 1e3          # us per launch
+
     if pure_near:
         Xb = torch.zeros((nf, k), dtype=torch.bfloat16, device=dev)
         Xn = torch.empty_like(Xb)
         out = torch.empty((nf, k), dtype=torch.float32, device=dev)
         base = torch.zeros_like(out)
         rho32 = rho_d.to(torch.float32)
-        for _ in range(3):
-            Pbf.mm_bf16(Xb, out, base=base, rho=rho32, xnext=Xn)
-        torch.cuda.synchronize()
-        ev0.record()
-        for _ in range(reps):
-            Pbf.mm_bf16(Xb, out, base=base, rho=rho32, xnext=Xn)
-        ev1.record()
-        torch.cuda.synchronize()
-        kname = "k_panel_mm_bf16"
-        k_ms = ev0.elapsed_time(ev1) / reps
-        # the residual kernel beside it (X slicing included)
+        us = time_launch(lambda: Pbf.mm_bf16(Xb, out, base=base, rho=rho32, xnext=Xn))
+        kernels["k_panel_mm_bf16 + k_panel_reduce_bf16"] = {
+            "us": us, "per_step": info["rounds"] * args.inner,
+            "bytes": nnz * (2 + 4) + 4 * (nf + 1) + k * nf * (2 + 4 + 4 + 2)}
+        del Xb, Xn, out, base
         P8 = _panel_engine(H, "int8")[1]
-        X64 = torch.ones((nf, k), dtype=torch.float64, device=dev)
+        X64 = torch.rand((nf, k), dtype=torch.float64, device=dev)
         R64 = torch.empty_like(X64)
         cmx = torch.empty(k, dtype=torch.float64, device=dev)
-        for _ in range(3):
-            P8.mm_f64(X64, R64, E=X64, B=X64, colmax=cmx, xscale=rho_d)
-        torch.cuda.synchronize()
-        ev0.record()
-        for _ in range(reps):
-            P8.mm_f64(X64, R64, E=X64, B=X64, colmax=cmx, xscale=rho_d)
-        ev1.record()
-        torch.cuda.synchronize()
-        res_us = ev0.elapsed_time(ev1) / reps * 1e3
-        log(f"[bench] int8-sliced residual (slice X + k_panel8_mm): {res_us:.1f} us")
+        P8.slice_x(X64, xscale=rho_d)
+        us = time_launch(lambda: P8.product(k, R64, E=X64, B=X64, colmax=cmx))
+        kernels["k_panel8_mm + k_panel8_reduce + k_colmax_blocks"] = {
+            "us": us, "per_step": info["rounds"] if args.residual == "int8" else 0,
+            "bytes": nnz * (5 + 4) + 4 * (nf + 1) + k * nf * (5 + 8 + 8 + 8)}
+        us_slice = time_launch(lambda: P8.slice_x(X64, xscale=rho_d))
+        log(f"[bench] X slicing (k_colmax_scaled + k_colmax_final + k_slice_x): {us_slice:.1f} us")
         del X64, R64
     else:
-        ev0.record()
-        for _ in range(reps):
-            opbf.apply(_kernels.convert(E64, torch.float32))
-        ev1.record()
-        torch.cuda.synchronize()
-        kname = "k_csr_rows<bf16> + far field"
-        k_ms = ev0.elapsed_time(ev1) / reps
-    # algorithmic H-matvec bytes: near nnz (bf16 value + int32 index), indptr,
-    # x in (bf16) and y out (fp32), plus the fused epilogue (base in, x next out)
-    alg_bytes = nnz * (2 + 4) + 4 * (nf + 1) + k * nf * (2 + 4) + k * nf * (4 + 2)
-    achieved = alg_bytes / (k_ms * 1e-3) / 1e9
-    log(f"[bench] {kname}: {k_ms * 1e3:.1f} us/launch, {achieved:.0f} GB/s algorithmic")
+        us = time_launch(lambda: opbf.apply(_kernels.convert(E64, torch.float32)))
+        kernels["k_csr_rows<bf16> + far field"] = {
+            "us": us, "per_step": info["rounds"] * args.inner,
+            "bytes": nnz * (2 + 4) + 4 * (nf + 1) + k * nf * (2 + 4)}
+    for name, kv in kernels.items():
+        kv["GB/s"] = kv["bytes"] / (kv["us"] * 1e-6) / 1e9
+        log(f"[bench] {name}: {kv['us']:.1f} us/launch, {kv['GB/s']:.0f} GB/s algorithmic, "
+            f"{kv['per_step']} launches per step")
+    kname = max(kernels, key=lambda n: kernels[n]["us"] * max(kernels[n]["per_step"], 1e-9))
+    k_ms = kernels[kname]["us"] * 1e-3
+    alg_bytes = kernels[kname]["bytes"]
+    achieved = kernels[kname]["GB/s"]
 
     if args.calib:
         # a plain device copy of known size: its FETCH_SIZE / WRITE_SIZE counters
@@ -294,10 +314,13 @@ def main():
         "compress_s": round(t_build, 3),
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": MEASURED_TRAFFIC.get(kname),
             "kernel": kname, "kernel_us": round(k_ms * 1e3, 2), "algorithmic_bytes": alg_bytes,
         },
-        "residual_kernel_us": None if res_us is None else round(res_us, 2),
+        "kernels": {n: {"us": round(v["us"], 2), "GB/s": round(v["GB/s"], 1),
+                        "launches_per_step": v["per_step"], "algorithmic_bytes": v["bytes"]}
+                    for n, v in kernels.items()},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -189,13 +189,24 @@ class SlicedPanelMatrix:
         """out = P (s X), or with E: out = E + r (P s X) - B and colmax = max|out|
         (r_i = rho_i if left; s_i = xscale_i, a tensor or scalar, or 1)."""
         k = X.shape[1] if X.dim() == 2 else 1
-        work = self.workspace(k)
-        st = _lib.stream()
+        self.slice_x(X, xscale)
+        return self.product(k, out, E=E, B=B, rho=rho, rho_s=rho_s, left=left, colmax=colmax)
+
+    def slice_x(self, X, xscale=None):
+        """Slices of s X into the workspace (the first half of mm_f64)."""
         t = _lib.torch()
+        k = X.shape[1] if X.dim() == 2 else 1
+        work = self.workspace(k)
         xs_t = xscale if isinstance(xscale, t.Tensor) else None
         xs_s = 1.0 if xscale is None or xs_t is not None else float(xscale)
         _lib.call("flux_slice_x", self.shape[1], k, X.data_ptr(), _lib.ptr(xs_t), xs_s,
-                  int(xscale is not None), work.data_ptr(), st)
+                  int(xscale is not None), work.data_ptr(), _lib.stream())
+
+    def product(self, k, out, E=None, B=None, rho=None, rho_s=1.0, left=False, colmax=None):
+        """The product from the slices last written by slice_x (second half
+        of mm_f64)."""
+        work = self.workspace(k)
+        st = _lib.stream()
         plan = self.plan.args("int8", k, 8) if self.plan else _NO_PLAN
         _lib.call("flux_panel8_mm", self.shape[0], self.colptr.data_ptr(), self.cols.data_ptr(),
                   self.qptr.data_ptr(), self.panel.data_ptr(), self.rowexp.data_ptr(), k,
