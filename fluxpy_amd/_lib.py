@@ -77,7 +77,7 @@ SIGNATURES = {
                                    _c_int, _vp, _vp, _vp]),
     "flux_refine_prepare": (_c_int, [_c_i64, _c_i64, _vp, _vp, ctypes.c_float, _c_int, _vp,
                                      _vp, _vp]),
-    "flux_refine_accumulate": (_c_int, [_c_i64, _c_i64, _vp, _vp, _c_int, _vp]),
+    "flux_refine_accumulate": (_c_int, [_c_i64, _c_i64, _vp, _vp, _vp, _vp]),
     "flux_scale_to_bf16": (_c_int, [_c_i64, _c_i64, _vp, _vp, ctypes.c_float, _c_int, _vp,
                                     _vp]),
     "flux_panel8_layout": (_c_int, [_c_i64, _vp, _vp, _vp, _vp, _vp]),

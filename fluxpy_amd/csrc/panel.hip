@@ -635,11 +635,12 @@ __global__ void k_refine_prepare(int64_t n, int64_t nrhs, const double *__restri
   }
 }
 
-// End of the inner sweeps: b = d (assign) or b += d, FP32 d into FP64 b.
+// End of the inner sweeps: b_out = b_in + d (b_in NULL: b_out = d), FP32 d
+// into FP64 (b_in may equal b_out).
 __global__ void k_refine_accumulate(int64_t total, const float *__restrict__ d,
-                                    double *__restrict__ b, int assign) {
+                                    const double *b_in, double *b_out) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e < total) b[e] = assign ? (double)d[e] : b[e] + (double)d[e];
+  if (e < total) b_out[e] = (b_in ? b_in[e] : 0.0) + (double)d[e];
 }
 
 }  // namespace flux
@@ -863,13 +864,13 @@ extern "C" int flux_refine_prepare(int64_t n, int64_t nrhs, const double *r, con
   return 0;
 }
 
-extern "C" int flux_refine_accumulate(int64_t n, int64_t nrhs, const float *d, double *b,
-                                      int assign, void *stream) {
+extern "C" int flux_refine_accumulate(int64_t n, int64_t nrhs, const float *d,
+                                      const double *b_in, double *b_out, void *stream) {
   FLUX_CHECK(n >= 0 && nrhs >= 0, "flux_refine_accumulate: bad sizes");
   if (n * nrhs == 0) return 0;
-  FLUX_CHECK(d && b, "flux_refine_accumulate: NULL array");
+  FLUX_CHECK(d && b_out, "flux_refine_accumulate: NULL array");
   hipLaunchKernelGGL(k_refine_accumulate, dim3(ceil_div_u(n * nrhs, 256)), dim3(256), 0,
-                     as_stream(stream), n * nrhs, d, b, assign);
+                     as_stream(stream), n * nrhs, d, b_in, b_out);
   FLUX_LAUNCH_CHECK();
   return 0;
 }

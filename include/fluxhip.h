@@ -349,11 +349,12 @@ int flux_panel_mm_f64(int64_t nrows, const int64_t *colptr, const int32_t *cols,
  *   prepare:    r32 = fp32(r), xb = bf16(s o r32)  (s_i = rho_i or rho_s if
  *               scale, else 1): the FP32 correction right-hand side and the
  *               first BF16 iterate from the FP64 residual r;
- *   accumulate: b = d (assign) or b += d, FP32 d into FP64 b. */
+ *   accumulate: b_out = b_in + d (b_in NULL: b_out = d), FP32 d into FP64
+ *               (b_in may equal b_out). */
 int flux_refine_prepare(int64_t n, int64_t nrhs, const double *r, const float *rho,
                         float rho_s, int scale, float *r32, void *xb, void *stream);
-int flux_refine_accumulate(int64_t n, int64_t nrhs, const float *d, double *b,
-                           int assign, void *stream);
+int flux_refine_accumulate(int64_t n, int64_t nrhs, const float *d, const double *b_in,
+                           double *b_out, void *stream);
 
 /* xb = bf16(s o x), s_i = rho_i (or rho_s) if scale, else 1. */
 int flux_scale_to_bf16(int64_t n, int64_t nrhs, const float *x,
