@@ -62,6 +62,8 @@ def main():
                     help="refinement residual: int8-sliced panels on int8 MFMA "
                          "(FP64-accurate) or FP32 panels on FP64 MFMA")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-thermal", action="store_true",
+                    help="skip the time-dependent thermal model measurement")
     ap.add_argument("--calib", action="store_true",
                     help="end with a 512 MiB device copy (counter calibration)")
     ap.add_argument("--lowrank", action="store_true",
@@ -276,6 +278,8 @@ def main():
             f"{dst.numel() * 4} bytes written per launch")
         del src, dst
 
+    thermal = None if args.no_thermal else thermal_steps(H, sm, rho, ev0, ev1)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(sm, H, E64, rho, V, F)
@@ -321,9 +325,64 @@ def main():
         "kernels": {n: {"us": round(v["us"], 2), "GB/s": round(v["GB/s"], 1),
                         "launches_per_step": v["per_step"], "algorithmic_bytes": v["bytes"]}
                     for n, v in kernels.items()},
+        "thermal": thermal,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def thermal_steps(H, sm, rho, ev0, ev1, nsteps=20):
+    """The time-dependent thermal model (model.py:108-234) on the same terrain
+    and F: hourly steps of a sun circling at 10 degrees elevation, 60
+    subsurface layers.  One step = device insolation, the two-column product
+    F [E + Qrefl, emiss sigma T^4 + (1 - emiss) QIR], the flux update and the
+    conduction of every facet, with the step's status read back.  Reports
+    the wall time per step and the conduction launch alone (HIP events); the
+    conduction's algorithmic bytes per launch are T in + T out = 2 x 8 B x
+    61 layers per facet, plus the elimination multipliers written and read
+    once (2 x 8 B x 60) and 5 per-facet vectors."""
+    import torch
+    from fluxpy_amd import _lib
+    from fluxpy_amd.model import ThermalModel
+    from fluxpy_amd.thermal import setgrid
+    nt = nsteps + 4
+    t = 3600.0 * np.arange(nt)
+    az = 2 * np.pi * np.arange(nt) / 24
+    el = np.deg2rad(10.0)
+    Ds = np.stack([np.cos(el) * np.cos(az), np.cos(el) * np.sin(az),
+                   np.full(nt, np.sin(el))], 1)
+    nz = 60
+    z = np.concatenate([[0.0], setgrid(nz, 1.05, 2.5)])
+    tm = ThermalModel(H, t, Ds, np.full(nt, 1365.0), rho, device=True, z=z, T0=200.0,
+                      ti=120.0, rhoc=960000.0, emiss=0.95, Fgeotherm=0.2, shape_model=sm)
+    for _ in range(3):
+        next(tm)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        next(tm)
+    torch.cuda.synchronize()
+    step_ms = (time.perf_counter() - t0) / nsteps * 1e3
+    pcc = tm.pcc_thermal_model_1d
+    nf = pcc.num_faces
+    Q = tm._Q.clone()
+    reps = 20
+    ev0.record()
+    for _ in range(reps):
+        _lib.call("flux_conduction_q", nf, nz, pcc._z.data_ptr(), 3600.0, Q.data_ptr(),
+                  Q.data_ptr(), pcc._T.data_ptr(), pcc._T_next.data_ptr(), pcc._ti.data_ptr(),
+                  pcc._rhoc.data_ptr(), pcc._emiss.data_ptr(), pcc._Fgeotherm.data_ptr(),
+                  pcc._Fsurf_next.data_ptr(), pcc._work.data_ptr(), pcc._status.data_ptr(),
+                  _lib.stream())
+    ev1.record()
+    torch.cuda.synchronize()
+    us = ev0.elapsed_time(ev1) / reps * 1e3
+    nbytes = nf * (2 * 8 * (nz + 1) + 2 * 8 * nz + 5 * 8)
+    log(f"[bench] thermal model: {step_ms:.3f} ms/step; flux_conduction_q {us:.1f} us "
+        f"({nbytes / us / 1e3:.0f} GB/s algorithmic), {nf} facets x {nz} layers")
+    return {"step_ms": round(step_ms, 4), "layers": nz, "facets": nf,
+            "conduction_us": round(us, 2), "conduction_bytes": nbytes,
+            "conduction_GBps": round(nbytes / us / 1e3, 1)}
 
 
 def cpu_baseline(sm, H, E64, rho, V, F):

@@ -7,6 +7,7 @@ of fluxpy, behind fluxpy's own module names:
     fluxpy_amd.quadtree / octree        (flux.quadtree / flux.octree)
     fluxpy_amd.linalg / util            (flux.linalg / flux.util)
     fluxpy_amd.solve / model            (flux.solve / flux.model)
+    fluxpy_amd.thermal                  (flux.thermal)
     fluxpy_amd.config                   (flux.config)
 
 Compute runs in libfluxhip.so (hand-written HIP for gfx950) through a C ABI

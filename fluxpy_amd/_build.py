@@ -4,6 +4,7 @@ Every source compiles on its own (no relocatable device code) and links into
 one shared library beside this file.  Sources whose results are compared bit
 for bit with the CPU oracle (geometry, ray tracing, form factors) are
 compiled with -ffp-contract=off so no multiply-add is fused.
+(thermal.hip: the subsurface conduction.)
 """
 import concurrent.futures as cf
 import os
@@ -28,7 +29,7 @@ BASE_FLAGS = [
 ]
 
 # Bit-exact parity sources: no FMA contraction.
-EXACT = {"scene.hip", "raytrace.hip", "formfactor.hip"}
+EXACT = {"scene.hip", "raytrace.hip", "formfactor.hip", "thermal.hip"}
 
 
 def _hipcc():

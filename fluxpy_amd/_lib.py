@@ -96,6 +96,15 @@ SIGNATURES = {
     "flux_vec_convert": (_c_int, [_c_i64, _vp, _c_int, _vp, _c_int, _vp]),
     "flux_temperature": (_c_int, [_c_i64, _c_i64, _vp, _vp, _vp, _c_dbl, _vp, _c_dbl, _c_dbl,
                                   _c_int, _c_dbl, _vp, _c_int, _vp]),
+    "flux_conduction_work_bytes": (_c_i64, [_c_i64, _c_int]),
+    "flux_conduction_q": (_c_int, [_c_i64, _c_int, _vp, _c_dbl, _vp, _vp, _vp, _vp, _vp, _vp,
+                                   _vp, _vp, _vp, _vp, _vp, _vp]),
+    "flux_conduction_t": (_c_int, [_c_i64, _c_int, _vp, _c_dbl, _vp, _vp, _vp, _vp, _vp, _vp,
+                                   _vp, _vp, _vp, _vp, _vp]),
+    "flux_thermal_sources": (_c_int, [_c_i64, _vp, _vp, _vp, _vp, _vp, _c_dbl, _c_int, _vp,
+                                      _vp]),
+    "flux_thermal_update": (_c_int, [_c_i64, _vp, _c_int, _vp, _vp, _vp, _c_int, _vp, _vp, _vp,
+                                     _vp]),
 }
 
 

@@ -451,6 +451,53 @@ int flux_temperature(int64_t n, int64_t nrhs, const void *B, const void *Q,
                      double emiss_s, double fsurf, int clamp, double sigma,
                      void *T, int dtype, void *stream);
 
+/* ---- Time-dependent thermal model --------------------------------------
+ * One step of 1-D subsurface heat conduction for nf facets at once.  It
+ * replaces the per-facet loop of thermal.pyx:218-246, which calls
+ * conductionQ (pcc/conductionQ.c:10-186) or conductionT
+ * (pcc/conductionT.c:4-80) and their Thomas solve (pcc/tridag.c:5-34).
+ *   z, ti, rhoc: nz + 1 device doubles, z[0] = 0 (the surface); nz >= 2.
+ *   T_in, T_out: (nz + 1) x nf layer-major, T[l nf + f]. The step reads
+ *                T_in and writes T_out, and the two must differ.
+ *   Fsurf: nf heat fluxes into the surface (output).
+ *   work: flux_conduction_work_bytes(nf, nz) bytes of device scratch.
+ *   status: 2 device ints, written by the step. status[0] is a bit set:
+ *     1 nonfinite input flux/temperature, 2 negative input, 4 zero first
+ *     pivot, 8 zero pivot, 16 nonfinite T, 32 negative T. status[1] counts
+ *     the facets whose linearisation was repeated 9 or more times.
+ * The reference exits the process on a zero pivot. Here the caller reads
+ * status and raises.
+ * flux_conduction_q: radiative surface, Qn / Qnp1 = absorbed flux at steps
+ *   n and n+1, emissivity and geothermal flux per facet.
+ * flux_conduction_t: prescribed surface temperatures Tsurf / Tsurfp1.
+ *   T_out[0] = T_in[0], as the reference leaves T[0]. */
+int64_t flux_conduction_work_bytes(int64_t nf, int nz);
+int flux_conduction_q(int64_t nf, int nz, const double *z, double dt, const double *Qn,
+                      const double *Qnp1, const double *T_in, double *T_out,
+                      const double *ti, const double *rhoc, const double *emiss,
+                      const double *Fgeotherm, double *Fsurf, void *work, int *status,
+                      void *stream);
+int flux_conduction_t(int64_t nf, int nz, const double *z, double dt, const double *T_in,
+                      double *T_out, const double *Tsurf, const double *Tsurfp1,
+                      const double *ti, const double *rhoc, const double *Fgeotherm,
+                      double *Fsurf, void *work, int *status, void *stream);
+
+/* Radiance update of the thermal model (model.py:44-69 with model.py:224),
+ * around one product Y = F X with two right-hand sides:
+ *   sources: X[i,0] = E_i + Qrefl_i (sigma: the Stefan-Boltzmann constant
+ *            the caller uses, scipy's in model.py:3),
+ *            X[i,1] = emiss_i sigma Tsurf_i^4 + (1 - emiss_i) QIR_i;
+ *            Qrefl and QIR are read as 0 when `first` (X is n x 2, FP64).
+ *   update:  Qrefl = rho Y[:,0], QIR = Y[:,1] (each clamped at 0 if clamp),
+ *            Q = (1 - rho)(E + Qrefl) + emiss QIR. Y is n x 2, FP32 or FP64.
+ * All vectors are n device doubles. */
+int flux_thermal_sources(int64_t n, const double *E, const double *Qrefl, const double *QIR,
+                         const double *Tsurf, const double *emiss, double sigma, int first,
+                         double *X, void *stream);
+int flux_thermal_update(int64_t n, const void *Y, int y_dtype, const double *E,
+                        const double *rho, const double *emiss, int clamp, double *Qrefl,
+                        double *QIR, double *Q, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

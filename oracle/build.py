@@ -16,10 +16,10 @@ LIB = os.path.join(HERE, "liboracle.so")
 
 
 def build(verbose=False):
-    src = os.path.join(HERE, "flux_oracle.c")
+    srcs = [os.path.join(HERE, f) for f in ("flux_oracle.c", "pcc_oracle.c")]
     tmp = LIB + ".tmp"
     cmd = ["gcc", "-O2", "-std=c99", "-fopenmp", "-fPIC", "-shared", "-ffp-contract=off",
-           "-fno-fast-math", "-Wall", "-Wno-unused-function", src, "-o", tmp, "-lm"]
+           "-fno-fast-math", "-Wall", "-Wno-unused-function", *srcs, "-o", tmp, "-lm"]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -14,6 +14,11 @@ Contents and what each part restates (paths relative to src/flux/):
 * ``solve_radiosity``     solve.py:4-45 (Jacobi left/right, reference
                           stopping rule) and a dense FP64 LU solve
 * ``steady_state_temp``   model.py:8-42
+* ``conduction_q``, ``conduction_t``
+                          pcc/conductionQ.c:10-186, pcc/conductionT.c:4-80,
+                          pcc/tridag.c:5-34 -- C code in pcc_oracle.c
+* ``thermal_model_run``   model.py:108-234 (ThermalModel iteration) over
+                          a SciPy F and the conduction above
 
 What pins each part (beyond restating it) is listed in tests/test_oracle.py:
 closed forms, the sphere invariants of tests/test_shape.py /
@@ -51,6 +56,11 @@ def lib():
         L.oracle_intersect1.argtypes = [vp, vp, vp, i64, vp, vp]
         L.oracle_ff_csr.argtypes = [vp, vp, vp, vp, i32, vp, i64, vp, i64, ctypes.c_double,
                                     i32, vp, vp, vp, vp]
+        dbl = ctypes.c_double
+        L.oracle_conduction_q.restype = i32
+        L.oracle_conduction_q.argtypes = [i64, i32, vp, dbl, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.oracle_conduction_t.restype = i32
+        L.oracle_conduction_t.argtypes = [i64, i32, vp, dbl, vp, vp, vp, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -230,3 +240,91 @@ def steady_state_temp(F, E, rho, emiss, Fsurf=0.0, clamp=True, solver=None):
             Q = np.maximum(0, Q)
         T.append((((1 - rho) * B + emiss * Q) / (emiss * SIGMA)) ** 0.25)
     return np.array(T)
+
+
+# --------------------------------------------------------------------------
+# Time-dependent thermal model: 1-D conduction per facet and the iteration.
+# --------------------------------------------------------------------------
+
+def _f64(a, n=None):
+    """Contiguous FP64 array; a scalar becomes n copies when n is given."""
+    a = np.asarray(a, dtype=np.float64)
+    if n is not None and a.ndim == 0:
+        a = np.full(n, float(a))
+    a = np.ascontiguousarray(a)
+    if n is not None and a.size != n:
+        raise ValueError(f"expected {n} values, got {a.size}")
+    return a
+
+
+def conduction_q(z, dt, Qn, Qnp1, T, ti, rhoc, emiss, Fgeotherm):
+    """pcc/conductionQ.c:10-186 for every row of T (nf x (nz + 1), updated in
+    place).  Returns (Fsurf, iters): the surface heat flux and the number of
+    predictor-corrector repeats per facet.  Raises on a zero pivot (the
+    reference exits the process, tridag.c:16-28)."""
+    z = _f64(z)
+    nz = z.size - 1
+    assert T.dtype == np.float64 and T.flags.c_contiguous and T.shape[1] == nz + 1
+    nf = T.shape[0]
+    Qn, Qnp1 = _f64(Qn, nf), _f64(Qnp1, nf)
+    emiss, Fg = _f64(emiss, nf), _f64(Fgeotherm, nf)
+    ti, rhoc = _f64(ti, nz + 1), _f64(rhoc, nz + 1)
+    Fsurf = np.empty(nf)
+    iters = np.zeros(nf, dtype=np.int32)
+    rc = lib().oracle_conduction_q(nf, nz, _p(z), float(dt), _p(Qn), _p(Qnp1), _p(T), _p(ti),
+                                   _p(rhoc), _p(emiss), _p(Fg), _p(Fsurf), _p(iters))
+    if rc:
+        raise RuntimeError("tridag: rewrite equations" if rc == 1 else "tridag failed")
+    return Fsurf, iters
+
+
+def conduction_t(z, dt, T, Tsurf, Tsurfp1, ti, rhoc, Fgeotherm):
+    """pcc/conductionT.c:4-80 for every row of T (updated in place; T[:, 0]
+    is left as it is, as in the reference).  Returns Fsurf."""
+    z = _f64(z)
+    nz = z.size - 1
+    assert T.dtype == np.float64 and T.flags.c_contiguous and T.shape[1] == nz + 1
+    nf = T.shape[0]
+    Ts, Ts1, Fg = _f64(Tsurf, nf), _f64(Tsurfp1, nf), _f64(Fgeotherm, nf)
+    ti, rhoc = _f64(ti, nz + 1), _f64(rhoc, nz + 1)
+    Fsurf = np.empty(nf)
+    rc = lib().oracle_conduction_t(nf, nz, _p(z), float(dt), _p(T), _p(Ts), _p(Ts1), _p(ti),
+                                   _p(rhoc), _p(Fg), _p(Fsurf))
+    if rc:
+        raise RuntimeError("tridag: rewrite equations" if rc == 1 else "tridag failed")
+    return Fsurf
+
+
+def thermal_model_run(F, E_steps, t, rho, emiss, z, T0, ti, rhoc, Fgeotherm, E0,
+                      out_dtype=np.float64, return_flux=False):
+    """model.py:108-234 (one sun per time): the ThermalModel iteration.
+
+    F: SciPy CSR (N x N) in the model dtype; E_steps[i] the direct
+    irradiance at time t[i] (what get_direct_irradiance returns, i = 1..);
+    E0 the irradiance of the initial state.  ``F @ x`` is rounded to
+    ``out_dtype`` as the reference's compressed operator does
+    (compressed_form_factors.py:408).  Yields T (N x nz+1) after each step,
+    or (T, E, Qrefl, QIR) with return_flux."""
+    nf = F.shape[0]
+    rho = np.asarray(rho, dtype=np.float64)
+    emiss = _f64(emiss, nf)
+    Fg = _f64(Fgeotherm, nf)
+    Q0 = (1 - rho) * E0 + Fg
+    Tsurf = (Q0 / (SIGMA * emiss)) ** 0.25
+    T = np.ascontiguousarray(np.broadcast_to(np.asarray(T0, dtype=np.float64),
+                                             (nf, np.size(z))).copy())
+    Qprev = Q0
+    Qrefl = QIR = 0
+    apply = lambda x: np.asarray(F @ x).astype(out_dtype)
+    for i in range(1, len(t)):
+        E = E_steps[i]
+        dt = t[i] - t[i - 1]
+        Qrefl = rho * apply(E + Qrefl)          # model.py:60-63 (clamp)
+        Qrefl = np.maximum(0, Qrefl)
+        QIR = apply(emiss * SIGMA * Tsurf ** 4 + (1 - emiss) * QIR)
+        QIR = np.maximum(0, QIR)
+        Q = (1 - rho) * (E + Qrefl) + emiss * QIR
+        conduction_q(z, dt, Qprev, Q, T, ti, rhoc, emiss, Fg)
+        Qprev = Q
+        Tsurf = T[:, 0].copy()
+        yield (T.copy(), E, Qrefl, QIR) if return_flux else T.copy()
