@@ -65,11 +65,31 @@ __global__ void __launch_bounds__(256) k_csr_rows(
     const bool active = col < nrhs;
     TAcc acc = 0;
     if (active) {
-      for (int64_t k = k0 + g; k < k1; k += G) {
-        const int64_t j = indices[k];
-        TAcc x = (TAcc)X[j * nrhs + col];
-        if (MODE == SPMM_JACOBI_RIGHT) x = (rho ? (TAcc)rho[j] : (TAcc)rho_s) * x;
-        acc += (TAcc)widen(data[k]) * x;
+      // kU consecutive strides per pass: the index and value loads of a pass,
+      // then its gathers, are all in flight before the first product (each
+      // lane still adds its terms in increasing k order)
+      constexpr int kU = 8;
+      for (int64_t kb = k0 + g; kb < k1; kb += (int64_t)kU * G) {
+        int32_t jj[kU];
+        TAcc av[kU], xv[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+          const int64_t k = kb + (int64_t)u * G;
+          jj[u] = k < k1 ? indices[k] : -1;
+          av[u] = k < k1 ? (TAcc)widen(data[k]) : (TAcc)0;
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+          TAcc x = 0;
+          if (jj[u] >= 0) {
+            x = (TAcc)X[(int64_t)jj[u] * nrhs + col];
+            if (MODE == SPMM_JACOBI_RIGHT) x = (rho ? (TAcc)rho[jj[u]] : (TAcc)rho_s) * x;
+          }
+          xv[u] = x;
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+          if (jj[u] >= 0) acc += av[u] * xv[u];
       }
     }
     for (int off = R; off < 64; off <<= 1) acc += __shfl_xor(acc, off, 64);

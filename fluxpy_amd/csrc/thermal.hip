@@ -94,19 +94,16 @@ __device__ __forceinline__ double conduction_solve(int64_t nf, int64_t f, int nz
   Tout[nf + f] = u;
   double tm = Tin[nf + f];                       // T_{j-1}
   double tc = Tin[2 * nf + f];                   // T_j
-  for (int j = 2; j <= nz; ++j) {
+  // rows 2 .. nz-1; the loads of Tin do not depend on the recurrence, so the
+  // unrolled body issues them ahead of the divisions
+#pragma unroll 8
+  for (int j = 2; j < nz; ++j) {
+    const double tp = Tin[(int64_t)(j + 1) * nf + f];
     const double aj = -gamma[j];
-    double bj, rj;
-    if (j < nz) {
-      const double tp = Tin[(int64_t)(j + 1) * nf + f];
-      bj = 1. + alpha[j] + gamma[j];
-      rj = gamma[j] * tm + (1 - alpha[j] - gamma[j]) * tc + alpha[j] * tp;
-      tm = tc;
-      tc = tp;
-    } else {
-      bj = 1. + gamma[nz];
-      rj = gamma[nz] * tm + (1. - gamma[nz]) * tc + dt / rhoc[nz] * fgeo / (z[nz] - z[nz - 1]);
-    }
+    const double bj = 1. + alpha[j] + gamma[j];
+    const double rj = gamma[j] * tm + (1 - alpha[j] - gamma[j]) * tc + alpha[j] * tp;
+    tm = tc;
+    tc = tp;
     const double gam = -alpha[j - 1] / bet;
     bet = bj - aj * gam;
     if (bet == 0.0) bad |= kPivot;
@@ -114,8 +111,21 @@ __device__ __forceinline__ double conduction_solve(int64_t nf, int64_t f, int nz
     gamw[(int64_t)j * nf + f] = gam;
     Tout[(int64_t)j * nf + f] = u;
   }
+  {  // bottom row: geothermal flux
+    const double aj = -gamma[nz];
+    const double bj = 1. + gamma[nz];
+    const double rj =
+        gamma[nz] * tm + (1. - gamma[nz]) * tc + dt / rhoc[nz] * fgeo / (z[nz] - z[nz - 1]);
+    const double gam = -alpha[nz - 1] / bet;
+    bet = bj - aj * gam;
+    if (bet == 0.0) bad |= kPivot;
+    u = (rj - aj * u) / bet;
+    gamw[(int64_t)nz * nf + f] = gam;
+    Tout[(int64_t)nz * nf + f] = u;
+  }
   if (!isfinite(u)) bad |= kNanT;
   if (u < 0.0) bad |= kNegT;
+#pragma unroll 8
   for (int j = nz - 1; j >= 1; --j) {
     u = Tout[(int64_t)j * nf + f] - gamw[(int64_t)(j + 1) * nf + f] * u;
     Tout[(int64_t)j * nf + f] = u;
