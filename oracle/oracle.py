@@ -295,8 +295,13 @@ def conduction_t(z, dt, T, Tsurf, Tsurfp1, ti, rhoc, Fgeotherm):
     return Fsurf
 
 
+def _absorbed(E, Qrefl, QIR, rho, emiss):
+    """model.py:224: the flux absorbed by the surface."""
+    return (1 - rho) * (E + Qrefl) + emiss * QIR
+
+
 def thermal_model_run(F, E_steps, t, rho, emiss, z, T0, ti, rhoc, Fgeotherm, E0,
-                      out_dtype=np.float64, return_flux=False):
+                      out_dtype=np.float64, return_flux=False, absorbed=_absorbed):
     """model.py:108-234 (one sun per time): the ThermalModel iteration.
 
     F: SciPy CSR (N x N) in the model dtype; E_steps[i] the direct
@@ -304,7 +309,8 @@ def thermal_model_run(F, E_steps, t, rho, emiss, z, T0, ti, rhoc, Fgeotherm, E0,
     E0 the irradiance of the initial state.  ``F @ x`` is rounded to
     ``out_dtype`` as the reference's compressed operator does
     (compressed_form_factors.py:408).  Yields T (N x nz+1) after each step,
-    or (T, E, Qrefl, QIR) with return_flux."""
+    or (T, E, Qrefl, QIR) with return_flux.  ``absorbed`` is the formula of
+    model.py:224 (a test replaces it to plant a mistake)."""
     nf = F.shape[0]
     rho = np.asarray(rho, dtype=np.float64)
     emiss = _f64(emiss, nf)
@@ -323,7 +329,7 @@ def thermal_model_run(F, E_steps, t, rho, emiss, z, T0, ti, rhoc, Fgeotherm, E0,
         Qrefl = np.maximum(0, Qrefl)
         QIR = apply(emiss * SIGMA * Tsurf ** 4 + (1 - emiss) * QIR)
         QIR = np.maximum(0, QIR)
-        Q = (1 - rho) * (E + Qrefl) + emiss * QIR
+        Q = absorbed(E, Qrefl, QIR, rho, emiss)
         conduction_q(z, dt, Qprev, Q, T, ti, rhoc, emiss, Fg)
         Qprev = Q
         Tsurf = T[:, 0].copy()
