@@ -11,8 +11,9 @@ leaves).
 
 One step = one batched solve of all 64 suns to <= 1e-6 relative error (the
 refinement stops at max|r_c| <= 1e-7 max|E_c|): BF16 near-field panels on
-MFMA for the inner sweeps, FP32 master panels with FP64 MFMA for the
-residual.  The plain FP64 path solves one sun at a time with FP64 values
+MFMA for the inner sweeps (one sweep per round by default), and the residual
+of every round on int8-sliced panels with int8 MFMA, FP64-accurate (or,
+with --residual fp64, FP32 master panels on FP64 MFMA).  The plain FP64 path solves one sun at a time with FP64 values
 and vectors on the same compressed matrix, reference Jacobi stopping rule
 (solve.py:6-8), no batching.  Inputs are resident in HBM before the timed
 region; visibility, assembly and compression are timed separately.
@@ -57,7 +58,8 @@ def main():
     ap.add_argument("--fp64-suns", type=int, default=4)
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--min-size", type=int, default=4096)
-    ap.add_argument("--inner", type=int, default=2)
+    ap.add_argument("--inner", type=int, default=1,
+                    help="BF16 Jacobi sweeps per refinement round")
     ap.add_argument("--residual", choices=["int8", "fp64"], default="int8",
                     help="refinement residual: int8-sliced panels on int8 MFMA "
                          "(FP64-accurate) or FP32 panels on FP64 MFMA")
@@ -299,9 +301,12 @@ def main():
         "data": "synthetic (seeded fractal terrain, seeded suns and albedo)",
         "config": {
             "workload": "fractal terrain 145^2 vertices (41,472 facets), 64 suns, compressed "
-                        "F tree (tol 1e-6, min_size 4096%s), BF16 panels + FP64-residual "
-                        "refinement to 1e-6" % ("" if args.lowrank else
-                                                ", low-rank leaves off: exact F"),
+                        "F tree (tol 1e-6, min_size 4096%s), refinement to max|r| <= 1e-7 "
+                        "max|E| per sun: %d BF16 panel sweep(s) per round, %s residual"
+                        % ("" if args.lowrank else ", low-rank leaves off: exact F", args.inner,
+                           "FP64-accurate int8-sliced" if args.residual == "int8"
+                           else "FP64-MFMA"),
+            "inner_sweeps": args.inner,
             "facets": nf, "suns": args.suns, "near_nnz": nnz, "tol": args.tol,
             "min_size": args.min_size, "compressed_nbytes": int(H.nbytes),
             "leaves": bs["leaf_counts"], "svd_leaves": bs["svd_leaves"],

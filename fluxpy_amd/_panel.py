@@ -9,6 +9,8 @@ FP32 panels of the master values with FP64 MFMA.
 """
 import ctypes
 
+import warnings
+
 import numpy as np
 
 from . import _kernels
@@ -302,17 +304,23 @@ class PanelRefinedSolver:
 
     def solve(self, E, rtol=1e-7, inner_iters=4, max_rounds=30):
         """E: n x k FP64 device tensor in panel order.  Returns (B, info); B
-        is one of the solver's two buffers (valid until the next solve)."""
+        is one of the solver's two buffers (valid until the next solve).
+        inner_iters: sweeps per round, an int or a sequence (round r uses
+        entry r, the last entry repeats)."""
         t = _lib.torch()
         k = self.k
+        sched = [int(inner_iters)] if np.ndim(inner_iters) == 0 else [int(v) for v in inner_iters]
+        if not sched or min(sched) < 1:
+            raise ValueError("inner_iters: at least one sweep per round")
+        sweeps = lambda r: sched[min(r, len(sched) - 1)]
         Bs, Rs, cms = self.Bs, self.Rs, self.cms
         emax_d = self.vo.maxabs(E)
         # rounds 1 and 2 without waiting in between (round 1 starts from B = 0,
         # whose residual is E)
-        self._round(E, E, None, Bs[0], Rs[0], cms[0], inner_iters)
+        self._round(E, E, None, Bs[0], Rs[0], cms[0], sweeps(0))
         spec = max_rounds >= 2
         if spec:
-            self._round(E, Rs[0], Bs[0], Bs[1], Rs[1], cms[1], inner_iters)
+            self._round(E, Rs[0], Bs[0], Bs[1], Rs[1], cms[1], sweeps(1))
         got = t.cat([cms[0], cms[1] if spec else cms[0], emax_d.reshape(-1)]).cpu().numpy()
         rmaxes, emax = [got[:k], got[k:2 * k]], got[2 * k:]
         scale = np.where(emax > 0, emax, 1.0)
@@ -326,9 +334,12 @@ class PanelRefinedSolver:
             while not np.all(rmaxes[1] <= thresh) and rounds < max_rounds:
                 rounds += 1
                 nxt = 1 - cur
-                self._round(E, Rs[cur], Bs[cur], Bs[nxt], Rs[nxt], cms[nxt], inner_iters)
+                self._round(E, Rs[cur], Bs[cur], Bs[nxt], Rs[nxt], cms[nxt], sweeps(rounds - 1))
                 cur = nxt
                 rmaxes[1] = cms[cur].cpu().numpy()
                 hist.append(float((rmaxes[1] / scale).max()))
-        return Bs[cur], {"rounds": rounds, "inner_iters": rounds * inner_iters,
+        if hist[-1] > rtol:
+            warnings.warn(f"refinement stopped after {rounds} rounds (residual "
+                          f"{hist[-1]:.2e} > {rtol:.1e})", RuntimeWarning)
+        return Bs[cur], {"rounds": rounds, "inner_iters": sum(sweeps(r) for r in range(rounds)),
                          "residual_history": hist}

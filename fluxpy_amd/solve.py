@@ -348,7 +348,8 @@ def solve_radiosity_refined(FF_lo, FF_hi, E, rho=1, albedo_placement='right', rt
 
     Inner solves use ``FF_lo`` (low-precision values, e.g. BF16 with FP32
     vectors and accumulation): ``inner_iters`` Jacobi sweeps on the
-    correction equation (I - F rho) d = r, with no host synchronisation.
+    correction equation (I - F rho) d = r, with no host synchronisation
+    (an int, or one count per round with the last repeated).
     The residual r = E - (I - F rho) B is formed in FP64 with ``FF_hi`` (the
     FP32 master values, FP64 vectors and accumulation).  Stops when every
     column satisfies max|r_c| <= rtol * max|E_c|; the error of B against the
@@ -395,6 +396,9 @@ def solve_radiosity_refined(FF_lo, FF_hi, E, rho=1, albedo_placement='right', rt
     D = t.empty_like(R32)
     D1 = t.empty_like(R32)
     cm32 = t.empty(k, dtype=t.float64, device=E2.device)
+    sched = [int(inner_iters)] if np.ndim(inner_iters) == 0 else [int(v) for v in inner_iters]
+    if not sched or min(sched) < 1:
+        raise ValueError("inner_iters: at least one sweep per round")
     rounds = 0
     inner_total = 0
     hist = []
@@ -403,10 +407,11 @@ def solve_radiosity_refined(FF_lo, FF_hi, E, rho=1, albedo_placement='right', rt
         # low-precision correction: D <- R + F(rho D) repeated, D0 = R
         R32.copy_(_kernels.convert(R, t.float32))
         D.copy_(R32)
-        for _ in range(inner_iters):
+        sweeps = sched[min(rounds - 1, len(sched) - 1)]
+        for _ in range(sweeps):
             op_lo.jacobi_step(R32, D, rho32, left, D1, cm32)
             D, D1 = D1, D
-        inner_total += inner_iters
+        inner_total += sweeps
         # B += D (FP64 += FP32)
         _kernels.axpby(None, _kernels.convert(D, t.float64), None, B)
         # FP64 residual: B1 = E + F(rho B); R = B1 - B; colmax = max|R|
