@@ -77,7 +77,7 @@ SIGNATURES = {
                                    _c_int, _vp, _vp, _vp]),
     "flux_refine_prepare": (_c_int, [_c_i64, _c_i64, _vp, _vp, ctypes.c_float, _c_int, _vp,
                                      _vp, _vp]),
-    "flux_refine_accumulate": (_c_int, [_c_i64, _c_i64, _vp, _vp, _vp, _vp]),
+    "flux_refine_accumulate": (_c_int, [_c_i64, _c_i64, _vp, _vp, _vp, _vp, _vp]),
     "flux_scale_to_bf16": (_c_int, [_c_i64, _c_i64, _vp, _vp, ctypes.c_float, _c_int, _vp,
                                     _vp]),
     "flux_panel8_layout": (_c_int, [_c_i64, _vp, _vp, _vp, _vp, _vp]),
@@ -87,7 +87,7 @@ SIGNATURES = {
     "flux_slice_x": (_c_int, [_c_i64, _c_i64, _vp, _vp, _c_dbl, _c_int, _vp, _vp]),
     "flux_panel8_mm": (_c_int, [_c_i64, _vp, _vp, _vp, _vp, _vp, _c_i64, _vp, _vp, _vp, _vp,
                                 _c_dbl, _c_int, _vp, _vp, _vp, _c_i64, _vp, _vp, _vp, _c_i64,
-                                _vp, _vp]),
+                                _vp, _c_int, _vp]),
     "flux_vec_reduce":(_c_int, [_c_i64, _c_i64, _vp, _vp, _c_int, _c_int, _vp, _vp, _vp]),
     "flux_vec_axpby": (_c_int, [_c_i64, _c_i64, _vp, _vp, _vp, _vp, _c_int, _vp]),
     "flux_vec_rowscale": (_c_int, [_c_i64, _c_i64, _vp, _c_dbl, _c_int, _c_dbl, _vp, _vp,

@@ -171,49 +171,62 @@ class SlicedPanelMatrix:
                   self.quads, self.rowexp.data_ptr(), self.panel.data_ptr(), st)
         self.nnz = csr.nnz
         self.plan = like.plan
-        self._work = None
+        self._work = {}
 
     @property
     def nbytes(self):
         return (self.panel.numel() + self.rowexp.numel() * 4 + self.qptr.numel() * 8
                 + self.cols.numel() * 4 + self.colptr.numel() * 8)
 
-    def workspace(self, k):
-        """X-slice scratch for k right-hand sides (grown on demand)."""
+    def workspace(self, k, slot=0):
+        """X-slice scratch for k right-hand sides (grown on demand); `slot`
+        selects one of several independent workspaces."""
         t = _lib.torch()
         nb = int(_lib.load().flux_slice_x_bytes(self.shape[1], k))
-        if self._work is None or self._work.numel() < nb:
-            self._work = t.empty(nb, dtype=t.uint8, device=self.panel.device)
-        return self._work
+        w = self._work.get(slot)
+        if w is None or w.numel() < nb:
+            w = self._work[slot] = t.empty(nb, dtype=t.uint8, device=self.panel.device)
+        return w
+
+    def xmax(self, k, slot=0):
+        """max_i |s_i X_ic| per column of the X last sliced into `slot` (a
+        device view into the workspace)."""
+        t = _lib.torch()
+        np_ = (k + 63) // 64 * 64
+        w = self.workspace(k, slot)
+        return w[np_ * 4:np_ * 12].view(t.float64)[:k]
 
     def mm_f64(self, X, out, E=None, B=None, rho=None, rho_s=1.0, left=False, colmax=None,
-               xscale=None):
+               xscale=None, nslices=5, slot=0):
         """out = P (s X), or with E: out = E + r (P s X) - B and colmax = max|out|
-        (r_i = rho_i if left; s_i = xscale_i, a tensor or scalar, or 1)."""
+        (r_i = rho_i if left; s_i = xscale_i, a tensor or scalar, or 1).
+        nslices < 5 keeps the leading slices only (see flux_panel8_mm)."""
         k = X.shape[1] if X.dim() == 2 else 1
-        self.slice_x(X, xscale)
-        return self.product(k, out, E=E, B=B, rho=rho, rho_s=rho_s, left=left, colmax=colmax)
+        self.slice_x(X, xscale, slot=slot)
+        return self.product(k, out, E=E, B=B, rho=rho, rho_s=rho_s, left=left, colmax=colmax,
+                            nslices=nslices, slot=slot)
 
-    def slice_x(self, X, xscale=None):
+    def slice_x(self, X, xscale=None, slot=0):
         """Slices of s X into the workspace (the first half of mm_f64)."""
         t = _lib.torch()
         k = X.shape[1] if X.dim() == 2 else 1
-        work = self.workspace(k)
+        work = self.workspace(k, slot)
         xs_t = xscale if isinstance(xscale, t.Tensor) else None
         xs_s = 1.0 if xscale is None or xs_t is not None else float(xscale)
         _lib.call("flux_slice_x", self.shape[1], k, X.data_ptr(), _lib.ptr(xs_t), xs_s,
                   int(xscale is not None), work.data_ptr(), _lib.stream())
 
-    def product(self, k, out, E=None, B=None, rho=None, rho_s=1.0, left=False, colmax=None):
+    def product(self, k, out, E=None, B=None, rho=None, rho_s=1.0, left=False, colmax=None,
+                nslices=5, slot=0):
         """The product from the slices last written by slice_x (second half
         of mm_f64)."""
-        work = self.workspace(k)
+        work = self.workspace(k, slot)
         st = _lib.stream()
         plan = self.plan.args("int8", k, 8) if self.plan else _NO_PLAN
         _lib.call("flux_panel8_mm", self.shape[0], self.colptr.data_ptr(), self.cols.data_ptr(),
                   self.qptr.data_ptr(), self.panel.data_ptr(), self.rowexp.data_ptr(), k,
                   work.data_ptr(), _lib.ptr(E), _lib.ptr(B), _lib.ptr(rho), float(rho_s),
-                  int(left), out.data_ptr(), _lib.ptr(colmax), *plan, st)
+                  int(left), out.data_ptr(), _lib.ptr(colmax), *plan, int(nslices), st)
         return out
 
 
@@ -243,6 +256,14 @@ class PanelRefinedSolver:
     with max|E| in one transfer: the host waits once per solve when two rounds
     suffice, as they do for the radiosity systems here.  A later round waits
     for its own check.  All vectors live in the panels' (cluster) order.
+
+    With int8 panels, rounds after the first form the residual as an update,
+    r_new = r_prev + F rho D - D for the round's correction D, on four
+    slices instead of five: D is small, so the truncation error (about
+    2^-28 of max|F_r.| max|rho D_.c| per term) stays under round 1's bound
+    (2^-35 of max|F_r.| max|rho B_.c|) whenever max|rho D_c| <= 2^-7
+    max|rho B_c|.  That condition is checked with the column maxima read back
+    with the residuals; a round that fails it recomputes its residual in full.
     """
 
     def __init__(self, P_lo, P_hi, n, k, rho=1.0, left=False):
@@ -259,6 +280,7 @@ class PanelRefinedSolver:
         self.R32 = t.empty((n, k), dtype=f32, device=dev)
         self.D = [t.empty((n, k), dtype=f32, device=dev) for _ in range(2)]
         self.Xb = [t.empty((n, k), dtype=t.bfloat16, device=dev) for _ in range(2)]
+        self.D64 = t.empty((n, k), dtype=f64, device=dev)
         self.vo = _kernels.VecOps(k)
 
     def set_rho(self, rho):
@@ -275,9 +297,17 @@ class PanelRefinedSolver:
             self.rho64 = self.rho32 = None
             self.rs = float(rho)
 
-    def _round(self, E, src, b_in, b_out, r_out, colmax, inner_iters):
-        """One refinement round: inner sweeps on src, b_out = b_in + D, and
-        the residual of b_out into r_out with its column maxima."""
+    # slices of the X of an update residual: the update X is at most 2^-7 of
+    # the first round's in every column (checked after the fact), so four
+    # slices keep the error bound of the five-slice product of round 1
+    UPDATE_SLICES = 4
+
+    def _round(self, E, src, b_in, b_out, r_out, colmax, sweeps, slot, r_prev=None):
+        """One refinement round: `sweeps` inner sweeps on src, b_out = b_in
+        + D, and the residual of b_out into r_out with its column maxima.
+        With r_prev (the residual of b_in) and int8 panels the residual is
+        the update r_prev + F rho D - D on UPDATE_SLICES slices; otherwise
+        E + F rho b_out - b_out in full."""
         st = _lib.stream()
         n, k = self.n, self.k
         # D0 = fp32(src); X0 = bf16(rho D0) (right) or bf16(D0) (left)
@@ -285,22 +315,40 @@ class PanelRefinedSolver:
                   float(self.rs), int(not self.left), self.R32.data_ptr(),
                   self.Xb[0].data_ptr(), st)
         cur = 0
-        for _ in range(inner_iters):
+        for _ in range(sweeps):
             self.P_lo.mm_bf16(self.Xb[cur], self.D[cur], base=self.R32, rho=self.rho32,
                               rho_s=self.rs, left=self.left, xnext=self.Xb[1 - cur])
             cur = 1 - cur
         D = self.D[1 - cur]
+        update = r_prev is not None and isinstance(self.P_hi, SlicedPanelMatrix)
         _lib.call("flux_refine_accumulate", n, k, D.data_ptr(), _lib.ptr(b_in),
-                  b_out.data_ptr(), st)
+                  b_out.data_ptr(), _lib.ptr(self.D64 if update else None), st)
         # residual R = E + F(rho B) - B  (right) / E + rho F B - B (left)
         rs = self.rho64 if self.rho64 is not None else self.rs
-        if isinstance(self.P_hi, SlicedPanelMatrix):
-            self.P_hi.mm_f64(b_out, r_out, E=E, B=b_out, rho=self.rho64, rho_s=self.rs,
-                             left=self.left, colmax=colmax, xscale=None if self.left else rs)
+        xs = None if self.left else rs
+        if update:
+            self.P_hi.mm_f64(self.D64, r_out, E=r_prev, B=self.D64, rho=self.rho64,
+                             rho_s=self.rs, left=self.left, colmax=colmax, xscale=xs,
+                             nslices=self.UPDATE_SLICES, slot=slot)
         else:
-            Xr = b_out if self.left else _kernels.rowscale(rs, b_out, out=self.Xr)
-            self.P_hi.mm_f64(Xr, r_out, E=E, B=b_out, rho=self.rho64, rho_s=self.rs,
+            self._full_residual(E, b_out, r_out, colmax, slot)
+        return update
+
+    def _full_residual(self, E, B, r_out, colmax, slot):
+        rs = self.rho64 if self.rho64 is not None else self.rs
+        if isinstance(self.P_hi, SlicedPanelMatrix):
+            self.P_hi.mm_f64(B, r_out, E=E, B=B, rho=self.rho64, rho_s=self.rs,
+                             left=self.left, colmax=colmax, xscale=None if self.left else rs,
+                             slot=slot)
+        else:
+            Xr = B if self.left else _kernels.rowscale(rs, B, out=self.Xr)
+            self.P_hi.mm_f64(Xr, r_out, E=E, B=B, rho=self.rho64, rho_s=self.rs,
                              left=self.left, colmax=colmax)
+
+    def _update_ok(self, xmax_b, xmax_d):
+        """True if the update X is at most 2^-7 of the first round's X in
+        every column (then UPDATE_SLICES slices keep round 1's bound)."""
+        return bool(np.all(xmax_d <= np.ldexp(xmax_b, -7)))
 
     def solve(self, E, rtol=1e-7, inner_iters=4, max_rounds=30):
         """E: n x k FP64 device tensor in panel order.  Returns (B, info); B
@@ -313,16 +361,29 @@ class PanelRefinedSolver:
         if not sched or min(sched) < 1:
             raise ValueError("inner_iters: at least one sweep per round")
         sweeps = lambda r: sched[min(r, len(sched) - 1)]
+        sliced = isinstance(self.P_hi, SlicedPanelMatrix)
         Bs, Rs, cms = self.Bs, self.Rs, self.cms
         emax_d = self.vo.maxabs(E)
         # rounds 1 and 2 without waiting in between (round 1 starts from B = 0,
-        # whose residual is E)
-        self._round(E, E, None, Bs[0], Rs[0], cms[0], sweeps(0))
+        # whose residual is E; round 2 updates round 1's residual)
+        self._round(E, E, None, Bs[0], Rs[0], cms[0], sweeps(0), 0)
         spec = max_rounds >= 2
+        upd = False
         if spec:
-            self._round(E, Rs[0], Bs[0], Bs[1], Rs[1], cms[1], sweeps(1))
-        got = t.cat([cms[0], cms[1] if spec else cms[0], emax_d.reshape(-1)]).cpu().numpy()
-        rmaxes, emax = [got[:k], got[k:2 * k]], got[2 * k:]
+            upd = self._round(E, Rs[0], Bs[0], Bs[1], Rs[1], cms[1], sweeps(1), 1, r_prev=Rs[0])
+        parts = [cms[0], cms[1] if spec else cms[0], emax_d.reshape(-1)]
+        if sliced:
+            parts += [self.P_hi.xmax(k, 0), self.P_hi.xmax(k, 1)]
+        got = t.cat(parts).cpu().numpy()
+        rmaxes, emax = [got[:k], got[k:2 * k]], got[2 * k:3 * k]
+        xmax_b = got[3 * k:4 * k] if sliced else None
+        fallbacks = 0
+        if upd and not self._update_ok(xmax_b, got[4 * k:5 * k]):
+            # the correction was too large for the short update: redo round
+            # 2's residual in full
+            self._full_residual(E, Bs[1], Rs[1], cms[1], 1)
+            rmaxes[1] = cms[1].cpu().numpy()
+            fallbacks += 1
         scale = np.where(emax > 0, emax, 1.0)
         thresh = rtol * scale
         hist = [float((rmaxes[0] / scale).max())]
@@ -334,12 +395,21 @@ class PanelRefinedSolver:
             while not np.all(rmaxes[1] <= thresh) and rounds < max_rounds:
                 rounds += 1
                 nxt = 1 - cur
-                self._round(E, Rs[cur], Bs[cur], Bs[nxt], Rs[nxt], cms[nxt], sweeps(rounds - 1))
+                upd = self._round(E, Rs[cur], Bs[cur], Bs[nxt], Rs[nxt], cms[nxt],
+                                  sweeps(rounds - 1), nxt, r_prev=Rs[cur])
                 cur = nxt
-                rmaxes[1] = cms[cur].cpu().numpy()
+                if upd:
+                    both = t.cat([cms[cur], self.P_hi.xmax(k, cur)]).cpu().numpy()
+                    rmaxes[1] = both[:k]
+                    if not self._update_ok(xmax_b, both[k:]):
+                        self._full_residual(E, Bs[cur], Rs[cur], cms[cur], cur)
+                        rmaxes[1] = cms[cur].cpu().numpy()
+                        fallbacks += 1
+                else:
+                    rmaxes[1] = cms[cur].cpu().numpy()
                 hist.append(float((rmaxes[1] / scale).max()))
         if hist[-1] > rtol:
             warnings.warn(f"refinement stopped after {rounds} rounds (residual "
                           f"{hist[-1]:.2e} > {rtol:.1e})", RuntimeWarning)
         return Bs[cur], {"rounds": rounds, "inner_iters": sum(sweeps(r) for r in range(rounds)),
-                         "residual_history": hist}
+                         "residual_history": hist, "full_residual_fallbacks": fallbacks}

@@ -636,11 +636,14 @@ __global__ void k_refine_prepare(int64_t n, int64_t nrhs, const double *__restri
 }
 
 // End of the inner sweeps: b_out = b_in + d (b_in NULL: b_out = d), FP32 d
-// into FP64 (b_in may equal b_out).
+// into FP64 (b_in may equal b_out); d64 (optional) = d in FP64.
 __global__ void k_refine_accumulate(int64_t total, const float *__restrict__ d,
-                                    const double *b_in, double *b_out) {
+                                    const double *b_in, double *b_out, double *d64) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e < total) b_out[e] = (b_in ? b_in[e] : 0.0) + (double)d[e];
+  if (e >= total) return;
+  const double de = (double)d[e];
+  b_out[e] = (b_in ? b_in[e] : 0.0) + de;
+  if (d64) d64[e] = de;
 }
 
 }  // namespace flux
@@ -865,12 +868,13 @@ extern "C" int flux_refine_prepare(int64_t n, int64_t nrhs, const double *r, con
 }
 
 extern "C" int flux_refine_accumulate(int64_t n, int64_t nrhs, const float *d,
-                                      const double *b_in, double *b_out, void *stream) {
+                                      const double *b_in, double *b_out, double *d64,
+                                      void *stream) {
   FLUX_CHECK(n >= 0 && nrhs >= 0, "flux_refine_accumulate: bad sizes");
   if (n * nrhs == 0) return 0;
   FLUX_CHECK(d && b_out, "flux_refine_accumulate: NULL array");
   hipLaunchKernelGGL(k_refine_accumulate, dim3(ceil_div_u(n * nrhs, 256)), dim3(256), 0,
-                     as_stream(stream), n * nrhs, d, b_in, b_out);
+                     as_stream(stream), n * nrhs, d, b_in, b_out, d64);
   FLUX_LAUNCH_CHECK();
   return 0;
 }

@@ -351,8 +351,13 @@ __global__ void __launch_bounds__(256) k_colmax_blocks(const double *__restrict_
   if (threadIdx.x == 0 && k < nrhs) colmax[k] = red[0];
 }
 
-constexpr int kUnits = 5;  // X staging units per thread and stage
-
+// S = number of leading slices used on both sides (S <= kSlices): the
+// products of slices i, j with i + j < S.  S = kSlices is the full product;
+// a smaller S reads S of the five F tiles and S of the five X slices of each
+// column (the truncation error grows to about 2^(e_r + e_k - 7 S) per term),
+// which the refinement uses for a residual update whose X is small.  Each
+// thread stages S 16-byte units of X per stage.
+template <int S>
 __global__ void __launch_bounds__(256, 2) k_panel8_mm(
     int64_t nrows, int64_t nblk, int nh, const int64_t *__restrict__ colptr,
     const int32_t *__restrict__ cols, const int64_t *__restrict__ qptr,
@@ -362,7 +367,7 @@ __global__ void __launch_bounds__(256, 2) k_panel8_mm(
     double rho_s, int left, double *__restrict__ out, double *__restrict__ bcm,
     const int32_t *__restrict__ items, const int32_t *__restrict__ nsplit,
     const int32_t *__restrict__ slot0, double *__restrict__ ws, int64_t nitems) {
-  __shared__ __align__(16) uint8_t xs[2][kSlices * kXsSlice];
+  __shared__ __align__(16) uint8_t xs[2][S * kXsSlice];
   __shared__ __align__(16) int32_t ids[4][kQuadCols];
   const int64_t pair = xcd_block(blockIdx.x, nitems * nh);
   const int64_t it = pair / nh;
@@ -379,36 +384,36 @@ __global__ void __launch_bounds__(256, 2) k_panel8_mm(
                                                 (16 * w + li) * kQuadCols + 16 * lg);
   const int8_t *xg = x8 + h * (int64_t)kXRow;
   const int64_t xstride = (int64_t)nh * kXRow;
-  v4i acc[kSlices][4];
-  for (int d = 0; d < kSlices; ++d)
+  v4i acc[S][4];
+  for (int d = 0; d < S; ++d)
     for (int t = 0; t < 4; ++t) acc[d][t] = v4i{0, 0, 0, 0};
   if (nq > 0) {
     // every load address is clamped into the block, so nothing is masked
     auto load_id = [&](int64_t s) -> int32_t {
       return cols[c0 + min(s * kQuadCols + lane, ncl - 1)];
     };
-    auto load_x = [&](int64_t s, v4i (&xr)[kUnits]) {
+    auto load_x = [&](int64_t s, v4i (&xr)[S]) {
       const int32_t *id = ids[s & 3];
-      for (int m = 0; m < kUnits; ++m) {
+      for (int m = 0; m < S; ++m) {
         const int u = tid + 256 * m;
-        const int c = u / 20, ch = u - c * 20;
+        const int c = u / (4 * S), ch = u - c * (4 * S);
         xr[m] = *reinterpret_cast<const v4i *>(xg + (int64_t)id[c] * xstride + 16 * ch);
       }
     };
-    auto store_x = [&](int buf, const v4i (&xr)[kUnits]) {
-      for (int m = 0; m < kUnits; ++m) {
+    auto store_x = [&](int buf, const v4i (&xr)[S]) {
+      for (int m = 0; m < S; ++m) {
         const int u = tid + 256 * m;
-        const int c = u / 20, ch = u - c * 20;
+        const int c = u / (4 * S), ch = u - c * (4 * S);
         *reinterpret_cast<v4i *>(&xs[buf][(ch >> 2) * kXsSlice + c * kXsRow + 16 * (ch & 3)]) =
             xr[m];
       }
     };
-    auto load_p = [&](int64_t s, v4i (&pf)[kSlices]) {
+    auto load_p = [&](int64_t s, v4i (&pf)[S]) {
       const v4i *p = pb + min(s, nq - 1) * (kQuadBytes / 16);
-      for (int i = 0; i < kSlices; ++i) pf[i] = p[i * (kSliceTile / 16)];
+      for (int i = 0; i < S; ++i) pf[i] = p[i * (kSliceTile / 16)];
     };
-    v4i xr[kUnits];
-    v4i p0[kSlices], p1[kSlices];
+    v4i xr[S];
+    v4i p0[S], p1[S];
     // prologue: ids(0..2) to LDS (every wave stores the same ids, so no
     // branch sits next to a load), X(0) to LDS, panel(0) in flight
     ids[0][lane] = load_id(0);
@@ -419,7 +424,7 @@ __global__ void __launch_bounds__(256, 2) k_panel8_mm(
     load_p(0, p0);
     store_x(0, xr);
     __syncthreads();
-    auto step = [&](int64_t s, const v4i (&pc)[kSlices], v4i (&pl)[kSlices]) {
+    auto step = [&](int64_t s, const v4i (&pc)[S], v4i (&pl)[S]) {
       const int buf = (int)(s & 1);
       load_x(s + 1, xr);
       const int32_t idn = load_id(s + 3);
@@ -430,11 +435,11 @@ __global__ void __launch_bounds__(256, 2) k_panel8_mm(
       // right-hand side 16 t + li, two transposing reads
       const uint8_t *xq = xs[buf] + (16 * lg + (li >> 1)) * kXsRow + 8 * (li & 1);
       for (int t = 0; t < 4; ++t) {
-        for (int j = 0; j < kSlices; ++j) {
+        for (int j = 0; j < S; ++j) {
           const uint8_t *pa = xq + j * kXsSlice + 16 * t;
           const v2i a0 = lds_read_tr8(pa), a1 = lds_read_tr8(pa + 8 * kXsRow);
           const v4i a = v4i{a0[0], a0[1], a1[0], a1[1]};
-          for (int i = 0; i + j < kSlices; ++i)
+          for (int i = 0; i + j < S; ++i)
             acc[i + j][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, pc[i], acc[i + j][t], 0, 0, 0);
         }
       }
@@ -449,11 +454,11 @@ __global__ void __launch_bounds__(256, 2) k_panel8_mm(
     }
     if (s < nq) step(s, p0, p1);
   }
-  // lane l item q of tile t = S^T[rhs 16 t + 4 lg + q][row li]; the five
+  // lane l item q of tile t = S^T[rhs 16 t + 4 lg + q][row li]; the S
   // sums combine exactly term by term in FP64
   auto combine = [&](int t, int q) -> double {
     double v = 0.0;
-    for (int d = kSlices - 1; d >= 0; --d) v += ldexp((double)acc[d][t][q], -7 * d);
+    for (int d = S - 1; d >= 0; --d) v += ldexp((double)acc[d][t][q], -7 * d);
     return v;
   };
   if (itm.nsplit > 1) {
@@ -611,8 +616,9 @@ extern "C" int flux_panel8_mm(int64_t nrows, const int64_t *colptr, const int32_
                               double *colmax, const int32_t *items, int64_t n_items,
                               const int32_t *nsplit, const int32_t *slot0,
                               const int32_t *split_blocks, int64_t n_split, double *ws,
-                              void *stream) {
+                              int nslices, void *stream) {
   FLUX_CHECK(nrows >= 0 && nrhs >= 0, "flux_panel8_mm: bad sizes");
+  FLUX_CHECK(nslices >= 3 && nslices <= kSlices, "flux_panel8_mm: nslices must be 3, 4 or 5");
   const int64_t nblk = (nrows + kPanelRows - 1) / kPanelRows;
   if (nblk == 0 || nrhs == 0) return 0;
   FLUX_CHECK(colptr && cols && qptr && panel8 && rowexp && work && out,
@@ -630,7 +636,8 @@ extern "C" int flux_panel8_mm(int64_t nrows, const int64_t *colptr, const int32_
   double *bcm = (colmax && E) ? (double *)((char *)work + np * 12 + nrows * nh * kXRow)
                               : nullptr;
   FLUX_CHECK(nit * nh < (1ll << 31), "flux_panel8_mm: too many workgroups");
-  hipLaunchKernelGGL(k_panel8_mm, dim3((unsigned)(nit * nh)), dim3(256), 0, st, nrows, nblk,
+  auto kernel = nslices == 5 ? k_panel8_mm<5> : nslices == 4 ? k_panel8_mm<4> : k_panel8_mm<3>;
+  hipLaunchKernelGGL(kernel, dim3((unsigned)(nit * nh)), dim3(256), 0, st, nrows, nblk,
                      (int)nh, colptr, cols, qptr, panel8, rowexp, x8, xexp, nrhs, E, B, rho,
                      rho_s, left, out, bcm, items, nsplit, slot0, ws, nit);
   if (items && n_split > 0)

@@ -350,11 +350,11 @@ int flux_panel_mm_f64(int64_t nrows, const int64_t *colptr, const int32_t *cols,
  *               scale, else 1): the FP32 correction right-hand side and the
  *               first BF16 iterate from the FP64 residual r;
  *   accumulate: b_out = b_in + d (b_in NULL: b_out = d), FP32 d into FP64
- *               (b_in may equal b_out). */
+ *               (b_in may equal b_out); d64, if not NULL, receives d in FP64. */
 int flux_refine_prepare(int64_t n, int64_t nrhs, const double *r, const float *rho,
                         float rho_s, int scale, float *r32, void *xb, void *stream);
 int flux_refine_accumulate(int64_t n, int64_t nrhs, const float *d, const double *b_in,
-                           double *b_out, void *stream);
+                           double *b_out, double *d64, void *stream);
 
 /* xb = bf16(s o x), s_i = rho_i (or rho_s) if scale, else 1. */
 int flux_scale_to_bf16(int64_t n, int64_t nrhs, const float *x,
@@ -400,7 +400,11 @@ int flux_slice_x(int64_t n, int64_t nrhs, const double *X, const double *rho,
  * max_i |out| (the refinement residual, column maxima without atomics);
  * without E: out = acc.  Plan arguments as for
  * flux_panel_mm_bf16 (items == NULL: one workgroup per row block); with a
- * plan, ws holds n_slots * ceil(nrhs / 64) * 4096 doubles. */
+ * plan, ws holds n_slots * ceil(nrhs / 64) * 4096 doubles.
+ * nslices (3..5): leading slices used on both sides, products of slices
+ * i + j < nslices; 5 is the full product, fewer slices bound the error by
+ * about 2^(e_r + e_k - 7 nslices) per term (for an X much smaller than the
+ * quantity the result is compared with). */
 int flux_panel8_mm(int64_t nrows, const int64_t *colptr, const int32_t *cols,
                    const int64_t *qptr, const int8_t *panel8,
                    const int32_t *rowexp, int64_t nrhs, const void *work,
@@ -408,7 +412,7 @@ int flux_panel8_mm(int64_t nrows, const int64_t *colptr, const int32_t *cols,
                    double rho_s, int left, double *out, double *colmax,
                    const int32_t *items, int64_t n_items, const int32_t *nsplit,
                    const int32_t *slot0, const int32_t *split_blocks,
-                   int64_t n_split, double *ws, void *stream);
+                   int64_t n_split, double *ws, int nslices, void *stream);
 
 /* ---------------------------------------------------------------------- */
 /* Column-blocked vector kernels (solver building blocks; blocks are      */

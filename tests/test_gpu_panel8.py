@@ -167,6 +167,66 @@ def test_refined_solve_int8_residual(setup, oracle_mod, left):
     assert (abs(B2.cpu().numpy() - Bd2).max(0) / abs(Bd2).max(0)).max() <= 1e-6
 
 
+@pytest.mark.parametrize("nslices", [3, 4])
+def test_panel8_fewer_slices_within_bound(setup, nslices):
+    """The leading-slice products: error <= (S + 1.2) 2^(e_r + e_k - 7 S)
+    nnz_r (the five-slice bound with 35 replaced by 7 S)."""
+    torch, sm, csr, Pb, P8, ct = setup
+    n = csr.shape[0]
+    sp = csr.to_scipy().astype(np.float64)
+    rng = np.random.default_rng(7 + nslices)
+    X = rng.standard_normal((n, 64))
+    Xd = torch.tensor(X, device=_dev())
+    Y = torch.empty_like(Xd)
+    P8.mm_f64(Xd, Y, nslices=nslices)
+    absp = abs(sp)
+    er = _exp(np.asarray(absp.max(axis=1).todense()).ravel())
+    ek = _exp(abs(X).max(axis=0))
+    nnz = np.diff(sp.indptr)
+    bound = ((nslices + 1.2) * np.ldexp(1.0, er[:, None] + ek[None, :] - 7 * nslices)
+             * nnz[:, None] + 4e-16 * (absp @ abs(X)))
+    err = abs(Y.cpu().numpy() - sp @ X)
+    assert (err <= bound).all(), (err / bound).max()
+    # and it is not the five-slice product
+    Y5 = torch.empty_like(Xd)
+    P8.mm_f64(Xd, Y5)
+    assert not torch.equal(Y, Y5)
+
+
+@pytest.mark.parametrize("force_full", [False, True])
+def test_refined_solve_update_residual(setup, oracle_mod, monkeypatch, force_full):
+    """Rounds after the first form the residual as an update on four slices
+    (no full recomputation needed here); with the magnitude check forced to
+    fail, every such round recomputes its residual in full.  Both meet 1e-6
+    against the dense FP64 solution."""
+    from fluxpy_amd._panel import PanelRefinedSolver
+    torch, sm, csr, Pb, P8, ct = setup
+    n = csr.shape[0]
+    sp = csr.to_scipy().astype(np.float64)
+    D = meshes.sun_directions(64, seed=4)
+    Ep = sm.get_direct_irradiance_multi(1365.0, D).astype(np.float64)[ct.perm]
+    rho = np.random.default_rng(1).uniform(0.1, 0.14, n)
+    if force_full:
+        monkeypatch.setattr(PanelRefinedSolver, "_update_ok", lambda self, b, d: False)
+    S = PanelRefinedSolver(Pb, P8, n, 64, torch.tensor(rho, device=_dev()))
+    B, info = S.solve(torch.tensor(Ep, device=_dev()), rtol=1e-7, inner_iters=1,
+                      max_rounds=4)
+    Bd = oracle_mod.solve_radiosity_dense(sp, Ep, rho, "right")
+    err = abs(B.cpu().numpy() - Bd).max(0) / abs(Bd).max(0)
+    assert err.max() <= 1e-6, err.max()
+    assert info["residual_history"][-1] <= 1e-7
+    if force_full:
+        assert info["full_residual_fallbacks"] == info["rounds"] - 1 >= 1
+    else:
+        assert info["full_residual_fallbacks"] == 0
+    # the update residual of the last round equals a full recomputation
+    # within the four-slice bound: compare against the FP64 residual
+    R = S.Rs[1 if info["rounds"] % 2 == 0 else 0].cpu().numpy()
+    Bn = B.cpu().numpy()
+    R_exact = Ep + sp @ (rho[:, None] * Bn) - Bn
+    assert abs(R - R_exact).max() <= 1e-9 * abs(Ep).max()
+
+
 @pytest.mark.parametrize("residual", ["int8", "fp64"])
 def test_public_refined_solve_on_panels(oracle_mod, residual):
     """solve_radiosity_refined(H, H, ...) on a compressed matrix with no
