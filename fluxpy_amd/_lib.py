@@ -164,11 +164,19 @@ def torch():
     return _torch
 
 
+_gpu_ok = False
+
+
 def require_gpu():
+    """torch, after checking (once per process) that a HIP device is visible
+    and the library loads."""
+    global _gpu_ok
     t = torch()
-    if not t.cuda.is_available():
-        raise FluxError("no HIP device is visible: fluxpy_amd has no CPU fallback")
-    load()
+    if not _gpu_ok:
+        if not t.cuda.is_available():
+            raise FluxError("no HIP device is visible: fluxpy_amd has no CPU fallback")
+        load()
+        _gpu_ok = True
     return t
 
 
@@ -178,7 +186,8 @@ def device():
 
 def stream():
     """Current torch (HIP) stream as a void* value."""
-    return require_gpu().cuda.current_stream().cuda_stream
+    t = require_gpu()
+    return t._C._cuda_getCurrentRawStream(t.cuda.current_device())
 
 
 def ptr(t):

@@ -350,11 +350,15 @@ class PanelRefinedSolver:
         every column (then UPDATE_SLICES slices keep round 1's bound)."""
         return bool(np.all(xmax_d <= np.ldexp(xmax_b, -7)))
 
-    def solve(self, E, rtol=1e-7, inner_iters=4, max_rounds=30):
+    def solve(self, E, rtol=1e-7, inner_iters=4, max_rounds=30, finish=None):
         """E: n x k FP64 device tensor in panel order.  Returns (B, info); B
         is one of the solver's two buffers (valid until the next solve).
         inner_iters: sweeps per round, an int or a sequence (round r uses
-        entry r, the last entry repeats)."""
+        entry r, the last entry repeats).  finish(B), if given, is applied
+        to the result and returned as info["finished"]; it is issued before
+        the host waits for the residual check, on the two-round result, so
+        its launches overlap the check (and it is issued again if the solve
+        ends on another round)."""
         t = _lib.torch()
         k = self.k
         sched = [int(inner_iters)] if np.ndim(inner_iters) == 0 else [int(v) for v in inner_iters]
@@ -371,6 +375,7 @@ class PanelRefinedSolver:
         upd = False
         if spec:
             upd = self._round(E, Rs[0], Bs[0], Bs[1], Rs[1], cms[1], sweeps(1), 1, r_prev=Rs[0])
+        pre = finish(Bs[1]) if (finish is not None and spec) else None
         parts = [cms[0], cms[1] if spec else cms[0], emax_d.reshape(-1)]
         if sliced:
             parts += [self.P_hi.xmax(k, 0), self.P_hi.xmax(k, 1)]
@@ -411,5 +416,8 @@ class PanelRefinedSolver:
         if hist[-1] > rtol:
             warnings.warn(f"refinement stopped after {rounds} rounds (residual "
                           f"{hist[-1]:.2e} > {rtol:.1e})", RuntimeWarning)
-        return Bs[cur], {"rounds": rounds, "inner_iters": sum(sweeps(r) for r in range(rounds)),
-                         "residual_history": hist, "full_residual_fallbacks": fallbacks}
+        info = {"rounds": rounds, "inner_iters": sum(sweeps(r) for r in range(rounds)),
+                "residual_history": hist, "full_residual_fallbacks": fallbacks}
+        if finish is not None:
+            info["finished"] = pre if (pre is not None and rounds == 2) else finish(Bs[cur])
+        return Bs[cur], info

@@ -338,8 +338,9 @@ def _solve_panels(FF, E2, rho, left, rtol, inner_iters, max_refine, residual):
         S = solvers[key] = PanelRefinedSolver(P_lo, P_hi, n, k, rho64, left=left)
     else:
         S.set_rho(rho64)
-    B, info = S.solve(Ei, rtol=rtol, inner_iters=inner_iters, max_rounds=max_refine)
-    return _from_internal(op, B), info
+    _, info = S.solve(Ei, rtol=rtol, inner_iters=inner_iters, max_rounds=max_refine,
+                      finish=lambda B: _from_internal(op, B))
+    return info.pop("finished"), info
 
 
 def solve_radiosity_refined(FF_lo, FF_hi, E, rho=1, albedo_placement='right', rtol=1e-7,
