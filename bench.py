@@ -39,7 +39,8 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE x 2, WRITE_SIZE x 1); the numbers and the commands are in
 # profiles/panel_kernels_fractal41k_64suns.md.  Kernels not measured: None.
 MEASURED_TRAFFIC = {
-    "k_panel8_mm + k_panel8_reduce + k_colmax_blocks": 2114790246,
+    "k_panel8_mm<5> + k_panel8_reduce + k_colmax_blocks": 2114790246,
+    "k_panel8_mm<4> + k_panel8_reduce + k_colmax_blocks": 1774695117,
     "k_panel_mm_bf16 + k_panel_reduce_bf16": 737875218,
 }
 
@@ -216,6 +217,8 @@ def main():
     #                  + k n (2 X in + 4 out + 4 base in + 2 next X out)
     #   int8 residual: nnz (5 slice bytes + 4 index) + 4 (n + 1)
     #                  + k n (5 X slices + 8 E + 8 B + 8 out)
+    #   int8 update residual (rounds after the first, 4 slices):
+    #                  nnz (4 + 4) + 4 (n + 1) + k n (4 + 8 r_prev + 8 D + 8 out)
     k = args.suns
     reps = 20
     kernels = {}
@@ -248,9 +251,15 @@ def main():
         cmx = torch.empty(k, dtype=torch.float64, device=dev)
         P8.slice_x(X64, xscale=rho_d)
         us = time_launch(lambda: P8.product(k, R64, E=X64, B=X64, colmax=cmx))
-        kernels["k_panel8_mm + k_panel8_reduce + k_colmax_blocks"] = {
-            "us": us, "per_step": info["rounds"] if args.residual == "int8" else 0,
+        kernels["k_panel8_mm<5> + k_panel8_reduce + k_colmax_blocks"] = {
+            "us": us, "per_step": 1 if args.residual == "int8" else 0,
             "bytes": nnz * (5 + 4) + 4 * (nf + 1) + k * nf * (5 + 8 + 8 + 8)}
+        us = time_launch(lambda: P8.product(k, R64, E=X64, B=X64, colmax=cmx, nslices=4))
+        kernels["k_panel8_mm<4> + k_panel8_reduce + k_colmax_blocks"] = {
+            "us": us,
+            "per_step": (info["rounds"] - 1 - info.get("full_residual_fallbacks", 0)
+                         if args.residual == "int8" else 0),
+            "bytes": nnz * (4 + 4) + 4 * (nf + 1) + k * nf * (4 + 8 + 8 + 8)}
         us_slice = time_launch(lambda: P8.slice_x(X64, xscale=rho_d))
         log(f"[bench] X slicing (k_colmax_scaled + k_colmax_final + k_slice_x): {us_slice:.1f} us")
         del X64, R64
