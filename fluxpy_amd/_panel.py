@@ -166,12 +166,16 @@ class SlicedPanelMatrix:
         self.quads = total.value
         self.panel = t.empty(max(self.quads * self.QUAD_BYTES, 16), dtype=t.int8, device=dev)
         self.rowexp = t.empty(max(m, 1), dtype=t.int32, device=dev)
+        # zero-subtile flags, one byte per (quad, 16-row group)
+        self.qmask = t.empty(max(self.quads, 1) * 4, dtype=t.uint8, device=dev)
         _lib.call("flux_panel8_fill", m, n, csr.indptr.data_ptr(), csr.indices.data_ptr(),
                   csr.data.data_ptr(), _lib.dtype_code(csr.data.dtype), self.qptr.data_ptr(),
-                  self.quads, self.rowexp.data_ptr(), self.panel.data_ptr(), st)
+                  self.quads, self.rowexp.data_ptr(), self.panel.data_ptr(),
+                  self.qmask.data_ptr(), st)
         self.nnz = csr.nnz
         self.plan = like.plan
         self._work = {}
+        self.skip_zero = True       # products skip the zero subtiles' reads
 
     @property
     def nbytes(self):
@@ -226,7 +230,8 @@ class SlicedPanelMatrix:
         _lib.call("flux_panel8_mm", self.shape[0], self.colptr.data_ptr(), self.cols.data_ptr(),
                   self.qptr.data_ptr(), self.panel.data_ptr(), self.rowexp.data_ptr(), k,
                   work.data_ptr(), _lib.ptr(E), _lib.ptr(B), _lib.ptr(rho), float(rho_s),
-                  int(left), out.data_ptr(), _lib.ptr(colmax), *plan, int(nslices), st)
+                  int(left), out.data_ptr(), _lib.ptr(colmax), *plan, int(nslices),
+                  self.qmask.data_ptr() if self.skip_zero else None, st)
         return out
 
 

@@ -168,6 +168,32 @@ __global__ void __launch_bounds__(256) k_panel8_fill(int64_t nrows, int64_t ncol
   }
 }
 
+// Zero subtiles: byte g of qmask[q] is 1 if rows 16 g .. 16 g + 15 of quad q
+// hold a nonzero slice byte, else 0.  One wave per (quad, row group).
+__global__ void __launch_bounds__(256) k_panel8_qmask(int64_t total_quads,
+                                                      const int8_t *__restrict__ panel8,
+                                                      uint8_t *__restrict__ qmask) {
+  const int64_t wv = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (wv >= total_quads * 4) return;
+  const int64_t q = wv >> 2;
+  const int g = (int)(wv & 3);
+  const v4i *p = reinterpret_cast<const v4i *>(panel8 + q * kQuadBytes +
+                                               (16 * g + (lane >> 2)) * kQuadCols +
+                                               16 * (lane & 3));
+  int any = 0;
+  for (int i = 0; i < kSlices; ++i) {
+    const v4i v = p[i * (kSliceTile / 16)];
+    any |= v[0] | v[1] | v[2] | v[3];
+  }
+  const bool nz = __any(any != 0);
+  if (lane == 0) qmask[4 * q + g] = nz ? 1 : 0;
+}
+
+// A quad of zero slices: the panel loads of a zero subtile read it instead
+// (an L2-resident line) and the subtile's bytes are not fetched from HBM.
+__device__ __align__(256) int8_t g_zero_quad[kQuadBytes];
+
 __global__ void k_zero_u32(uint32_t *p, int64_t n) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) p[k] = 0u;
@@ -366,9 +392,11 @@ __global__ void __launch_bounds__(256, 2) k_panel8_mm(
     const double *__restrict__ E, const double *__restrict__ B, const double *__restrict__ rho,
     double rho_s, int left, double *__restrict__ out, double *__restrict__ bcm,
     const int32_t *__restrict__ items, const int32_t *__restrict__ nsplit,
-    const int32_t *__restrict__ slot0, double *__restrict__ ws, int64_t nitems) {
+    const int32_t *__restrict__ slot0, double *__restrict__ ws, int64_t nitems,
+    const uint32_t *__restrict__ qmask) {
   __shared__ __align__(16) uint8_t xs[2][S * kXsSlice];
   __shared__ __align__(16) int32_t ids[4][kQuadCols];
+  __shared__ uint32_t qms[4];
   const int64_t pair = xcd_block(blockIdx.x, nitems * nh);
   const int64_t it = pair / nh;
   const int h = (int)(pair - it * nh);
@@ -380,8 +408,10 @@ __global__ void __launch_bounds__(256, 2) k_panel8_mm(
   const int64_t c0 = colptr[b] + kQuadCols * itm.q0;
   const int64_t ncl = max((int64_t)0, min(colptr[b + 1], colptr[b] + kQuadCols * itm.q1) - c0);
   const int64_t nq = itm.q1 - itm.q0;
-  const v4i *pb = reinterpret_cast<const v4i *>(panel8 + (qptr[b] + itm.q0) * kQuadBytes +
+  const int64_t qbase = qptr[b] + itm.q0;
+  const v4i *pb = reinterpret_cast<const v4i *>(panel8 + qbase * kQuadBytes +
                                                 (16 * w + li) * kQuadCols + 16 * lg);
+  const v4i *pz = reinterpret_cast<const v4i *>(g_zero_quad + (16 * w + li) * kQuadCols + 16 * lg);
   const int8_t *xg = x8 + h * (int64_t)kXRow;
   const int64_t xstride = (int64_t)nh * kXRow;
   v4i acc[S][4];
@@ -391,6 +421,10 @@ __global__ void __launch_bounds__(256, 2) k_panel8_mm(
     // every load address is clamped into the block, so nothing is masked
     auto load_id = [&](int64_t s) -> int32_t {
       return cols[c0 + min(s * kQuadCols + lane, ncl - 1)];
+    };
+    // the quad's zero-subtile flags (all ones without qmask)
+    auto load_qm = [&](int64_t s) -> uint32_t {
+      return qmask ? qmask[qbase + min(s, nq - 1)] : 0xffffffffu;
     };
     auto load_x = [&](int64_t s, v4i (&xr)[S]) {
       const int32_t *id = ids[s & 3];
@@ -409,7 +443,9 @@ __global__ void __launch_bounds__(256, 2) k_panel8_mm(
       }
     };
     auto load_p = [&](int64_t s, v4i (&pf)[S]) {
-      const v4i *p = pb + min(s, nq - 1) * (kQuadBytes / 16);
+      // a zero subtile of this wave reads the zero quad (no branch)
+      const bool nz = (qms[s & 3] >> (8 * w)) & 0xffu;
+      const v4i *p = nz ? pb + min(s, nq - 1) * (kQuadBytes / 16) : pz;
       for (int i = 0; i < S; ++i) pf[i] = p[i * (kSliceTile / 16)];
     };
     v4i xr[S];
@@ -419,6 +455,7 @@ __global__ void __launch_bounds__(256, 2) k_panel8_mm(
     ids[0][lane] = load_id(0);
     ids[1][lane] = load_id(1);
     ids[2][lane] = load_id(2);
+    if (tid < 3) qms[tid] = load_qm(tid);
     __syncthreads();
     load_x(0, xr);
     load_p(0, p0);
@@ -428,6 +465,7 @@ __global__ void __launch_bounds__(256, 2) k_panel8_mm(
       const int buf = (int)(s & 1);
       load_x(s + 1, xr);
       const int32_t idn = load_id(s + 3);
+      const uint32_t qmn = load_qm(s + 3);
       load_p(s + 1, pl);
       // keep every load of the stage ahead of its MFMAs
       __builtin_amdgcn_sched_barrier(0);
@@ -445,6 +483,7 @@ __global__ void __launch_bounds__(256, 2) k_panel8_mm(
       }
       store_x(buf ^ 1, xr);
       ids[(s + 3) & 3][lane] = idn;
+      if (tid == 0) qms[(s + 3) & 3] = qmn;
       __syncthreads();
     };
     int64_t s = 0;
@@ -541,7 +580,7 @@ extern "C" int flux_panel8_layout(int64_t nblk, const int64_t *colptr, int64_t *
 extern "C" int flux_panel8_fill(int64_t nrows, int64_t ncols, const int64_t *indptr,
                                 const int32_t *indices, const void *data, int dtype,
                                 const int64_t *qptr, int64_t total_quads, int32_t *rowexp,
-                                int8_t *panel8, void *stream) {
+                                int8_t *panel8, uint8_t *qmask, void *stream) {
   FLUX_CHECK(nrows >= 0 && ncols >= 0 && total_quads >= 0, "flux_panel8_fill: bad sizes");
   FLUX_CHECK(2 * panel_lds_words(ncols) * 4 <= 150 * 1024,
              "flux_panel8_fill: %lld columns exceed the LDS bitmap limit", (long long)ncols);
@@ -567,6 +606,9 @@ extern "C" int flux_panel8_fill(int64_t nrows, int64_t ncols, const int64_t *ind
     hipLaunchKernelGGL(k_panel8_fill<double>, dim3((unsigned)nblk), dim3(256), lds, st, nrows,
                        ncols, indptr, indices, (const double *)data, qptr, rowexp, panel8);
   }
+  if (qmask && total_quads > 0)
+    hipLaunchKernelGGL(k_panel8_qmask, dim3(ceil_div_u(total_quads * 4, 4)), dim3(256), 0, st,
+                       total_quads, panel8, qmask);
   FLUX_LAUNCH_CHECK();
   return 0;
 }
@@ -616,13 +658,14 @@ extern "C" int flux_panel8_mm(int64_t nrows, const int64_t *colptr, const int32_
                               double *colmax, const int32_t *items, int64_t n_items,
                               const int32_t *nsplit, const int32_t *slot0,
                               const int32_t *split_blocks, int64_t n_split, double *ws,
-                              int nslices, void *stream) {
+                              int nslices, const uint8_t *qmask, void *stream) {
   FLUX_CHECK(nrows >= 0 && nrhs >= 0, "flux_panel8_mm: bad sizes");
   FLUX_CHECK(nslices >= 3 && nslices <= kSlices, "flux_panel8_mm: nslices must be 3, 4 or 5");
   const int64_t nblk = (nrows + kPanelRows - 1) / kPanelRows;
   if (nblk == 0 || nrhs == 0) return 0;
   FLUX_CHECK(colptr && cols && qptr && panel8 && rowexp && work && out,
              "flux_panel8_mm: NULL array");
+  FLUX_CHECK(((uintptr_t)qmask & 3) == 0, "flux_panel8_mm: qmask must be 4-byte aligned");
   FLUX_CHECK(E == nullptr || B != nullptr, "flux_panel8_mm: E needs B");
   FLUX_CHECK(items == nullptr ||
                  (n_items >= nblk && nsplit && slot0 && (n_split == 0 || (split_blocks && ws))),
@@ -639,7 +682,8 @@ extern "C" int flux_panel8_mm(int64_t nrows, const int64_t *colptr, const int32_
   auto kernel = nslices == 5 ? k_panel8_mm<5> : nslices == 4 ? k_panel8_mm<4> : k_panel8_mm<3>;
   hipLaunchKernelGGL(kernel, dim3((unsigned)(nit * nh)), dim3(256), 0, st, nrows, nblk,
                      (int)nh, colptr, cols, qptr, panel8, rowexp, x8, xexp, nrhs, E, B, rho,
-                     rho_s, left, out, bcm, items, nsplit, slot0, ws, nit);
+                     rho_s, left, out, bcm, items, nsplit, slot0, ws, nit,
+                     reinterpret_cast<const uint32_t *>(qmask));
   if (items && n_split > 0)
     hipLaunchKernelGGL(k_panel8_reduce, dim3((unsigned)(n_split * nh)), dim3(256), 0, st,
                        split_blocks, nsplit, slot0, ws, (int)nh, nrows, rowexp, xexp, nrhs, E, B,

@@ -379,11 +379,14 @@ int flux_panel8_layout(int64_t nblk, const int64_t *colptr, int64_t *qptr,
                        int64_t *total_quads, int64_t *max_cols, void *stream);
 
 /* rowexp[r] (nrows int32) and the sliced panels (total_quads * 20480
- * bytes) from the CSR values (F32 or F64). */
+ * bytes) from the CSR values (F32 or F64).  qmask (optional, total_quads *
+ * 4 bytes, 4-byte aligned): byte 4 q + g is 1 if rows 16 g .. 16 g + 15 of
+ * quad q hold a nonzero, else 0; flux_panel8_mm skips the HBM reads of the
+ * zero subtiles it marks. */
 int flux_panel8_fill(int64_t nrows, int64_t ncols, const int64_t *indptr,
                      const int32_t *indices, const void *data, int dtype,
                      const int64_t *qptr, int64_t total_quads, int32_t *rowexp,
-                     int8_t *panel8, void *stream);
+                     int8_t *panel8, uint8_t *qmask, void *stream);
 
 /* Bytes of the X-slice workspace for n rows and nrhs right-hand sides (it
  * also holds the column maxima of flux_panel8_mm on the square n x n
@@ -405,6 +408,7 @@ int flux_slice_x(int64_t n, int64_t nrhs, const double *X, const double *rho,
  * i + j < nslices; 5 is the full product, fewer slices bound the error by
  * about 2^(e_r + e_k - 7 nslices) per term (for an X much smaller than the
  * quantity the result is compared with). */
+/* qmask: the flags of flux_panel8_fill, or NULL (every subtile read). */
 int flux_panel8_mm(int64_t nrows, const int64_t *colptr, const int32_t *cols,
                    const int64_t *qptr, const int8_t *panel8,
                    const int32_t *rowexp, int64_t nrhs, const void *work,
@@ -412,7 +416,8 @@ int flux_panel8_mm(int64_t nrows, const int64_t *colptr, const int32_t *cols,
                    double rho_s, int left, double *out, double *colmax,
                    const int32_t *items, int64_t n_items, const int32_t *nsplit,
                    const int32_t *slot0, const int32_t *split_blocks,
-                   int64_t n_split, double *ws, int nslices, void *stream);
+                   int64_t n_split, double *ws, int nslices, const uint8_t *qmask,
+                   void *stream);
 
 /* ---------------------------------------------------------------------- */
 /* Column-blocked vector kernels (solver building blocks; blocks are      */

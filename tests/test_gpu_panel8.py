@@ -83,6 +83,27 @@ def test_panel8_layout_and_slices(setup):
         tol = np.ldexp(1.0, er[r0:r1] - 35)[:, None]
         assert (abs(dec - want) <= tol).all(), b
         assert (dec[:, ncl:] == 0).all() and (dec[want == 0] == 0).all()
+    # zero-subtile flags: byte 4 q + g set iff rows 16 g .. 16 g + 15 of
+    # quad q hold a nonzero slice byte
+    qm = P8.qmask.cpu().numpy()[:4 * P8.quads].reshape(P8.quads, 4)
+    sub = (panel != 0).reshape(P8.quads, SLICES, 4, 16, 64).any(axis=(1, 3, 4))
+    assert np.array_equal(qm.astype(bool), sub)
+    assert 0 < qm.sum() < qm.size
+
+
+def test_panel8_zero_subtile_skip_is_exact(setup):
+    """Reading the zero quad for flagged subtiles changes nothing."""
+    torch, sm, csr, Pb, P8, ct = setup
+    n = csr.shape[0]
+    X = torch.rand((n, 64), dtype=torch.float64, device=_dev())
+    Y1, Y0 = torch.empty_like(X), torch.empty_like(X)
+    P8.mm_f64(X, Y1)
+    P8.skip_zero = False
+    try:
+        P8.mm_f64(X, Y0)
+    finally:
+        P8.skip_zero = True
+    assert torch.equal(Y1, Y0)
 
 
 @pytest.mark.parametrize("k", [64, 16, 5, 70, 33])
