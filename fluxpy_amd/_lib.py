@@ -67,8 +67,9 @@ SIGNATURES = {
     "flux_jacobi_update": (_c_int, [_c_i64, _c_i64, _vp, _vp, _vp, _vp, _c_dbl, _c_int, _vp,
                                     _c_int, _vp, _vp]),
     "flux_panel_count": (_c_int, [_c_i64, _c_i64, _vp, _vp, _vp, _vp]),
+    "flux_panel_order": (_c_int, [_c_i64, _c_i64, _vp, _vp, _vp, _c_i64, _vp, _vp, _vp]),
     "flux_panel_fill": (_c_int, [_c_i64, _c_i64, _vp, _vp, _vp, _c_int, _vp, _c_i64, _vp, _vp,
-                                 _c_int, _vp]),
+                                 _c_int, _vp, _vp]),
     "flux_panel_plan": (_c_int, [_c_i64, _vp, _c_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "flux_panel_mm_bf16": (_c_int, [_c_i64, _vp, _vp, _vp, _vp, _c_i64, _vp, _vp,
                                     ctypes.c_float, _c_int, _vp, _vp, _vp, _vp, _vp, _c_i64,
@@ -82,7 +83,7 @@ SIGNATURES = {
                                     _vp]),
     "flux_panel8_layout": (_c_int, [_c_i64, _vp, _vp, _vp, _vp, _vp]),
     "flux_panel8_fill": (_c_int, [_c_i64, _c_i64, _vp, _vp, _vp, _c_int, _vp, _c_i64, _vp, _vp,
-                                  _vp, _vp]),
+                                  _vp, _vp, _vp, _vp]),
     "flux_slice_x_bytes": (_c_i64, [_c_i64, _c_i64]),
     "flux_slice_x": (_c_int, [_c_i64, _c_i64, _vp, _vp, _c_dbl, _c_int, _vp, _vp]),
     "flux_panel8_mm": (_c_int, [_c_i64, _vp, _vp, _vp, _vp, _vp, _c_i64, _vp, _vp, _vp, _vp,

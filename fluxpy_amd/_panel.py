@@ -70,7 +70,11 @@ class PanelMatrix:
 
     ROWS = 64
 
-    def __init__(self, csr, values="bf16", colptr=None, cols=None, plan=None):
+    # columns of each block ordered by their 16-row occupancy mask (see
+    # flux_panel_order); False keeps them in increasing order
+    ORDER_BY_MASK = True
+
+    def __init__(self, csr, values="bf16", colptr=None, cols=None, plan=None, order=None):
         t = _lib.require_gpu()
         m, n = csr.shape
         self.shape = (m, n)
@@ -89,10 +93,19 @@ class PanelMatrix:
             total_cols = total.value
             cols = t.empty(max(total_cols, 1), dtype=t.int32, device=dev)
             fill_cols = True
+            if self.ORDER_BY_MASK and nblk > 0:
+                cp = colptr.cpu().numpy()
+                max_cols = int(np.diff(cp).max())
+                nw = (n + 31) // 32
+                if (2 * nw + max_cols) * 4 <= 140 * 1024:
+                    order = t.empty(max(total_cols, 1), dtype=t.int32, device=dev)
+                    _lib.call("flux_panel_order", m, n, csr.indptr.data_ptr(),
+                              csr.indices.data_ptr(), colptr.data_ptr(), max_cols,
+                              order.data_ptr(), cols.data_ptr(), st)
         else:
             total_cols = int(colptr[-1].item())
             fill_cols = False
-        self.colptr, self.cols = colptr, cols
+        self.colptr, self.cols, self.order = colptr, cols, order
         self.total_cols = total_cols
         pdt = {"bf16": t.bfloat16, "fp32": t.float32}[values]
         self.panel = t.empty(max(total_cols * self.ROWS, 1), dtype=pdt, device=dev)
@@ -100,14 +113,15 @@ class PanelMatrix:
         _lib.call("flux_panel_fill", m, n, csr.indptr.data_ptr(), csr.indices.data_ptr(),
                   csr.data.data_ptr(), _lib.dtype_code(csr.data.dtype), colptr.data_ptr(),
                   total_cols, scratch_cols.data_ptr(), self.panel.data_ptr(),
-                  _lib.dtype_code(pdt), st)
+                  _lib.dtype_code(pdt), _lib.ptr(order), st)
         self.nnz = csr.nnz
         # balanced work items (shared with siblings); None -> one per block
         self.plan = plan if plan is not None else PanelPlan(colptr, nblk)
 
     def sibling(self, csr, values):
         """Panels of the same pattern in another precision (shares cols)."""
-        return PanelMatrix(csr, values, colptr=self.colptr, cols=self.cols, plan=self.plan)
+        return PanelMatrix(csr, values, colptr=self.colptr, cols=self.cols, plan=self.plan,
+                           order=self.order)
 
     @property
     def nbytes(self):
@@ -158,7 +172,7 @@ class SlicedPanelMatrix:
         dev = csr.data.device
         st = _lib.stream()
         nblk = (m + self.ROWS - 1) // self.ROWS
-        self.colptr, self.cols = like.colptr, like.cols
+        self.colptr, self.cols, self.order = like.colptr, like.cols, like.order
         self.qptr = t.empty(nblk + 1, dtype=t.int64, device=dev)
         total, widest = ctypes.c_int64(), ctypes.c_int64()
         _lib.call("flux_panel8_layout", nblk, self.colptr.data_ptr(), self.qptr.data_ptr(),
@@ -171,7 +185,7 @@ class SlicedPanelMatrix:
         _lib.call("flux_panel8_fill", m, n, csr.indptr.data_ptr(), csr.indices.data_ptr(),
                   csr.data.data_ptr(), _lib.dtype_code(csr.data.dtype), self.qptr.data_ptr(),
                   self.quads, self.rowexp.data_ptr(), self.panel.data_ptr(),
-                  self.qmask.data_ptr(), st)
+                  self.qmask.data_ptr(), self.colptr.data_ptr(), _lib.ptr(self.order), st)
         self.nnz = csr.nnz
         self.plan = like.plan
         self._work = {}

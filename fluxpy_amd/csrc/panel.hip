@@ -76,13 +76,86 @@ __device__ __forceinline__ void store_val(__hip_bfloat16 *p, double v) {
 __device__ __forceinline__ void store_val(float *p, double v) { *p = (float)v; }
 __device__ __forceinline__ void store_val(double *p, double v) { *p = v; }
 
-// cols / panel of every row block (panel must be zeroed beforehand).
+// Column order of every row block: the distinct columns sorted by their
+// occupancy mask (bit g: rows 16 g .. 16 g + 15 of the block touch the
+// column), columns of one mask in increasing order.  Columns of one mask
+// share their zero 16-row subtiles, so sorting by mask gathers the zero
+// subtiles of a 64-column quad that a product can skip.  Writes cols in the
+// new order (padding positions get column 0) and order[colptr[b] + r] =
+// position of the block's r-th smallest column.  LDS: the bitmap and its
+// prefix (2 ceil(ncols / 32) words) and one mask word per distinct column.
+__global__ void __launch_bounds__(256) k_panel_order(int64_t nrows, int64_t ncols,
+                                                     const int64_t *__restrict__ indptr,
+                                                     const int32_t *__restrict__ indices,
+                                                     const int64_t *__restrict__ colptr,
+                                                     int32_t *__restrict__ order,
+                                                     int32_t *__restrict__ cols) {
+  extern __shared__ uint32_t sm[];
+  __shared__ uint32_t part[256];
+  __shared__ uint32_t tab[16 * 256];  // per (mask, thread): count, then next position
+  const int64_t b = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int64_t nw = (ncols + 31) / 32;
+  uint32_t *bm = sm, *pre = sm + nw, *msk = sm + 2 * nw;
+  const int64_t r0 = b * kPanelRows, r1 = min(nrows, r0 + kPanelRows);
+  const int64_t nd = block_columns(indptr, indices, r0, r1, nw, bm, pre, part);
+  const int64_t base = colptr[b], npad = colptr[b + 1] - base;
+  for (int64_t i = tid; i < nd; i += blockDim.x) msk[i] = 0u;
+  for (int i = tid; i < 16 * 256; i += blockDim.x) tab[i] = 0u;
+  __syncthreads();
+  for (int64_t r = r0; r < r1; ++r) {
+    const uint32_t bit = 1u << ((r - r0) >> 4);
+    for (int64_t k = indptr[r] + tid; k < indptr[r + 1]; k += blockDim.x)
+      atomicOr(&msk[column_pos(bm, pre, indices[k])], bit);
+  }
+  __syncthreads();
+  // stable counting sort by mask: thread t owns the ranks [q0, q1)
+  const int64_t per = (nd + blockDim.x - 1) / blockDim.x;
+  const int64_t q0 = min(nd, (int64_t)tid * per), q1 = min(nd, q0 + per);
+  for (int64_t i = q0; i < q1; ++i) tab[msk[i] * 256 + tid] += 1u;
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t acc = 0;
+    for (int i = 0; i < 16 * 256; ++i) {
+      const uint32_t v = tab[i];
+      tab[i] = acc;
+      acc += v;
+    }
+  }
+  __syncthreads();
+  if (q0 < q1) {
+    // word holding rank q0: the last w with pre[w] <= q0
+    int64_t lo = 0, hi = nw - 1;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi + 1) >> 1;
+      if (pre[mid] <= (uint32_t)q0) lo = mid; else hi = mid - 1;
+    }
+    int64_t w = lo;
+    uint32_t bits = bm[w];
+    for (uint32_t skip = (uint32_t)q0 - pre[w]; skip > 0; --skip) bits &= bits - 1;
+    for (int64_t i = q0; i < q1; ++i) {
+      while (bits == 0) bits = bm[++w];
+      const int q = __ffs(bits) - 1;
+      bits &= bits - 1;
+      const uint32_t pos = tab[msk[i] * 256 + tid]++;
+      order[base + i] = (int32_t)pos;
+      cols[base + pos] = (int32_t)(w * 32 + q);
+    }
+  }
+  for (int64_t p = nd + tid; p < npad; p += blockDim.x) cols[base + p] = 0;
+}
+
+// cols / panel of every row block (panel must be zeroed beforehand).  With
+// order (k_panel_order), column j of block b sits at position
+// order[colptr[b] + rank of j] and cols is not written; without it, the
+// columns are in increasing order.
 template <typename TA, typename TP>
 __global__ void __launch_bounds__(256) k_panel_fill(int64_t nrows, int64_t ncols,
                                                     const int64_t *__restrict__ indptr,
                                                     const int32_t *__restrict__ indices,
                                                     const TA *__restrict__ data,
                                                     const int64_t *__restrict__ colptr,
+                                                    const int32_t *__restrict__ order,
                                                     int32_t *__restrict__ cols,
                                                     TP *__restrict__ panel) {
   extern __shared__ uint32_t sm[];
@@ -96,21 +169,24 @@ __global__ void __launch_bounds__(256) k_panel_fill(int64_t nrows, int64_t ncols
   const int64_t base = colptr[b];
   const int64_t npad = colptr[b + 1] - base;
   // column list
-  for (int64_t w = threadIdx.x; w < nw; w += blockDim.x) {
-    uint32_t bits = bm[w];
-    uint32_t pos = pre[w];
-    while (bits) {
-      const int q = __ffs(bits) - 1;
-      bits &= bits - 1;
-      cols[base + pos] = (int32_t)(w * 32 + q);
-      ++pos;
+  if (!order) {
+    for (int64_t w = threadIdx.x; w < nw; w += blockDim.x) {
+      uint32_t bits = bm[w];
+      uint32_t pos = pre[w];
+      while (bits) {
+        const int q = __ffs(bits) - 1;
+        bits &= bits - 1;
+        cols[base + pos] = (int32_t)(w * 32 + q);
+        ++pos;
+      }
     }
+    for (int64_t p = ndist + threadIdx.x; p < npad; p += blockDim.x) cols[base + p] = 0;
   }
-  for (int64_t p = ndist + threadIdx.x; p < npad; p += blockDim.x) cols[base + p] = 0;
   // values
   for (int64_t r = r0; r < r1; ++r) {
     for (int64_t k = indptr[r] + threadIdx.x; k < indptr[r + 1]; k += blockDim.x) {
-      const uint32_t pos = column_pos(bm, pre, indices[k]);
+      const uint32_t rank = column_pos(bm, pre, indices[k]);
+      const uint32_t pos = order ? (uint32_t)order[base + rank] : rank;
       const int64_t chunk = pos >> 4, kk = pos & 15;
       store_val(panel + (base + chunk * 16) * kPanelRows + (r - r0) * 16 + kk,
                 (double)data[k]);
@@ -679,10 +755,28 @@ extern "C" int flux_panel_count(int64_t nrows, int64_t ncols, const int64_t *ind
   return 0;
 }
 
+extern "C" int flux_panel_order(int64_t nrows, int64_t ncols, const int64_t *indptr,
+                                const int32_t *indices, const int64_t *colptr,
+                                int64_t max_cols, int32_t *order, int32_t *cols,
+                                void *stream) {
+  FLUX_CHECK(nrows >= 0 && ncols >= 0 && max_cols >= 0, "flux_panel_order: bad sizes");
+  const size_t lds = (2 * panel_lds_words(ncols) + max_cols) * 4;
+  FLUX_CHECK(lds <= 140 * 1024,
+             "flux_panel_order: %lld columns per block exceed the LDS limit",
+             (long long)max_cols);
+  const int64_t nblk = (nrows + kPanelRows - 1) / kPanelRows;
+  if (nblk == 0) return 0;
+  FLUX_CHECK(indptr && indices && colptr && order && cols, "flux_panel_order: NULL array");
+  hipLaunchKernelGGL(k_panel_order, dim3((unsigned)nblk), dim3(256), lds, as_stream(stream),
+                     nrows, ncols, indptr, indices, colptr, order, cols);
+  FLUX_LAUNCH_CHECK();
+  return 0;
+}
+
 extern "C" int flux_panel_fill(int64_t nrows, int64_t ncols, const int64_t *indptr,
                                const int32_t *indices, const void *data, int dtype,
                                const int64_t *colptr, int64_t total_cols, int32_t *cols,
-                               void *panel, int pdtype, void *stream) {
+                               void *panel, int pdtype, const int32_t *order, void *stream) {
   FLUX_CHECK(nrows >= 0 && ncols >= 0 && total_cols >= 0, "flux_panel_fill: bad sizes");
   FLUX_CHECK(2 * panel_lds_words(ncols) * 4 <= 150 * 1024,
              "flux_panel_fill: %lld columns exceed the LDS bitmap limit", (long long)ncols);
@@ -698,7 +792,8 @@ extern "C" int flux_panel_fill(int64_t nrows, int64_t ncols, const int64_t *indp
   do {                                                                                          \
     hipLaunchKernelGGL(k_zero_any<TP>, dim3(gz), dim3(256), 0, st, (TP *)panel, nel);            \
     hipLaunchKernelGGL((k_panel_fill<TA, TP>), dim3((unsigned)nblk), dim3(256), lds, st, nrows, \
-                       ncols, indptr, indices, (const TA *)data, colptr, cols, (TP *)panel);    \
+                       ncols, indptr, indices, (const TA *)data, colptr, order, cols,           \
+                       (TP *)panel);                                                              \
   } while (0)
   if (dtype == FLUX_F32 && pdtype == FLUX_BF16)
     PF(float, __hip_bfloat16);

@@ -145,6 +145,8 @@ __global__ void __launch_bounds__(256) k_panel8_fill(int64_t nrows, int64_t ncol
                                                      const int32_t *__restrict__ indices,
                                                      const TA *__restrict__ data,
                                                      const int64_t *__restrict__ qptr,
+                                                     const int64_t *__restrict__ colptr,
+                                                     const int32_t *__restrict__ order,
                                                      const int32_t *__restrict__ rowexp,
                                                      int8_t *__restrict__ panel8) {
   extern __shared__ uint32_t sm[];
@@ -159,7 +161,8 @@ __global__ void __launch_bounds__(256) k_panel8_fill(int64_t nrows, int64_t ncol
   for (int64_t r = r0; r < r1; ++r) {
     const int e = rowexp[r];
     for (int64_t k = indptr[r] + threadIdx.x; k < indptr[r + 1]; k += blockDim.x) {
-      const uint32_t pos = column_pos(bm, pre, indices[k]);
+      const uint32_t rank = column_pos(bm, pre, indices[k]);
+      const uint32_t pos = order ? (uint32_t)order[colptr[b] + rank] : rank;
       int8_t a[kSlices];
       slice5((double)data[k], e, a);
       int8_t *dst = base + (int64_t)(pos >> 6) * kQuadBytes + (r - r0) * kQuadCols + (pos & 63);
@@ -580,7 +583,8 @@ extern "C" int flux_panel8_layout(int64_t nblk, const int64_t *colptr, int64_t *
 extern "C" int flux_panel8_fill(int64_t nrows, int64_t ncols, const int64_t *indptr,
                                 const int32_t *indices, const void *data, int dtype,
                                 const int64_t *qptr, int64_t total_quads, int32_t *rowexp,
-                                int8_t *panel8, uint8_t *qmask, void *stream) {
+                                int8_t *panel8, uint8_t *qmask, const int64_t *colptr,
+                                const int32_t *order, void *stream) {
   FLUX_CHECK(nrows >= 0 && ncols >= 0 && total_quads >= 0, "flux_panel8_fill: bad sizes");
   FLUX_CHECK(2 * panel_lds_words(ncols) * 4 <= 150 * 1024,
              "flux_panel8_fill: %lld columns exceed the LDS bitmap limit", (long long)ncols);
@@ -589,6 +593,7 @@ extern "C" int flux_panel8_fill(int64_t nrows, int64_t ncols, const int64_t *ind
   FLUX_CHECK(indptr && indices && data && qptr && rowexp && panel8,
              "flux_panel8_fill: NULL array");
   FLUX_CHECK(dtype == FLUX_F32 || dtype == FLUX_F64, "flux_panel8_fill: values must be F32/F64");
+  FLUX_CHECK(order == nullptr || colptr != nullptr, "flux_panel8_fill: order needs colptr");
   hipStream_t st = as_stream(stream);
   const int64_t nword = total_quads * (kQuadBytes / 4);
   if (nword > 0)
@@ -599,12 +604,14 @@ extern "C" int flux_panel8_fill(int64_t nrows, int64_t ncols, const int64_t *ind
     hipLaunchKernelGGL(k_row_exp<float>, dim3(ceil_div_u(nrows, 4)), dim3(256), 0, st, nrows,
                        indptr, (const float *)data, rowexp);
     hipLaunchKernelGGL(k_panel8_fill<float>, dim3((unsigned)nblk), dim3(256), lds, st, nrows,
-                       ncols, indptr, indices, (const float *)data, qptr, rowexp, panel8);
+                       ncols, indptr, indices, (const float *)data, qptr, colptr, order, rowexp,
+                       panel8);
   } else {
     hipLaunchKernelGGL(k_row_exp<double>, dim3(ceil_div_u(nrows, 4)), dim3(256), 0, st, nrows,
                        indptr, (const double *)data, rowexp);
     hipLaunchKernelGGL(k_panel8_fill<double>, dim3((unsigned)nblk), dim3(256), lds, st, nrows,
-                       ncols, indptr, indices, (const double *)data, qptr, rowexp, panel8);
+                       ncols, indptr, indices, (const double *)data, qptr, colptr, order, rowexp,
+                       panel8);
   }
   if (qmask && total_quads > 0)
     hipLaunchKernelGGL(k_panel8_qmask, dim3(ceil_div_u(total_quads * 4, 4)), dim3(256), 0, st,

@@ -297,12 +297,25 @@ int flux_jacobi_update(int64_t n, int64_t nrhs, const void *E, const void *Y,
 int flux_panel_count(int64_t nrows, int64_t ncols, const int64_t *indptr,
                      const int32_t *indices, int64_t *counts, void *stream);
 
-/* Fill cols (total_cols = colptr[nblk]) and the zero-padded panels with the
- * CSR values converted to pdtype (BF16, F32 or F64). */
+/* Column order of each block by occupancy mask: the distinct columns of
+ * block b sorted by the 4-bit mask of the 16-row groups that touch them,
+ * then by index, so that 64-column quads gather zero 16-row subtiles the
+ * products skip.  Writes cols (total_cols = colptr[nblk]; padding positions
+ * get column 0) and order[colptr[b] + r] = position of the block's r-th
+ * smallest column.  max_cols: the largest colptr[b+1] - colptr[b]; the
+ * bitmap and one word per column must fit in 140 KiB of LDS. */
+int flux_panel_order(int64_t nrows, int64_t ncols, const int64_t *indptr,
+                     const int32_t *indices, const int64_t *colptr, int64_t max_cols,
+                     int32_t *order, int32_t *cols, void *stream);
+
+/* Fill the zero-padded panels with the CSR values converted to pdtype
+ * (BF16, F32 or F64).  order == NULL: columns in increasing order per block,
+ * cols written here; with order (flux_panel_order), cols is the order's and
+ * is not written. */
 int flux_panel_fill(int64_t nrows, int64_t ncols, const int64_t *indptr,
                     const int32_t *indices, const void *data, int dtype,
                     const int64_t *colptr, int64_t total_cols, int32_t *cols,
-                    void *panel, int pdtype, void *stream);
+                    void *panel, int pdtype, const int32_t *order, void *stream);
 
 /* Work plan of the panel products (load balance): every row block is cut
  * into ceil(quads / cap_quads) items of balanced size, in block order with
@@ -386,7 +399,8 @@ int flux_panel8_layout(int64_t nblk, const int64_t *colptr, int64_t *qptr,
 int flux_panel8_fill(int64_t nrows, int64_t ncols, const int64_t *indptr,
                      const int32_t *indices, const void *data, int dtype,
                      const int64_t *qptr, int64_t total_quads, int32_t *rowexp,
-                     int8_t *panel8, uint8_t *qmask, void *stream);
+                     int8_t *panel8, uint8_t *qmask, const int64_t *colptr,
+                     const int32_t *order, void *stream);
 
 /* Bytes of the X-slice workspace for n rows and nrhs right-hand sides (it
  * also holds the column maxima of flux_panel8_mm on the square n x n

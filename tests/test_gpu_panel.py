@@ -37,12 +37,20 @@ def test_panel_structure(setup):
     colptr = P.colptr.cpu().numpy()
     cols = P.cols.cpu().numpy()
     assert (np.diff(colptr) % 16 == 0).all()
+    order = P.order.cpu().numpy()
     for b in range(len(colptr) - 1):
-        blk = sp[64 * b:64 * (b + 1)]
+        blk = sp[64 * b:64 * (b + 1)].tocsr()
         want = np.unique(blk.indices)
         got = cols[colptr[b]:colptr[b + 1]]
-        assert np.array_equal(got[:want.size], want)
+        # columns ordered by (16-row occupancy mask, index), padding with 0
+        mask = np.zeros(want.size, dtype=np.int64)
+        for r in range(blk.shape[0]):
+            mask[np.searchsorted(want, blk.indices[blk.indptr[r]:blk.indptr[r + 1]])] |= 1 << (r // 16)
+        key = np.lexsort((want, mask))
+        assert np.array_equal(got[:want.size], want[key])
         assert (got[want.size:] == 0).all()
+        # order maps the r-th smallest column to its position
+        assert np.array_equal(got[order[colptr[b]:colptr[b] + want.size]], want)
     assert 0.0 < P.fill <= 1.0
 
 
