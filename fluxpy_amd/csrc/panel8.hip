@@ -475,6 +475,10 @@ __global__ void __launch_bounds__(256, 2) k_panel8_mm(
       // A operand of tile t, slice j: columns 16 lg + 8 h + e (h = 0, 1) of
       // right-hand side 16 t + li, two transposing reads
       const uint8_t *xq = xs[buf] + (16 * lg + (li >> 1)) * kXsRow + 8 * (li & 1);
+      // a zero subtile of this wave contributes nothing: its MFMAs are skipped
+      // (a wave-uniform branch with no load inside)
+      const bool nzs = (qms[s & 3] >> (8 * w)) & 0xffu;
+      if (nzs)
       for (int t = 0; t < 4; ++t) {
         for (int j = 0; j < S; ++j) {
           const uint8_t *pa = xq + j * kXsSlice + 16 * t;
