@@ -54,11 +54,18 @@ __global__ void __launch_bounds__(256) k_csr_rows(
     const TA *__restrict__ data, const TX *__restrict__ X, int64_t nrhs, int R,
     TX *__restrict__ Y, const TX *__restrict__ E, const TX *__restrict__ rho, double rho_s,
     double *__restrict__ colmax) {
-  const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  if (row >= m) return;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int G = 64 / R;
   const int g = lane / R, c = lane % R;
+  // Jacobi steps with at most 64 right-hand sides: each wave walks rows
+  // w0, w0 + (waves in the grid), ... and keeps its column maxima, and the
+  // block issues one atomic per column (one atomic per row from every wave
+  // contends on the same address).  Otherwise one row per wave.
+  const bool stride = MODE != SPMM_PLAIN && nrhs <= 64;
+  const int64_t step = stride ? (int64_t)gridDim.x * (blockDim.x >> 6) : m;
+  double rmax = 0.0;
+  for (int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; row < m;
+       row += step) {
   const int64_t k0 = indptr[row], k1 = indptr[row + 1];
   for (int64_t c0 = 0; c0 < nrhs; c0 += R) {
     const int64_t col = c0 + c;
@@ -68,28 +75,28 @@ __global__ void __launch_bounds__(256) k_csr_rows(
       // kU consecutive strides per pass: the index and value loads of a pass,
       // then its gathers, are all in flight before the first product (each
       // lane still adds its terms in increasing k order)
+      // Loads go out unconditionally from clamped addresses (k1 - 1 >= kb
+      // is a valid entry of this row): a load behind a branch would make
+      // the compiler wait for the earlier ones and serialise the pass.
       constexpr int kU = 8;
       for (int64_t kb = k0 + g; kb < k1; kb += (int64_t)kU * G) {
         int32_t jj[kU];
         TAcc av[kU], xv[kU];
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
-          const int64_t k = kb + (int64_t)u * G;
-          jj[u] = k < k1 ? indices[k] : -1;
-          av[u] = k < k1 ? (TAcc)widen(data[k]) : (TAcc)0;
+          const int64_t k = min(kb + (int64_t)u * G, k1 - 1);
+          jj[u] = indices[k];
+          av[u] = (TAcc)widen(data[k]);
         }
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
-          TAcc x = 0;
-          if (jj[u] >= 0) {
-            x = (TAcc)X[(int64_t)jj[u] * nrhs + col];
-            if (MODE == SPMM_JACOBI_RIGHT) x = (rho ? (TAcc)rho[jj[u]] : (TAcc)rho_s) * x;
-          }
+          TAcc x = (TAcc)X[(int64_t)jj[u] * nrhs + col];
+          if (MODE == SPMM_JACOBI_RIGHT) x = (rho ? (TAcc)rho[jj[u]] : (TAcc)rho_s) * x;
           xv[u] = x;
         }
 #pragma unroll
         for (int u = 0; u < kU; ++u)
-          if (jj[u] >= 0) acc += av[u] * xv[u];
+          if (kb + (int64_t)u * G < k1) acc += av[u] * xv[u];
       }
     }
     for (int off = R; off < 64; off <<= 1) acc += __shfl_xor(acc, off, 64);
@@ -103,8 +110,21 @@ __global__ void __launch_bounds__(256) k_csr_rows(
         const TX b1 = (TX)((TAcc)E[o] + y);
         Y[o] = b1;
         const double diff = fabs((double)b1 - (double)X[o]);
-        atomic_max_nonneg(colmax + col, diff);
+        if (stride)
+          rmax = fmax(rmax, diff);
+        else
+          atomic_max_nonneg(colmax + col, diff);
       }
+    }
+  }
+  }
+  if (stride) {
+    __shared__ double sd[4][64];
+    sd[w][lane] = lane < R ? rmax : 0.0;
+    __syncthreads();
+    if (threadIdx.x < R && threadIdx.x < nrhs) {
+      const int t = threadIdx.x;
+      atomic_max_nonneg(colmax + t, fmax(fmax(sd[0][t], sd[1][t]), fmax(sd[2][t], sd[3][t])));
     }
   }
 }
@@ -116,7 +136,10 @@ static int launch_rows(int64_t m, const int64_t *indptr, const int32_t *indices,
   using TAcc = typename AccOf<decltype(typename WideOf<TA>::type() + TX())>::type;
   int R = 1;
   while (R < nrhs && R < 64) R <<= 1;
-  const unsigned blocks = ceil_div_u(m, 4);
+  // Jacobi steps with nrhs <= 64 walk rows with a bounded grid (see the kernel)
+  const bool stride = MODE != SPMM_PLAIN && nrhs <= 64;
+  const unsigned rb = ceil_div_u(m, 4), blocks = stride && rb > 2048u ? 2048u : rb;
+  if (blocks == 0) return 0;
   hipLaunchKernelGGL((k_csr_rows<TA, TX, TAcc, MODE>), dim3(blocks), dim3(256), 0, st, m,
                      indptr, indices, (const TA *)data, (const TX *)X, nrhs, R, (TX *)Y,
                      (const TX *)E, (const TX *)rho, rho_s, colmax);

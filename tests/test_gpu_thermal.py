@@ -172,3 +172,37 @@ def test_thermal_model_matches_oracle(torch_dev, oracle_mod):
         assert _max_rel(Tg, To) <= 1e-12, nstep
     assert nstep == nt - 1
     assert np.isfinite(Tg).all() and Tg.min() > 0
+
+
+def test_ff_npz_round_trip_drives_solves_and_thermal_model(torch_dev, tmp_path):
+    """The reference's examples keep F as a SciPy .npz (scipy.sparse.save_npz
+    of get_form_factor_matrix, examples/spherical_crater/collect_data.py:141)
+    and hand the loaded matrix to solve_radiosity and ThermalModel
+    (examples/gerlache/collect_time_dep_data_spinup.py:30-37).  The same
+    file drives the GPU solve and the thermal model unchanged."""
+    import scipy.sparse
+    from fluxpy_amd import shape
+    from fluxpy_amd.form_factors import get_form_factor_matrix
+    from fluxpy_amd.model import ThermalModel
+    from fluxpy_amd.solve import solve_radiosity
+    print("case: FF.npz round trip, spherical crater", flush=True)
+    V, F = meshes.spherical_crater(npts=20, seed=1003)
+    sm = shape.CgalTrimeshShapeModel(V.astype(np.float64), F)
+    FF = get_form_factor_matrix(sm)
+    path = tmp_path / "FF.npz"
+    scipy.sparse.save_npz(path, FF)
+    FF2 = scipy.sparse.load_npz(path)
+    E = sm.get_direct_irradiance(1365.0, np.array([0.6, 0.0, 0.8]))
+    B1, n1 = solve_radiosity(FF, E, 0.12)
+    B2, n2 = solve_radiosity(FF2, E, 0.12)
+    assert n1 == n2 and np.array_equal(B1, B2)
+    nz = 10
+    z = np.concatenate([[0.0], setgrid(nz, 1.1, 0.2)])
+    t = np.linspace(0, 3600.0, 3)
+    D = np.array([[0.6, 0.0, 0.8]] * 3)
+    runs = []
+    for M in (FF, FF2):
+        tm = ThermalModel(M, t, D, np.full(3, 1365.0), 0.12, z=z, T0=200.0, ti=120.0,
+                          rhoc=960000.0, emiss=0.95, Fgeotherm=0.0, shape_model=sm)
+        runs.append([T for T in tm])
+    assert len(runs[0]) == 2 and all(np.array_equal(a, b) for a, b in zip(*runs))
