@@ -206,3 +206,40 @@ def test_ff_npz_round_trip_drives_solves_and_thermal_model(torch_dev, tmp_path):
                           rhoc=960000.0, emiss=0.95, Fgeotherm=0.0, shape_model=sm)
         runs.append([T for T in tm])
     assert len(runs[0]) == 2 and all(np.array_equal(a, b) for a, b in zip(*runs))
+
+
+def test_thermal_model_extended_source_matches_oracle(torch_dev, oracle_mod):
+    """ThermalModel with several sun directions per time point (the
+    extended-source branch of model.py:186-199, insolation averaged over the
+    directions as shape.py:234-240) against the oracle's iteration."""
+    from fluxpy_amd import shape
+    from fluxpy_amd.form_factors import get_form_factor_matrix
+    from fluxpy_amd.model import ThermalModel
+    print("case: ThermalModel, extended source (4 directions), crater", flush=True)
+    V, F = meshes.spherical_crater(npts=20, seed=1004)
+    sm = shape.CgalTrimeshShapeModel(V.astype(np.float64), F)
+    FF = get_form_factor_matrix(sm)
+    nf = sm.num_faces
+    nt, M = 5, 4
+    t = np.linspace(0, 7200.0, nt)
+    rng = np.random.default_rng(9)
+    az = rng.uniform(0, 2 * np.pi, (nt, M))
+    el = np.deg2rad(rng.uniform(20, 40, (nt, M)))
+    D = np.stack([np.cos(el) * np.cos(az), np.cos(el) * np.sin(az), np.sin(el)], -1)
+    D = D.reshape(nt * M, 3)
+    F0 = np.full((nt * M, 1), 1365.0)
+    nz = 12
+    z = np.concatenate([[0.0], setgrid(nz, 1.1, 0.2)])
+    model = ThermalModel(FF, t, D, F0, 0.12, z=z, T0=210.0, ti=150.0, rhoc=1.0e6,
+                         emiss=0.9, Fgeotherm=0.0, shape_model=sm)
+    E_steps = [sm.get_direct_irradiance(F0[i * M:(i + 1) * M, :], D[i * M:(i + 1) * M])
+               for i in range(nt)]
+    assert E_steps[1].shape == (nf,)
+    ref = oracle_mod.thermal_model_run(FF, E_steps, t, 0.12, np.full(nf, 0.9), z, 210.0,
+                                       np.full(nz + 1, 150.0), np.full(nz + 1, 1.0e6), 0.0,
+                                       E_steps[0])
+    n = 0
+    for Tg, To in zip(model, ref):
+        n += 1
+        assert _max_rel(Tg, To) <= 1e-12, n
+    assert n == nt - 1
