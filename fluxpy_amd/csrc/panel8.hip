@@ -524,8 +524,12 @@ __global__ void __launch_bounds__(256, 2) k_panel8_mm(
 }
 
 // Split blocks of a planned product: the sum of the block's partials, in
-// slot order, and the epilogue (one workgroup per split block and group of
-// 64 right-hand sides).
+// slot order, and the epilogue.  One workgroup per (split block, group of 64
+// right-hand sides, tile t of 16 of them), so a block's four tiles reduce in
+// parallel; each thread sums 4 values of every slot.  Slots go four at a
+// time with all 16 loads issued before any add; the index of a slot past the
+// last is clamped to the last (a cached re-read) and its add skipped, so no
+// load sits behind a branch.
 __global__ void __launch_bounds__(256) k_panel8_reduce(
     const int32_t *__restrict__ split_blocks, const int32_t *__restrict__ nsplit,
     const int32_t *__restrict__ slot0, const double *__restrict__ ws, int nh, int64_t nrows,
@@ -533,25 +537,34 @@ __global__ void __launch_bounds__(256) k_panel8_reduce(
     const double *__restrict__ E, const double *__restrict__ B, const double *__restrict__ rho,
     double rho_s, int left, double *__restrict__ out, double *__restrict__ bcm) {
   __shared__ double s_cm[4 * kGroup];
-  const int64_t b = split_blocks[blockIdx.x / nh];
-  const int h = (int)(blockIdx.x % nh);
+  const int t = (int)(blockIdx.x & 3);
+  const int64_t g = blockIdx.x >> 2;
+  const int64_t b = split_blocks[g / nh];
+  const int h = (int)(g % nh);
   const int ns = nsplit[b], s0 = slot0[b];
-  // all 16 values of a slot are loaded before any is added, so a slot's
-  // loads are in flight together; the sums run over the slots in order
-  double y[4][4];
-  for (int t = 0; t < 4; ++t)
-    for (int q = 0; q < 4; ++q) y[t][q] = 0.0;
-#pragma unroll 2
-  for (int k = 0; k < ns; ++k) {
-    const double *src = ws + ((int64_t)(s0 + k) * nh + h) * 4096 + threadIdx.x;
-    double v[16];
-    for (int e = 0; e < 16; ++e) v[e] = src[e * 256];
-    for (int t = 0; t < 4; ++t)
-      for (int q = 0; q < 4; ++q) y[t][q] += v[4 * t + q];
+  const int64_t stride = (int64_t)nh * 4096;
+  const double *src = ws + ((int64_t)s0 * nh + h) * 4096 + 1024 * t + threadIdx.x;
+  double y[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int k = 0; k < ns; k += 4) {
+    double v[4][4];
+    for (int j = 0; j < 4; ++j) {
+      const int sl = k + j < ns ? k + j : ns - 1;
+      for (int q = 0; q < 4; ++q) v[j][q] = src[sl * stride + q * 256];
+    }
+    for (int j = 0; j < 4; ++j)
+      if (k + j < ns)
+        for (int q = 0; q < 4; ++q) y[q] += v[j][q];
   }
-  for (int t = 0; t < 4; ++t)
-    panel8_epilogue(t, y[t], b, h, nrows, nrhs, rowexp, xexp, E, B, rho, rho_s, left, out, s_cm);
-  if (bcm && E) panel8_block_colmax(s_cm, b, h, nh, bcm);
+  panel8_epilogue(t, y, b, h, nrows, nrhs, rowexp, xexp, E, B, rho, rho_s, left, out, s_cm);
+  if (bcm && E) {
+    __syncthreads();
+    if (threadIdx.x < 16) {
+      const int c = 16 * t + (int)threadIdx.x;
+      const double m = fmax(fmax(s_cm[c], s_cm[kGroup + c]),
+                            fmax(s_cm[2 * kGroup + c], s_cm[3 * kGroup + c]));
+      bcm[(b * nh + h) * kGroup + c] = m;
+    }
+  }
 }
 
 }  // namespace flux
@@ -689,14 +702,15 @@ extern "C" int flux_panel8_mm(int64_t nrows, const int64_t *colptr, const int32_
   // block column maxima after x8 (the matrix is square: X has nrows rows)
   double *bcm = (colmax && E) ? (double *)((char *)work + np * 12 + nrows * nh * kXRow)
                               : nullptr;
-  FLUX_CHECK(nit * nh < (1ll << 31), "flux_panel8_mm: too many workgroups");
+  FLUX_CHECK(nit * nh < (1ll << 31) && n_split * nh * 4 < (1ll << 31),
+             "flux_panel8_mm: too many workgroups");
   auto kernel = nslices == 5 ? k_panel8_mm<5> : nslices == 4 ? k_panel8_mm<4> : k_panel8_mm<3>;
   hipLaunchKernelGGL(kernel, dim3((unsigned)(nit * nh)), dim3(256), 0, st, nrows, nblk,
                      (int)nh, colptr, cols, qptr, panel8, rowexp, x8, xexp, nrhs, E, B, rho,
                      rho_s, left, out, bcm, items, nsplit, slot0, ws, nit,
                      reinterpret_cast<const uint32_t *>(qmask));
   if (items && n_split > 0)
-    hipLaunchKernelGGL(k_panel8_reduce, dim3((unsigned)(n_split * nh)), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_panel8_reduce, dim3((unsigned)(n_split * nh * 4)), dim3(256), 0, st,
                        split_blocks, nsplit, slot0, ws, (int)nh, nrows, rowexp, xexp, nrhs, E, B,
                        rho, rho_s, left, out, bcm);
   if (bcm)

@@ -332,13 +332,14 @@ def test_plan_covers_blocks(setup):
 @pytest.mark.parametrize("k", [64, 70])
 def test_planned_products_match_unplanned(setup, k):
     """Split blocks (cap of 2 quads, so nearly every block is cut) give the
-    same products as one workgroup per block, and the same bits on a second
-    call."""
+    same products, residuals and residual column maxima as one workgroup per
+    block, and the same bits on a second call."""
     from fluxpy_amd._panel import PanelPlan
     torch, sm, csr, Pb, P8, ct = setup
     n = csr.shape[0]
     rng = np.random.default_rng(50 + k)
     X = torch.tensor(rng.standard_normal((n, k)), device=_dev())
+    E = torch.tensor(rng.standard_normal((n, k)), device=_dev())
     Xb = X.to(torch.float32).to(torch.bfloat16)
     nblk = (n + 63) // 64
     tiny = PanelPlan(Pb.colptr, nblk, cap=2)
@@ -350,12 +351,21 @@ def test_planned_products_match_unplanned(setup, k):
         Pb.mm_bf16(Xb, y32)
         y64 = torch.empty((n, k), dtype=torch.float64, device=_dev())
         P8.mm_f64(X, y64)
+        # residual with column maxima (the split blocks' epilogue)
+        r64 = torch.empty((n, k), dtype=torch.float64, device=_dev())
+        cm = torch.empty(k, dtype=torch.float64, device=_dev())
+        P8.mm_f64(X, r64, E=E, B=X, colmax=cm, xscale=0.12)
         torch.cuda.synchronize()
-        outs[name] = (y32.cpu().numpy(), y64.cpu().numpy())
+        outs[name] = (y32.cpu().numpy(), y64.cpu().numpy(), r64.cpu().numpy(),
+                      cm.cpu().numpy())
     Pb.plan = P8.plan = Pb.__dict__.get("_default_plan", tiny)
-    a32, a64 = outs["none"]
-    b32, b64 = outs["tiny"]
-    c32, c64 = outs["tiny2"]
+    a32, a64, ar, acm = outs["none"]
+    b32, b64, br, bcm = outs["tiny"]
+    c32, c64, cr, ccm = outs["tiny2"]
     assert np.array_equal(b32, c32) and np.array_equal(b64, c64)      # deterministic
+    assert np.array_equal(br, cr) and np.array_equal(bcm, ccm)
     assert np.allclose(b32, a32, rtol=1e-5, atol=1e-5 * abs(a32).max())
     assert np.allclose(b64, a64, rtol=1e-14, atol=1e-14 * abs(a64).max())
+    assert np.allclose(br, ar, rtol=0, atol=1e-13 * abs(ar).max())
+    assert np.array_equal(bcm, abs(br).max(axis=0))
+    assert np.allclose(bcm, acm, rtol=1e-12)
