@@ -212,12 +212,12 @@ __global__ void k_zero_any(T *p, int64_t n) {
 // 4096) and k_panel_reduce_bf16 sums them in slot order (so the result does
 // not depend on scheduling) and runs the epilogue.
 // ---------------------------------------------------------------------------
-// Jacobi-sweep epilogue of one 64-row block and one group of 64 right-hand
-// sides (g0 ..), from the product acc in the MFMA layout: acc[t][q] =
+// Jacobi-sweep epilogue of tile t (right-hand sides g0 + 16 t ..) of one
+// 64-row block, from the product a in the MFMA layout: a[q] =
 // Y[row 16 w + li][rhs g0 + 16 t + 4 lg + q] (w = wave, li = lane % 16,
 // lg = lane / 16).
-__device__ __forceinline__ void bf16_epilogue(
-    const v4f (&acc)[4], int64_t b, int64_t g0, int64_t nrows, int64_t nrhs,
+__device__ __forceinline__ void bf16_epilogue_tile(
+    int t, const v4f &a, int64_t b, int64_t g0, int64_t nrows, int64_t nrhs,
     const float *__restrict__ base, const float *__restrict__ rho, float rho_s, int left,
     float *__restrict__ out, __hip_bfloat16 *__restrict__ xnext, const float *__restrict__ prev,
     double *__restrict__ colmax) {
@@ -228,13 +228,13 @@ __device__ __forceinline__ void bf16_epilogue(
   const int64_t r = row_base + w * 16 + li;
   const bool row_ok = r < nrows;
   const float rr = row_ok ? (rho ? rho[r] : rho_s) : 0.f;
-  for (int t = 0; t < 4; ++t) {
+  {
     const int64_t cb = g0 + t * 16 + lg * 4;
     double cm[4] = {0.0, 0.0, 0.0, 0.0};
     if (row_ok && cb < nrhs) {
       const int64_t o = r * nrhs + cb;
       float v[4];
-      for (int q = 0; q < 4; ++q) v[q] = left ? acc[t][q] * rr : acc[t][q];
+      for (int q = 0; q < 4; ++q) v[q] = left ? a[q] * rr : a[q];
       if (vec_x && cb + 3 < nrhs) {
         if (base) {
           const float4 bv = *reinterpret_cast<const float4 *>(base + o);
@@ -277,6 +277,17 @@ __device__ __forceinline__ void bf16_epilogue(
       }
     }
   }
+}
+
+// The epilogue of one group of 64 right-hand sides (g0 ..): its four tiles.
+__device__ __forceinline__ void bf16_epilogue(
+    const v4f (&acc)[4], int64_t b, int64_t g0, int64_t nrows, int64_t nrhs,
+    const float *__restrict__ base, const float *__restrict__ rho, float rho_s, int left,
+    float *__restrict__ out, __hip_bfloat16 *__restrict__ xnext, const float *__restrict__ prev,
+    double *__restrict__ colmax) {
+  for (int t = 0; t < 4; ++t)
+    bf16_epilogue_tile(t, acc[t], b, g0, nrows, nrhs, base, rho, rho_s, left, out, xnext, prev,
+                       colmax);
 }
 
 // SC chunks of 16 columns per stage.  LDS block per stage (bf16):
@@ -476,30 +487,38 @@ __global__ void __launch_bounds__(256, SC == 4 ? 3 : 2) k_panel_mm_bf16(
 }
 
 // Split blocks of a planned product: the sum of the block's partials, in
-// slot order, and the epilogue (one workgroup per split block).
+// slot order, and the epilogue.  One workgroup per (split block, group of 64
+// right-hand sides, tile t of 16 of them); each thread sums 4 values of every
+// slot, four slots at a time with all loads issued before any add (a slot
+// index past the last is clamped to the last and its add skipped, so no load
+// sits behind a branch).
 __global__ void __launch_bounds__(256) k_panel_reduce_bf16(
     const int32_t *__restrict__ split_blocks, const int32_t *__restrict__ nsplit,
     const int32_t *__restrict__ slot0, const float *__restrict__ ws, int64_t nrows,
     int64_t nrhs, const float *__restrict__ base, const float *__restrict__ rho, float rho_s,
     int left, float *__restrict__ out, __hip_bfloat16 *__restrict__ xnext,
     const float *__restrict__ prev, double *__restrict__ colmax) {
-  const int64_t b = split_blocks[blockIdx.x];
-  const int ns = nsplit[b], s0 = slot0[b];
   const int64_t nh = (nrhs + 63) / 64;
-  for (int64_t g0 = 0; g0 < nrhs; g0 += 64) {
-    v4f acc[4];
-    for (int t = 0; t < 4; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
-    // a slot's 16 values are loaded together; slots are summed in order
-#pragma unroll 2
-    for (int k = 0; k < ns; ++k) {
-      const float *src = ws + ((int64_t)(s0 + k) * nh + g0 / 64) * 4096 + threadIdx.x;
-      float v[16];
-      for (int e = 0; e < 16; ++e) v[e] = src[e * 256];
-      for (int t = 0; t < 4; ++t)
-        for (int q = 0; q < 4; ++q) acc[t][q] += v[4 * t + q];
+  const int t = (int)(blockIdx.x & 3);
+  const int64_t g = blockIdx.x >> 2;
+  const int64_t b = split_blocks[g / nh];
+  const int64_t h = g % nh;
+  const int ns = nsplit[b], s0 = slot0[b];
+  const int64_t stride = nh * 4096;
+  const float *src = ws + ((int64_t)s0 * nh + h) * 4096 + 1024 * t + threadIdx.x;
+  v4f a = v4f{0.f, 0.f, 0.f, 0.f};
+  for (int k = 0; k < ns; k += 4) {
+    float v[4][4];
+    for (int j = 0; j < 4; ++j) {
+      const int sl = k + j < ns ? k + j : ns - 1;
+      for (int q = 0; q < 4; ++q) v[j][q] = src[sl * stride + q * 256];
     }
-    bf16_epilogue(acc, b, g0, nrows, nrhs, base, rho, rho_s, left, out, xnext, prev, colmax);
+    for (int j = 0; j < 4; ++j)
+      if (k + j < ns)
+        for (int q = 0; q < 4; ++q) a[q] += v[j][q];
   }
+  bf16_epilogue_tile(t, a, b, 64 * h, nrows, nrhs, base, rho, rho_s, left, out, xnext, prev,
+                     colmax);
 }
 
 // ---------------------------------------------------------------------------
@@ -896,6 +915,8 @@ extern "C" int flux_panel_mm_bf16(int64_t nrows, const int64_t *colptr, const in
                  (n_items >= nblk && nsplit && slot0 && (n_split == 0 || (split_blocks && ws))),
              "flux_panel_mm_bf16: a plan needs nsplit, slot0, split_blocks and ws");
   const int64_t ngrid = items ? n_items : nblk;
+  FLUX_CHECK(ngrid < (1ll << 31) && n_split * ((nrhs + 63) / 64) * 4 < (1ll << 31),
+             "flux_panel_mm_bf16: too many workgroups");
   hipStream_t st = as_stream(stream);
   if (colmax)
     hipLaunchKernelGGL(k_zero_f64_p, dim3(ceil_div_u(nrhs, 256)), dim3(256), 0, st, colmax,
@@ -919,7 +940,8 @@ extern "C" int flux_panel_mm_bf16(int64_t nrows, const int64_t *colptr, const in
   }
 #undef MMB
   if (items && n_split > 0)
-    hipLaunchKernelGGL(k_panel_reduce_bf16, dim3((unsigned)n_split), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_panel_reduce_bf16, dim3((unsigned)(n_split * ((nrhs + 63) / 64) * 4)),
+                       dim3(256), 0, st,
                        split_blocks, nsplit, slot0, ws, nrows, nrhs, base, rho, rho_s, left, out,
                        (__hip_bfloat16 *)xnext, prev, colmax);
   FLUX_LAUNCH_CHECK();

@@ -344,11 +344,18 @@ def test_planned_products_match_unplanned(setup, k):
     nblk = (n + 63) // 64
     tiny = PanelPlan(Pb.colptr, nblk, cap=2)
     assert tiny.n_slots > 0
-    outs = {}
+    base = torch.tensor(rng.uniform(0, 1, (n, k)), dtype=torch.float32, device=_dev())
+    rho = torch.tensor(rng.uniform(0.1, 0.5, n), dtype=torch.float32, device=_dev())
+    outs, sweeps = {}, {}
     for name, plan in (("none", False), ("tiny", tiny), ("tiny2", tiny)):
         Pb.plan = P8.plan = plan
         y32 = torch.empty((n, k), dtype=torch.float32, device=_dev())
         Pb.mm_bf16(Xb, y32)
+        # a Jacobi sweep: base, albedo on the right, next iterate, change maxima
+        s32 = torch.empty((n, k), dtype=torch.float32, device=_dev())
+        xn = torch.empty((n, k), dtype=torch.bfloat16, device=_dev())
+        scm = torch.empty(k, dtype=torch.float64, device=_dev())
+        Pb.mm_bf16(Xb, s32, base=base, rho=rho, xnext=xn, prev=base, colmax=scm)
         y64 = torch.empty((n, k), dtype=torch.float64, device=_dev())
         P8.mm_f64(X, y64)
         # residual with column maxima (the split blocks' epilogue)
@@ -358,6 +365,7 @@ def test_planned_products_match_unplanned(setup, k):
         torch.cuda.synchronize()
         outs[name] = (y32.cpu().numpy(), y64.cpu().numpy(), r64.cpu().numpy(),
                       cm.cpu().numpy())
+        sweeps[name] = (s32.cpu().numpy(), xn.float().cpu().numpy(), scm.cpu().numpy())
     Pb.plan = P8.plan = Pb.__dict__.get("_default_plan", tiny)
     a32, a64, ar, acm = outs["none"]
     b32, b64, br, bcm = outs["tiny"]
@@ -369,3 +377,11 @@ def test_planned_products_match_unplanned(setup, k):
     assert np.allclose(br, ar, rtol=0, atol=1e-13 * abs(ar).max())
     assert np.array_equal(bcm, abs(br).max(axis=0))
     assert np.allclose(bcm, acm, rtol=1e-12)
+    (sa, xa, ca), (sb, xb, cb), (sc, xc, cc) = sweeps["none"], sweeps["tiny"], sweeps["tiny2"]
+    assert np.array_equal(sb, sc) and np.array_equal(xb, xc) and np.array_equal(cb, cc)
+    assert np.allclose(sb, sa, rtol=1e-5, atol=1e-5 * abs(sa).max())
+    assert np.isfinite(xb).all()
+    assert np.allclose(xb, rho.cpu().numpy()[:, None] * sb, rtol=1e-2, atol=1e-6)
+    basen = base.cpu().numpy()
+    assert np.array_equal(cb, abs(sb.astype(np.float64) - basen).max(axis=0))
+    assert np.allclose(cb, ca, rtol=1e-4)
