@@ -76,8 +76,9 @@ SIGNATURES = {
                                     _vp, _vp, _vp, _c_i64, _vp, _vp]),
     "flux_panel_mm_f64": (_c_int, [_c_i64, _vp, _vp, _vp, _vp, _c_i64, _vp, _vp, _vp, _c_dbl,
                                    _c_int, _vp, _vp, _vp]),
+    "flux_refine_prepare_work_bytes": (_c_i64, [_c_i64]),
     "flux_refine_prepare": (_c_int, [_c_i64, _c_i64, _vp, _vp, ctypes.c_float, _c_int, _vp,
-                                     _vp, _vp]),
+                                     _vp, _vp, _vp, _vp]),
     "flux_refine_accumulate": (_c_int, [_c_i64, _c_i64, _vp, _vp, _vp, _vp, _vp]),
     "flux_scale_to_bf16": (_c_int, [_c_i64, _c_i64, _vp, _vp, ctypes.c_float, _c_int, _vp,
                                     _vp]),

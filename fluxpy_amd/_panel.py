@@ -301,6 +301,16 @@ class PanelRefinedSolver:
         self.Xb = [t.empty((n, k), dtype=t.bfloat16, device=dev) for _ in range(2)]
         self.D64 = t.empty((n, k), dtype=f64, device=dev)
         self.vo = _kernels.VecOps(k)
+        # readback of the residual maxima: one device buffer, a pinned host
+        # copy and an event the host polls (a busy wait wakes sooner than a
+        # blocking copy)
+        self._rb_dev = t.empty(5 * k, dtype=f64, device=dev)
+        self._rb_host = t.empty(5 * k, dtype=f64, pin_memory=True)
+        self._rb_event = t.cuda.Event()
+        # max|E| per column comes out of the first round's prepare pass
+        self.emax = t.empty(k, dtype=f64, device=dev)
+        self._prep_work = t.empty(max(int(_lib.load().flux_refine_prepare_work_bytes(k)), 16),
+                                  dtype=t.uint8, device=dev)
 
     def set_rho(self, rho):
         """Albedo: an n-vector device tensor (panel order) or a scalar."""
@@ -321,9 +331,11 @@ class PanelRefinedSolver:
     # slices keep the error bound of the five-slice product of round 1
     UPDATE_SLICES = 4
 
-    def _round(self, E, src, b_in, b_out, r_out, colmax, sweeps, slot, r_prev=None):
+    def _round(self, E, src, b_in, b_out, r_out, colmax, sweeps, slot, r_prev=None,
+               src_max=None):
         """One refinement round: `sweeps` inner sweeps on src, b_out = b_in
-        + D, and the residual of b_out into r_out with its column maxima.
+        + D, and the residual of b_out into r_out with its column maxima
+        (src_max, if given, receives max|src| per column).
         With r_prev (the residual of b_in) and int8 panels the residual is
         the update r_prev + F rho D - D on UPDATE_SLICES slices; otherwise
         E + F rho b_out - b_out in full."""
@@ -332,7 +344,8 @@ class PanelRefinedSolver:
         # D0 = fp32(src); X0 = bf16(rho D0) (right) or bf16(D0) (left)
         _lib.call("flux_refine_prepare", n, k, src.data_ptr(), _lib.ptr(self.rho32),
                   float(self.rs), int(not self.left), self.R32.data_ptr(),
-                  self.Xb[0].data_ptr(), st)
+                  self.Xb[0].data_ptr(), _lib.ptr(src_max),
+                  self._prep_work.data_ptr() if src_max is not None else None, st)
         cur = 0
         for _ in range(sweeps):
             self.P_lo.mm_bf16(self.Xb[cur], self.D[cur], base=self.R32, rho=self.rho32,
@@ -364,6 +377,19 @@ class PanelRefinedSolver:
             self.P_hi.mm_f64(Xr, r_out, E=E, B=B, rho=self.rho64, rho_s=self.rs,
                              left=self.left, colmax=colmax)
 
+    def _readback(self, parts):
+        """The concatenated device vectors ``parts`` as a host array."""
+        t = _lib.torch()
+        m = sum(int(p.numel()) for p in parts)
+        dev, host = self._rb_dev[:m], self._rb_host[:m]
+        t.cat(parts, out=dev)
+        host.copy_(dev, non_blocking=True)
+        ev = self._rb_event
+        ev.record()
+        while not ev.query():
+            pass
+        return host.numpy().copy()
+
     def _update_ok(self, xmax_b, xmax_d):
         """True if the update X is at most 2^-7 of the first round's X in
         every column (then UPDATE_SLICES slices keep round 1's bound)."""
@@ -386,10 +412,11 @@ class PanelRefinedSolver:
         sweeps = lambda r: sched[min(r, len(sched) - 1)]
         sliced = isinstance(self.P_hi, SlicedPanelMatrix)
         Bs, Rs, cms = self.Bs, self.Rs, self.cms
-        emax_d = self.vo.maxabs(E)
+        emax_d = self.emax
         # rounds 1 and 2 without waiting in between (round 1 starts from B = 0,
-        # whose residual is E; round 2 updates round 1's residual)
-        self._round(E, E, None, Bs[0], Rs[0], cms[0], sweeps(0), 0)
+        # whose residual is E, and also yields max|E|; round 2 updates round
+        # 1's residual)
+        self._round(E, E, None, Bs[0], Rs[0], cms[0], sweeps(0), 0, src_max=emax_d)
         spec = max_rounds >= 2
         upd = False
         if spec:
@@ -398,7 +425,7 @@ class PanelRefinedSolver:
         parts = [cms[0], cms[1] if spec else cms[0], emax_d.reshape(-1)]
         if sliced:
             parts += [self.P_hi.xmax(k, 0), self.P_hi.xmax(k, 1)]
-        got = t.cat(parts).cpu().numpy()
+        got = self._readback(parts)
         rmaxes, emax = [got[:k], got[k:2 * k]], got[2 * k:3 * k]
         xmax_b = got[3 * k:4 * k] if sliced else None
         fallbacks = 0

@@ -730,6 +730,71 @@ __global__ void k_refine_prepare(int64_t n, int64_t nrhs, const double *__restri
   }
 }
 
+// The same with the column maxima of |r|: 64 columns by 4 rows of threads
+// per workgroup, rows strided over the grid (four rows in flight per
+// thread); the workgroup's maxima go to pmax[blockIdx.x][64 blockIdx.y + tx]
+// and k_colmax_partials reduces them.
+constexpr int64_t kPrepGrid = 512;
+
+__global__ void __launch_bounds__(256) k_refine_prepare_max(
+    int64_t n, int64_t nrhs, int64_t np, const double *__restrict__ r,
+    const float *__restrict__ rho, float rho_s, int scale, float *__restrict__ r32,
+    __hip_bfloat16 *__restrict__ xb, double *__restrict__ pmax) {
+  __shared__ double red[4][64];
+  const int64_t k = (int64_t)blockIdx.y * 64 + threadIdx.x;
+  const int64_t step = (int64_t)gridDim.x * 4;
+  double m[4] = {0.0, 0.0, 0.0, 0.0};
+  if (k < nrhs) {
+    int64_t i = (int64_t)blockIdx.x * 4 + threadIdx.y;
+    for (; i + 3 * step < n; i += 4 * step) {
+      double v[4];
+      for (int u = 0; u < 4; ++u) v[u] = r[(i + u * step) * nrhs + k];
+      for (int u = 0; u < 4; ++u) {
+        const int64_t row = i + u * step, e = row * nrhs + k;
+        const float f = (float)v[u];
+        const float sc = scale ? (rho ? rho[row] : rho_s) : 1.0f;
+        r32[e] = f;
+        xb[e] = __float2bfloat16(sc * f);
+        m[u] = fmax(m[u], fabs(v[u]));
+      }
+    }
+    for (; i < n; i += step) {
+      const int64_t e = i * nrhs + k;
+      const double v = r[e];
+      const float f = (float)v;
+      const float sc = scale ? (rho ? rho[i] : rho_s) : 1.0f;
+      r32[e] = f;
+      xb[e] = __float2bfloat16(sc * f);
+      m[0] = fmax(m[0], fabs(v));
+    }
+  }
+  red[threadIdx.y][threadIdx.x] = fmax(fmax(m[0], m[1]), fmax(m[2], m[3]));
+  __syncthreads();
+  if (threadIdx.y == 0 && k < np) {
+    const double v = fmax(fmax(red[0][threadIdx.x], red[1][threadIdx.x]),
+                          fmax(red[2][threadIdx.x], red[3][threadIdx.x]));
+    pmax[(int64_t)blockIdx.x * np + k] = v;
+  }
+}
+
+// cmax[k] = max over the nb rows of pmax[.][k], k < nrhs (one workgroup per
+// column).
+__global__ void __launch_bounds__(256) k_colmax_partials(const double *__restrict__ pmax,
+                                                         int64_t nb, int64_t np,
+                                                         double *__restrict__ cmax) {
+  __shared__ double red[256];
+  const int64_t k = blockIdx.x;
+  double m = 0.0;
+  for (int64_t b = threadIdx.x; b < nb; b += 256) m = fmax(m, pmax[b * np + k]);
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int sh = 128; sh > 0; sh >>= 1) {
+    if ((int)threadIdx.x < sh) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + sh]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) cmax[k] = red[0];
+}
+
 // End of the inner sweeps: b_out = b_in + d (b_in NULL: b_out = d), FP32 d
 // into FP64 (b_in may equal b_out); d64 (optional) = d in FP64.
 __global__ void k_refine_accumulate(int64_t total, const float *__restrict__ d,
@@ -972,14 +1037,37 @@ extern "C" int flux_panel_mm_f64(int64_t nrows, const int64_t *colptr, const int
   return 0;
 }
 
+extern "C" int64_t flux_refine_prepare_work_bytes(int64_t nrhs) {
+  return nrhs > 0 ? kPrepGrid * ((nrhs + 63) / 64) * 64 * 8 : 0;
+}
+
 extern "C" int flux_refine_prepare(int64_t n, int64_t nrhs, const double *r, const float *rho,
-                                   float rho_s, int scale, float *r32, void *xb, void *stream) {
+                                   float rho_s, int scale, float *r32, void *xb, double *colmax,
+                                   void *work, void *stream) {
   FLUX_CHECK(n >= 0 && nrhs >= 0, "flux_refine_prepare: bad sizes");
-  if (n * nrhs == 0) return 0;
+  FLUX_CHECK(colmax == nullptr || work != nullptr, "flux_refine_prepare: colmax needs work");
+  hipStream_t st = as_stream(stream);
+  if (n * nrhs == 0) {
+    if (colmax && nrhs > 0)
+      hipLaunchKernelGGL(k_zero_f64_p, dim3(ceil_div_u(nrhs, 256)), dim3(256), 0, st, colmax,
+                         nrhs);
+    FLUX_LAUNCH_CHECK();
+    return 0;
+  }
   FLUX_CHECK(r && r32 && xb, "flux_refine_prepare: NULL array");
-  hipLaunchKernelGGL(k_refine_prepare, dim3(ceil_div_u(n * nrhs, 1024)), dim3(256), 0,
-                     as_stream(stream), n, nrhs, r, rho, rho_s, scale, r32,
-                     (__hip_bfloat16 *)xb);
+  if (colmax) {
+    const int64_t nh = (nrhs + 63) / 64, np = nh * 64;
+    const int64_t gx = (n + 3) / 4 < kPrepGrid ? (n + 3) / 4 : kPrepGrid;
+    FLUX_CHECK(nh < 65536, "flux_refine_prepare: too many right-hand sides");
+    double *pmax = (double *)work;
+    hipLaunchKernelGGL(k_refine_prepare_max, dim3((unsigned)gx, (unsigned)nh), dim3(64, 4), 0,
+                       st, n, nrhs, np, r, rho, rho_s, scale, r32, (__hip_bfloat16 *)xb, pmax);
+    hipLaunchKernelGGL(k_colmax_partials, dim3((unsigned)nrhs), dim3(256), 0, st, pmax, gx, np,
+                       colmax);
+  } else {
+    hipLaunchKernelGGL(k_refine_prepare, dim3(ceil_div_u(n * nrhs, 1024)), dim3(256), 0, st, n,
+                       nrhs, r, rho, rho_s, scale, r32, (__hip_bfloat16 *)xb);
+  }
   FLUX_LAUNCH_CHECK();
   return 0;
 }

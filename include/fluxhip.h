@@ -361,11 +361,17 @@ int flux_panel_mm_f64(int64_t nrows, const int64_t *colptr, const int32_t *cols,
 /* Refinement round steps (batched solve on the panels):
  *   prepare:    r32 = fp32(r), xb = bf16(s o r32)  (s_i = rho_i or rho_s if
  *               scale, else 1): the FP32 correction right-hand side and the
- *               first BF16 iterate from the FP64 residual r;
+ *               first BF16 iterate from the FP64 residual r; with colmax
+ *               (NULL: skipped) also colmax[c] = max_i |r[i, c]|, which needs
+ *               `work` of flux_refine_prepare_work_bytes(nrhs) bytes (the
+ *               first round's r is E, so this is the max|E| of the stopping
+ *               rule);
  *   accumulate: b_out = b_in + d (b_in NULL: b_out = d), FP32 d into FP64
  *               (b_in may equal b_out); d64, if not NULL, receives d in FP64. */
+int64_t flux_refine_prepare_work_bytes(int64_t nrhs);
 int flux_refine_prepare(int64_t n, int64_t nrhs, const double *r, const float *rho,
-                        float rho_s, int scale, float *r32, void *xb, void *stream);
+                        float rho_s, int scale, float *r32, void *xb, double *colmax,
+                        void *work, void *stream);
 int flux_refine_accumulate(int64_t n, int64_t nrhs, const float *d, const double *b_in,
                            double *b_out, double *d64, void *stream);
 

@@ -112,3 +112,37 @@ def test_refined_panel_solve(setup, oracle_mod, left):
     err = abs(B.cpu().numpy() - Bd).max(0) / abs(Bd).max(0)
     assert err.max() <= 1e-6, err.max()
     assert info["rounds"] <= 6
+
+
+@pytest.mark.parametrize("n,k,scale", [(1000, 64, 1), (5003, 70, 1), (37, 5, 0), (0, 8, 1)])
+def test_refine_prepare_with_column_maxima(n, k, scale):
+    """flux_refine_prepare: r32 = fp32(r), xb = bf16(s r32) and, with colmax,
+    max|r| per column; the same r32 and xb as the pass without maxima."""
+    import torch
+    from fluxpy_amd import _lib
+    print(f"case: refine_prepare n={n} k={k} scale={scale}", flush=True)
+    rng = np.random.default_rng(n + k)
+    r = torch.tensor(rng.standard_normal((n, k)) * rng.uniform(0.1, 100.0, k),
+                     device=_dev())
+    rho = torch.tensor(rng.uniform(0.1, 0.5, n), dtype=torch.float32, device=_dev())
+    st = _lib.stream()
+    outs = []
+    for with_max in (False, True):
+        r32 = torch.full((n, k), -1.0, dtype=torch.float32, device=_dev())
+        xb = torch.zeros((n, k), dtype=torch.bfloat16, device=_dev())
+        cm = torch.full((k,), -7.0, dtype=torch.float64, device=_dev())
+        work = torch.empty(max(int(_lib.load().flux_refine_prepare_work_bytes(k)), 16),
+                           dtype=torch.uint8, device=_dev())
+        _lib.call("flux_refine_prepare", n, k, r.data_ptr(), rho.data_ptr(), 1.0, scale,
+                  r32.data_ptr(), xb.data_ptr(), cm.data_ptr() if with_max else None,
+                  work.data_ptr() if with_max else None, st)
+        torch.cuda.synchronize()
+        outs.append((r32.cpu(), xb.float().cpu(), cm.cpu()))
+    (a32, axb, _), (b32, bxb, bcm) = outs
+    ref32 = r.float().cpu()
+    s = rho.cpu()[:, None] if scale else 1.0
+    assert torch.equal(a32, ref32) and torch.equal(b32, ref32)
+    assert torch.equal(axb, (s * ref32).to(torch.bfloat16).float())
+    assert torch.equal(bxb, axb)
+    want = r.abs().amax(dim=0).cpu() if n else torch.zeros(k, dtype=torch.float64)
+    assert torch.equal(bcm, want)
