@@ -387,8 +387,11 @@ __global__ void __launch_bounds__(256, SC == 4 ? 3 : 2) k_panel_mm_bf16(
         const int64_t ch = s * SC + c;
         const bool ok = ch < nch;
         // B fragment: row 16 w + li, columns 4 lg .. 4 lg + 3 of chunk ch
-        const uint2 v =
-            P2[((c0 + (ok ? ch : 0) * 16) * kPanelRows + (w * 16 + li) * 16 + lg * 4) / 4];
+        // a streaming load: the panels are read once, the X rows many times
+        const unsigned long long raw = __builtin_nontemporal_load(
+            reinterpret_cast<const unsigned long long *>(P2) +
+            ((c0 + (ok ? ch : 0) * 16) * kPanelRows + (w * 16 + li) * 16 + lg * 4) / 4);
+        const uint2 v = make_uint2((unsigned)raw, (unsigned)(raw >> 32));
         if constexpr (VEC)
           pf[c] = v;
         else

@@ -449,7 +449,8 @@ __global__ void __launch_bounds__(256, 2) k_panel8_mm(
       // a zero subtile of this wave reads the zero quad (no branch)
       const bool nz = (qms[s & 3] >> (8 * w)) & 0xffu;
       const v4i *p = nz ? pb + min(s, nq - 1) * (kQuadBytes / 16) : pz;
-      for (int i = 0; i < S; ++i) pf[i] = p[i * (kSliceTile / 16)];
+      // streaming loads: the panels are read once, the X slices many times
+      for (int i = 0; i < S; ++i) pf[i] = __builtin_nontemporal_load(p + i * (kSliceTile / 16));
     };
     v4i xr[S];
     v4i p0[S], p1[S];
