@@ -482,7 +482,7 @@ __global__ void __launch_bounds__(256, SC == 4 ? 3 : 2) k_panel_mm_bf16(
       // partial of a split block: to its slot (k_panel_reduce_bf16 finishes)
       float *slot = ws + ((int64_t)(itm.slot0 + itm.piece) * nh + g0 / 64) * 4096 + tid;
       for (int t = 0; t < 4; ++t)
-        for (int q = 0; q < 4; ++q) slot[(4 * t + q) * 256] = acc[t][q];
+        for (int q = 0; q < 4; ++q) __builtin_nontemporal_store(acc[t][q], slot + (4 * t + q) * 256);
       continue;
     }
     bf16_epilogue(acc, b, g0, nrows, nrhs, base, rho, rho_s, left, out, xnext, prev, colmax);

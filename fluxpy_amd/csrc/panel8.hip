@@ -512,7 +512,7 @@ __global__ void __launch_bounds__(256, 2) k_panel8_mm(
     // partial of a split block: to its slot (k_panel8_reduce finishes)
     double *slot = ws + ((int64_t)(itm.slot0 + itm.piece) * nh + h) * 4096 + tid;
     for (int t = 0; t < 4; ++t)
-      for (int q = 0; q < 4; ++q) slot[(4 * t + q) * 256] = combine(t, q);
+      for (int q = 0; q < 4; ++q) __builtin_nontemporal_store(combine(t, q), slot + (4 * t + q) * 256);
     return;
   }
   __shared__ double s_cm[4 * kGroup];
