@@ -39,9 +39,9 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE x 2, WRITE_SIZE x 1); the numbers and the commands are in
 # profiles/panel_kernels_fractal41k_64suns.md.  Kernels not measured: None.
 MEASURED_TRAFFIC = {
-    "k_panel8_mm<5> + k_panel8_reduce + k_colmax_blocks": 1787339776,
-    "k_panel8_mm<4> + k_panel8_reduce + k_colmax_blocks": 1519930368,
-    "k_panel_mm_bf16 + k_panel_reduce_bf16": 749777920,
+    "k_panel8_mm<5> + k_panel8_reduce + k_colmax_blocks": 1501635584,
+    "k_panel8_mm<4> + k_panel8_reduce + k_colmax_blocks": 1271291904,
+    "k_panel_mm_bf16 + k_panel_reduce_bf16": 632678400,
 }
 
 

@@ -605,8 +605,9 @@ __global__ void __launch_bounds__(256, 3) k_panel_mm_f64(
       for (int c = 0; c < SC; ++c) {
         const int64_t ch = s * SC + c;
         const bool ok = ch < nch;
-        const v4f v =
-            P4[((c0 + (ok ? ch : 0) * 16) * kPanelRows + (w * 16 + li) * 16 + lg * 4) / 4];
+        // a streaming load: the panels are read once, the X rows many times
+        const v4f v = __builtin_nontemporal_load(
+            P4 + ((c0 + (ok ? ch : 0) * 16) * kPanelRows + (w * 16 + li) * 16 + lg * 4) / 4);
         if constexpr (VEC)
           pf[c] = v;
         else
