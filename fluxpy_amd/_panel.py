@@ -347,9 +347,11 @@ class PanelRefinedSolver:
                   self.Xb[0].data_ptr(), _lib.ptr(src_max),
                   self._prep_work.data_ptr() if src_max is not None else None, st)
         cur = 0
-        for _ in range(sweeps):
+        for it in range(sweeps):
+            # the last sweep's next iterate would go unread
+            xn = self.Xb[1 - cur] if it + 1 < sweeps else None
             self.P_lo.mm_bf16(self.Xb[cur], self.D[cur], base=self.R32, rho=self.rho32,
-                              rho_s=self.rs, left=self.left, xnext=self.Xb[1 - cur])
+                              rho_s=self.rs, left=self.left, xnext=xn)
             cur = 1 - cur
         D = self.D[1 - cur]
         update = r_prev is not None and isinstance(self.P_hi, SlicedPanelMatrix)
