@@ -74,8 +74,8 @@ def main():
                          "default, so every leaf is dense / CSR and the operator is exact")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU sharding is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: the solve runs on one GPU; --gpus {args.gpus} is not supported "
+              "(sharding the row blocks of F over GPUs is not implemented)", file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -107,7 +107,7 @@ class _SvdBatch:
     def __init__(self, F, FT, rngs, ls, seed):
         t = _lib.require_gpu()
         if F.data.dtype == t.float64 and not FP64_SKETCH:
-            raise _lib.FluxError("FP64 randomized sketches are switched off in this build "
+            raise _lib.FluxError("FP64 randomized sketches are switched off "
                                  "(see FP64_SKETCH in fluxpy_amd/_hbuild.py)")
         dev = _lib.device()
         st = _lib.stream()
