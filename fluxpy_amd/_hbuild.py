@@ -31,17 +31,72 @@ K0 = 40               # linalg.py:20 (k0 = 40)
 L_MAX = 1024          # largest sketch the batched eigensolver accepts
 EIG_CUT = 1e-14       # drop sketch directions with D_i <= EIG_CUT * D_0
 SKETCH_BYTES = 2 << 30
-# FP64 randomized sketches are switched off.  An FP64 build with low-rank
-# leaves (1,280-facet stretched icosphere, tol 1e-4, min_size 1024) ended in
-# a device memory fault, and the same kernels built for the host under
-# AddressSanitizer (also with FMA contraction, and with every allocation
-# filled with garbage) ran that build with no out-of-bounds access.  Until
-# the cause is shown, FP64 models get dense / CSR leaves only (exact
-# products, more bytes) and an FP64 sketch raises.
-FP64_SKETCH = False
 # False: no block is offered to the sketch, so every leaf is dense / CSR /
 # zero (the product is exact; nbytes is that of the near field).
 LOWRANK = True
+
+
+# --------------------------------------------------------------------------
+# host-side bounds checks of the batched launches
+#
+# Every work buffer of the sketch pipeline is addressed through per-block
+# offsets and GEMM descriptors built here on the host.  Before a launch the
+# largest element each descriptor can touch is checked against the size of
+# the buffer it addresses, so a wrong descriptor raises instead of reaching
+# the GPU.  The device arrays handed to a launch are held in local names
+# until the call returns: a tensor created inside an argument list is freed
+# as soon as its data_ptr() is taken, and the caching allocator may give the
+# same block to the next temporary of that list, so several pointers of one
+# launch would alias one array.
+# --------------------------------------------------------------------------
+
+def _check_extent(what, off, extent, size):
+    """Every block b touches [off[b], off[b] + extent[b]) of a buffer of
+    `size` elements."""
+    off = np.asarray(off, dtype=np.int64)
+    extent = np.asarray(extent, dtype=np.int64)
+    if off.size and ((off < 0).any() or (extent < 0).any() or (off + extent > size).any()):
+        b = int(np.argmax((off < 0) | (extent < 0) | (off + extent > size)))
+        raise _lib.FluxError(f"{what}: block {b} addresses [{off[b]}, {off[b] + extent[b]}) "
+                             f"of a buffer of {size} elements")
+
+
+def _gemm_extent(rows, cols, ld, trans):
+    """Elements spanned by a row-major rows x cols operand (stored transposed
+    when trans) with leading dimension ld."""
+    rows, cols, ld = (np.asarray(a, dtype=np.int64) for a in (rows, cols, ld))
+    trans = np.broadcast_to(np.asarray(trans, dtype=bool), rows.shape)
+    r = np.where(trans, cols, rows)      # stored rows
+    c = np.where(trans, rows, cols)      # stored row length
+    ext = np.where((r > 0) & (c > 0), (r - 1) * ld + c, 0)
+    if ((c > ld) & (r > 1)).any():
+        raise _lib.FluxError("batched GEMM: leading dimension below the row length")
+    return ext
+
+
+def _check_gemm(desc, sizeA, sizeB, sizeC):
+    d = np.asarray(desc, dtype=np.int64).reshape(-1, 12)
+    M, N, K = d[:, 0], d[:, 1], d[:, 2]
+    if (d[:, :3] < 0).any():
+        raise _lib.FluxError("batched GEMM: negative dimension")
+    _check_extent("batched GEMM operand A", d[:, 3], _gemm_extent(M, K, d[:, 4], d[:, 5] != 0),
+                  sizeA)
+    _check_extent("batched GEMM operand B", d[:, 6], _gemm_extent(K, N, d[:, 7], d[:, 8] != 0),
+                  sizeB)
+    _check_extent("batched GEMM result C", d[:, 9], _gemm_extent(M, N, d[:, 10], 0), sizeC)
+
+
+def _gemm(desc, A, B, C, st):
+    """flux_batched_gemm after the host-side bounds check."""
+    desc = np.ascontiguousarray(desc, dtype=np.int64).reshape(-1, 12)
+    if desc.shape[0] == 0:
+        return
+    _check_gemm(desc, A.numel(), B.numel(), C.numel())
+    tiles = int((((desc[:, 0] + 15) // 16) * ((desc[:, 1] + 15) // 16)).max())
+    d_desc = _i64(desc)
+    _lib.call("flux_batched_gemm", desc.shape[0], d_desc.data_ptr(), tiles, A.data_ptr(),
+              B.data_ptr(), C.data_ptr(), st)
+    del d_desc
 
 
 class _Node:
@@ -102,20 +157,28 @@ def _stats(F, FT, nodes):
 
 class _SvdBatch:
     """Sketch + Gram + eigensolver for a batch of blocks; keeps the
-    intermediates so accepted blocks can be factorised afterwards."""
+    intermediates so accepted blocks can be factorised afterwards.
+
+    Every launch is preceded by a host-side check that the offsets and
+    descriptors stay inside the buffers they address (``_check_extent``,
+    ``_gemm``), and the one data-dependent index of the pipeline (the column
+    ids read by flux_block_spmm) is range-checked in the kernel, which
+    reports through ``status`` instead of reading outside its block."""
 
     def __init__(self, F, FT, rngs, ls, seed):
         t = _lib.require_gpu()
-        if F.data.dtype == t.float64 and not FP64_SKETCH:
-            raise _lib.FluxError("FP64 randomized sketches are switched off "
-                                 "(see FP64_SKETCH in fluxpy_amd/_hbuild.py)")
         dev = _lib.device()
         st = _lib.stream()
         self.rngs = np.array(rngs, dtype=np.int64).reshape(-1, 4)
         self.ls = np.array(ls, dtype=np.int64)
         nb = len(self.ls)
-        m = self.rngs[:, 1] - self.rngs[:, 0]
-        n = self.rngs[:, 3] - self.rngs[:, 2]
+        r0, r1, c0, c1 = self.rngs.T
+        m, n = r1 - r0, c1 - c0
+        nrow, ncol = F.shape
+        if ((r0 < 0) | (r1 > nrow) | (c0 < 0) | (c1 > ncol) | (m < 1) | (n < 1)).any():
+            raise _lib.FluxError("sketch: block range outside the matrix")
+        if ((self.ls < 1) | (self.ls > np.minimum(np.minimum(m, n), L_MAX))).any():
+            raise _lib.FluxError("sketch: sketch width outside [1, min(m, n, L_MAX)]")
         self.m, self.n, self.nb = m, n, nb
         l = self.ls
         offY = np.concatenate([[0], np.cumsum(m * l)])
@@ -132,23 +195,32 @@ class _SvdBatch:
         d = t.empty(max(int(offE[-1]), 1), dtype=f64, device=dev)
         self.Tm = t.empty_like(G)
         kept = t.empty(nb, dtype=t.int64, device=dev)
+        status = t.zeros(1, dtype=t.int32, device=dev)
+        for what, off, ext, buf in (("sketch Y", offY[:-1], m * l, self.Y),
+                                    ("sketch Z", offZ[:-1], n * l, self.Z),
+                                    ("Gram", offG[:-1], l * l, G),
+                                    ("eigenvalues", offE[:-1], l, d)):
+            _check_extent(what, off, ext, buf.numel())
         rowstart = np.concatenate([[0], np.cumsum(m)])[:-1]
         rowstartT = np.concatenate([[0], np.cumsum(n)])[:-1]
         d_blk, d_blkT = _i64(self.rngs), _i64(self.rngs[:, [2, 3, 0, 1]])
         d_ls = _i64(l)
-        d_offY, d_offZ, d_offG, d_offE = _i64(offY[:-1]), _i64(offZ[:-1]), _i64(offG[:-1]), \
-            _i64(offE[:-1])
-        code = _lib.dtype_code(F.data.dtype)
+        d_offY, d_offZ, d_offG, d_offE = (_i64(offY[:-1]), _i64(offZ[:-1]), _i64(offG[:-1]),
+                                          _i64(offE[:-1]))
+        d_m, d_n = _i64(m), _i64(n)
         d_rs, d_rsT = _i64(rowstart), _i64(rowstartT)
+        code = _lib.dtype_code(F.data.dtype)
+        # Y = A W (W Rademacher), Z = A^T Y
         _lib.call("flux_block_spmm", F.indptr.data_ptr(), F.indices.data_ptr(), F.data.data_ptr(),
                   code, nb, d_blk.data_ptr(), d_ls.data_ptr(), d_rs.data_ptr(),
-                  int(m.sum()), None, None, self.Y.data_ptr(), d_offY.data_ptr(), seed, st)
+                  int(m.sum()), None, None, self.Y.data_ptr(), d_offY.data_ptr(), seed,
+                  status.data_ptr(), st)
         _lib.call("flux_block_spmm", FT.indptr.data_ptr(), FT.indices.data_ptr(),
                   FT.data.data_ptr(), code, nb, d_blkT.data_ptr(), d_ls.data_ptr(),
                   d_rsT.data_ptr(), int(n.sum()), self.Y.data_ptr(),
-                  d_offY.data_ptr(), self.Z.data_ptr(), d_offZ.data_ptr(), seed, st)
+                  d_offY.data_ptr(), self.Z.data_ptr(), d_offZ.data_ptr(), seed,
+                  status.data_ptr(), st)
         lmax = int(l.max())
-        d_m, d_n = _i64(m), _i64(n)
         _lib.call("flux_batched_gram", nb, d_m.data_ptr(), d_ls.data_ptr(), lmax,
                   self.Y.data_ptr(), d_offY.data_ptr(), G.data_ptr(), d_offG.data_ptr(), st)
         _lib.call("flux_batched_syevj", nb, d_ls.data_ptr(), lmax, G.data_ptr(),
@@ -159,26 +231,26 @@ class _SvdBatch:
                   kept.data_ptr(), st)
         _lib.call("flux_batched_gram", nb, d_n.data_ptr(), d_ls.data_ptr(), lmax,
                   self.Z.data_ptr(), d_offZ.data_ptr(), G.data_ptr(), d_offG.data_ptr(), st)
-        # K = Tm^T (Z^T Z) Tm
+        # K = Tm^T (Z^T Z) Tm, through H1 = (Z^T Z) Tm
         H1 = Vw
-        desc = np.array([[li, li, li, oG, li, 0, oG, li, 0, oG, li, 0]
-                         for li, oG in zip(l, offG[:-1])], dtype=np.int64)
-        tiles = int(((l + 15) // 16).max() ** 2)
-        d_desc1 = _i64(desc)
-        _lib.call("flux_batched_gemm", nb, d_desc1.data_ptr(), tiles, G.data_ptr(),
-                  self.Tm.data_ptr(), H1.data_ptr(), st)
-        desc[:, 5] = 1
-        d_desc2 = _i64(desc)
-        _lib.call("flux_batched_gemm", nb, d_desc2.data_ptr(), tiles, self.Tm.data_ptr(),
-                  H1.data_ptr(), G.data_ptr(), st)
+        og = offG[:-1]
+        zeros = np.zeros(nb, dtype=np.int64)
+        _gemm(np.stack([l, l, l, og, l, zeros, og, l, zeros, og, l, zeros], 1),
+              G, self.Tm, H1, st)
+        _gemm(np.stack([l, l, l, og, l, zeros + 1, og, l, zeros, og, l, zeros], 1),
+              self.Tm, H1, G, st)
         self.Xe = Vm
         self.lam = d
         _lib.call("flux_batched_syevj", nb, d_ls.data_ptr(), lmax, G.data_ptr(),
                   d_offG.data_ptr(), Vw.data_ptr(), self.Xe.data_ptr(), self.lam.data_ptr(),
                   d_offE.data_ptr(), 30, st)
         lam = self.lam.cpu().numpy()
-        self.sigma = [np.sqrt(np.maximum(lam[offE[b]:offE[b + 1]], 0.0)) for b in range(nb)]
         self.kept = kept.cpu().numpy()
+        if int(status.cpu()[0]) != 0:
+            raise _lib.FluxError("sketch: a CSR row holds column ids outside its block "
+                                 "(rows must be sorted)")
+        self.sigma = [np.sqrt(np.maximum(lam[offE[b]:offE[b + 1]], 0.0)) for b in range(nb)]
+        del d_blk, d_blkT, d_ls, d_offY, d_offZ, d_offG, d_offE, d_m, d_n, d_rs, d_rsT
 
     def factors(self, picks, out_dtype):
         """U (m x r), S (r), Vt (r x n) for (batch index, rank) pairs, as
@@ -191,7 +263,11 @@ class _SvdBatch:
             return res
         bs = np.array([p[0] for p in picks], dtype=np.int64)
         rs = np.array([p[1] for p in picks], dtype=np.int64)
+        if ((bs < 0) | (bs >= self.nb)).any():
+            raise _lib.FluxError("sketch factors: batch index out of range")
         l = self.ls[bs]
+        if ((rs < 1) | (rs > l)).any():
+            raise _lib.FluxError("sketch factors: rank outside [1, sketch width]")
         m, n = self.m[bs], self.n[bs]
         np_ = len(picks)
         offC = np.concatenate([[0], np.cumsum(l * rs)])
@@ -203,34 +279,24 @@ class _SvdBatch:
         Vt = t.empty(max(int(offV[-1]), 1), dtype=t.float64, device=dev)
         oG = self.offG[bs]
         oE = self.offE[bs]
-        # C = Tm X[:, :r]
-        desc = np.array([[li, r, li, g, li, 0, g, li, 0, c, r, 0]
-                         for li, r, g, c in zip(l, rs, oG, offC[:-1])], dtype=np.int64)
-        tiles = int((((l + 15) // 16) * ((rs + 15) // 16)).max())
-        d_desc1 = _i64(desc)
-        _lib.call("flux_batched_gemm", np_, d_desc1.data_ptr(), tiles, self.Tm.data_ptr(),
-                  self.Xe.data_ptr(), C.data_ptr(), st)
+        zeros = np.zeros(np_, dtype=np.int64)
+        # C = Tm Xe[:, :r]  (l x r)
+        _gemm(np.stack([l, rs, l, oG, l, zeros, oG, l, zeros, offC[:-1], rs, zeros], 1),
+              self.Tm, self.Xe, C, st)
         C2.copy_(C)
+        # C2[:, j] /= sqrt(lam_j)
         dims = np.stack([l, rs], 1).astype(np.int64)
+        _check_extent("scaled basis", offC[:-1], l * rs, C2.numel())
+        _check_extent("eigenvalues", oE, rs, self.lam.numel())
         d_dims, d_offC, d_oE = _i64(dims), _i64(offC[:-1]), _i64(oE)
         _lib.call("flux_scale_cols_inv_sqrt", np_, d_dims.data_ptr(), C2.data_ptr(),
                   d_offC.data_ptr(), self.lam.data_ptr(), d_oE.data_ptr(), st)
         # U = Y C  (m x r)
-        desc = np.array([[mi, r, li, y, li, 0, c, r, 0, u, r, 0]
-                         for mi, r, li, y, c, u in zip(m, rs, l, self.offY[bs], offC[:-1],
-                                                       offU[:-1])], dtype=np.int64)
-        tiles = int((((m + 15) // 16) * ((rs + 15) // 16)).max())
-        d_desc2 = _i64(desc)
-        _lib.call("flux_batched_gemm", np_, d_desc2.data_ptr(), tiles, self.Y.data_ptr(),
-                  C.data_ptr(), U.data_ptr(), st)
+        _gemm(np.stack([m, rs, l, self.offY[bs], l, zeros, offC[:-1], rs, zeros, offU[:-1], rs,
+                        zeros], 1), self.Y, C, U, st)
         # Vt = C2^T Z^T  (r x n)
-        desc = np.array([[r, ni, li, c, r, 1, z, li, 1, v, ni, 0]
-                         for r, ni, li, c, z, v in zip(rs, n, l, offC[:-1], self.offZ[bs],
-                                                       offV[:-1])], dtype=np.int64)
-        tiles = int((((rs + 15) // 16) * ((n + 15) // 16)).max())
-        d_desc3 = _i64(desc)
-        _lib.call("flux_batched_gemm", np_, d_desc3.data_ptr(), tiles, C2.data_ptr(),
-                  self.Z.data_ptr(), Vt.data_ptr(), st)
+        _gemm(np.stack([rs, n, l, offC[:-1], rs, zeros + 1, self.offZ[bs], l, zeros + 1,
+                        offV[:-1], n, zeros], 1), C2, self.Z, Vt, st)
         lam = self.lam
         for q in range(np_):
             b, r = int(bs[q]), int(rs[q])
@@ -240,7 +306,7 @@ class _SvdBatch:
             res.append((_kernels.convert(u.contiguous(), out_dtype),
                         _kernels.convert(s.contiguous(), out_dtype),
                         _kernels.convert(vt.contiguous(), out_dtype)))
-        del C, C2, U, Vt
+        del C, C2, U, Vt, d_dims, d_offC, d_oE
         return res
 
 
@@ -441,7 +507,7 @@ def build(H, dims):
             if nd.max_depth is None or nd.max_depth > 1:
                 nxt.extend(_expand(ct, nd, None if nd.max_depth is None else nd.max_depth - 1,
                                    False))
-        if cands and (not LOWRANK or (not FP64_SKETCH and dt == np.float64)):
+        if cands and not LOWRANK:
             for nd in cands:
                 nd.svd = None
             cands = []

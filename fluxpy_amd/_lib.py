@@ -54,7 +54,7 @@ SIGNATURES = {
                                         _vp, _vp]),
     "flux_block_stats": (_c_int, [_vp, _vp, _c_i64, _vp, _vp, _vp, _vp]),
     "flux_block_spmm": (_c_int, [_vp, _vp, _vp, _c_int, _c_i64, _vp, _vp, _vp, _c_i64, _vp,
-                                 _vp, _vp, _vp, ctypes.c_uint64, _vp]),
+                                 _vp, _vp, _vp, ctypes.c_uint64, _vp, _vp]),
     "flux_batched_gram": (_c_int, [_c_i64, _vp, _vp, _c_i64, _vp, _vp, _vp, _vp, _vp]),
     "flux_batched_gemm": (_c_int, [_c_i64, _vp, _c_i64, _vp, _vp, _vp, _vp]),
     "flux_batched_syevj": (_c_int, [_c_i64, _vp, _c_i64, _vp, _vp, _vp, _vp, _vp, _vp, _c_int,

@@ -61,12 +61,12 @@ def _csr_nbytes(m, nnz, itemsize):
 
 
 def _factor_nnz(m, n, nzrows, nzcols, k):
-    """nnz(U), nnz(Vt) of a k-term factorisation: zero lines of the block on
-    its longer side give zero lines of that factor (SciPy's svds forms the
-    Gram matrix of the shorter side)."""
-    if m >= n:
-        return k * nzrows, k * n
-    return k * m, k * nzcols
+    """nnz(U), nnz(Vt) of the k-term svds factors of an m x n block with
+    nzrows nonzero rows and nzcols nonzero columns: a zero row of the block
+    is a zero row of U and a zero column a zero column of Vt (the ARPACK
+    Gram eigenvectors of linalg.py:12 are exactly zero on its null lines)."""
+    del m, n
+    return k * nzrows, k * nzcols
 
 
 def _svd_nbytes(m, n, nzrows, nzcols, k, itemsize):

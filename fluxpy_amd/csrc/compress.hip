@@ -78,7 +78,8 @@ __global__ void __launch_bounds__(256) k_block_spmm(
     const TA *__restrict__ data, int64_t nb, const int64_t *__restrict__ blk,
     const int64_t *__restrict__ ls, const int64_t *__restrict__ rowstart,
     int64_t total_rows, const double *__restrict__ X, const int64_t *__restrict__ offX,
-    double *__restrict__ Y, const int64_t *__restrict__ offY, uint64_t seed) {
+    double *__restrict__ Y, const int64_t *__restrict__ offY, uint64_t seed,
+    int32_t *__restrict__ status) {
   const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   if (w >= total_rows) return;
@@ -106,6 +107,13 @@ __global__ void __launch_bounds__(256) k_block_spmm(
     if (c < l) {
       for (int64_t k = k0; k < k1; ++k) {
         const int64_t j = indices[k];
+        // The column ids of a row are sorted, so [k0, k1) holds only ids in
+        // [c0, c1).  An id outside (an unsorted row) would address X outside
+        // block b: it is skipped and reported instead of read.
+        if (j < c0 || j >= c1) {
+          atomicOr(status, 1);
+          continue;
+        }
         const double x = X ? X[offX[b] + (j - c0) * l + c] : rademacher(seed, j, c);
         acc += (double)data[k] * x;
       }
@@ -373,10 +381,10 @@ extern "C" int flux_block_spmm(const int64_t *indptr, const int32_t *indices, co
                                int dtype, int64_t nb, const int64_t *blk, const int64_t *ls,
                                const int64_t *rowstart, int64_t total_rows, const double *X,
                                const int64_t *offX, double *Y, const int64_t *offY,
-                               uint64_t seed, void *stream) {
+                               uint64_t seed, int32_t *status, void *stream) {
   FLUX_CHECK(nb >= 0 && total_rows >= 0, "flux_block_spmm: bad sizes");
   if (nb == 0 || total_rows == 0) return 0;
-  FLUX_CHECK(indptr && indices && data && blk && ls && rowstart && Y && offY,
+  FLUX_CHECK(indptr && indices && data && blk && ls && rowstart && Y && offY && status,
              "flux_block_spmm: NULL array");
   FLUX_CHECK(X == nullptr || offX != nullptr, "flux_block_spmm: offX required with X");
   hipStream_t st = as_stream(stream);
@@ -384,11 +392,11 @@ extern "C" int flux_block_spmm(const int64_t *indptr, const int32_t *indices, co
   if (dtype == FLUX_F64)
     hipLaunchKernelGGL(k_block_spmm<double>, dim3(g), dim3(256), 0, st, indptr, indices,
                        (const double *)data, nb, blk, ls, rowstart, total_rows, X, offX, Y,
-                       offY, seed);
+                       offY, seed, status);
   else if (dtype == FLUX_F32)
     hipLaunchKernelGGL(k_block_spmm<float>, dim3(g), dim3(256), 0, st, indptr, indices,
                        (const float *)data, nb, blk, ls, rowstart, total_rows, X, offX, Y,
-                       offY, seed);
+                       offY, seed, status);
   else
     FLUX_CHECK(false, "flux_block_spmm: dtype must be F32/F64");
   FLUX_LAUNCH_CHECK();

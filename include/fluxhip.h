@@ -221,7 +221,8 @@ int flux_block_spmm(const int64_t *indptr, const int32_t *indices,
                     const void *data, int dtype, int64_t nb, const int64_t *blk,
                     const int64_t *ls, const int64_t *rowstart,
                     int64_t total_rows, const double *X, const int64_t *offX,
-                    double *Y, const int64_t *offY, uint64_t seed, void *stream);
+                    double *Y, const int64_t *offY, uint64_t seed,
+                    int32_t *status, void *stream);
 
 /* G_b = X_b^T X_b (l_b x l_b) for X_b (rows[b] x l_b); lmax = max l_b. */
 int flux_batched_gram(int64_t nb, const int64_t *rows, const int64_t *ls,

@@ -18,11 +18,17 @@ Restates, with NumPy / SciPy / LAPACK:
                        children (:416-420)
 * block matvec         compressed_form_factors.py:407-414
 
-Factor sparsity model (the only part not restated from source): SciPy's
-svds takes the Gram matrix of the smaller side, so a zero row of a tall
-block (or a zero column of a wide one) gives a zero line in that factor;
-nnz(U) and nnz(Vt) are counted that way.  ARPACK's exact zero pattern is
-parity unpinned (it depends on LAPACK internals).
+Factor sparsity model (the only part not restated from source): the svds of
+the SciPy the reference pinned (scipy>=1.5.4, requirements.txt:34) takes
+ARPACK eigenpairs of the Gram matrix of the shorter side and forms the other
+factor as A v / s (tall) or A^T u / s (wide).  ARPACK's Ritz vectors are
+exactly zero on the null lines of that Gram matrix, and the product with A
+is exactly zero on the zero lines of A, so a zero row of the block gives a
+zero row of U and a zero column gives a zero column of Vt on either shape:
+nnz(U) = k * (nonzero rows), nnz(Vt) = k * (nonzero columns).  Pinned by
+tests/test_host.py against real ARPACK factors of blocks with zero rows and
+columns (tall, wide, square); the SciPy installed here (1.15) keeps the
+same Vt pattern and has U dense up to LAPACK rounding on the zero rows.
 
 ``force_max_depth=True`` with max_depth > 1 hits ``assert False`` in the
 reference (compressed_form_factors.py:300-301); the oracle implements the
@@ -66,13 +72,11 @@ def csr_nbytes(m, nnz, itemsize):
 
 
 def factor_nnz(A, k):
-    """(nnz(U), nnz(Vt)) of a k-term svds factorisation of block A."""
-    m, n = A.shape
-    if m >= n:
-        nzr = int((np.diff(A.indptr) > 0).sum())
-        return k * nzr, k * n
+    """(nnz(U), nnz(Vt)) of a k-term svds factorisation of block A: k per
+    nonzero row of A, k per nonzero column of A."""
+    nzr = int((np.diff(A.indptr) > 0).sum())
     nzc = int(np.unique(A.indices).size)
-    return k * m, k * nzc
+    return k * nzr, k * nzc
 
 
 def svd_nbytes(A, k, itemsize):

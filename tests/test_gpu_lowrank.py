@@ -1,0 +1,221 @@
+"""GPU tests of the low-rank path: the drop-in rank estimation
+(linalg.py:8-50 of fluxpy) against LAPACK singular values, the sketch's
+range check, the compressed operator in every stored precision, the
+(I - rho F) kernel operator, and the plain configuration (spherical crater,
+1,922 facets, FP64) at its full size."""
+import numpy as np
+import pytest
+
+from fluxpy_amd import meshes
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_dev():
+    import torch
+    torch.cuda.set_device(0)
+    return torch
+
+
+@pytest.fixture(scope="module")
+def crater32(oracle_mod):
+    """Config-1 mesh: spherical crater, Delaunay of 32 x 32 jittered points."""
+    from fluxpy_amd.form_factors import get_form_factor_matrix
+    from fluxpy_amd.shape import CgalTrimeshShapeModel
+    V, F = meshes.spherical_crater(npts=32, seed=1001)
+    sm = CgalTrimeshShapeModel(V, F)
+    return sm, get_form_factor_matrix(sm)
+
+
+def _far_block(sm, FF, size=160):
+    """A well-separated block of F (rows and columns from opposite sides of
+    the crater bowl), which is numerically low-rank."""
+    P = sm.P
+    r = np.argsort(P[:, 0])[:size]
+    c = np.argsort(-P[:, 0])[:size]
+    return FF.tocsr()[np.sort(r)][:, np.sort(c)]
+
+
+# --------------------------------------------------------------------------
+# linalg drop-ins
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_sparse_svd_matches_lapack(torch_dev, crater32, dt):
+    from fluxpy_amd import linalg
+    sm, FF = crater32
+    A = _far_block(sm, FF).astype(dt)
+    assert A.nnz > 1000
+    s_ref = np.linalg.svd(A.toarray().astype(np.float64), compute_uv=False)
+    k = 6
+    U, S, Vt = linalg.sparse_svd(A, k)
+    assert U.shape == (A.shape[0], k) and Vt.shape == (k, A.shape[1]) and S.shape == (k,)
+    assert np.all(np.diff(S) <= 0)
+    # leading singular values: the sketch (2k + 8 columns) resolves them to
+    # rounding of the stored values
+    rel = 1e-10 if dt == np.float64 else 2e-5
+    assert np.allclose(S[:4], s_ref[:4], rtol=rel, atol=rel * s_ref[0])
+    # orthonormal singular vectors
+    Ud, Vd = U.toarray(), Vt.toarray()
+    assert abs(Ud.T @ Ud - np.eye(k)).max() < 50 * rel
+    assert abs(Vd @ Vd.T - np.eye(k)).max() < 50 * rel
+    # and a k-term approximation no worse than the optimum by more than the
+    # sketch's tail
+    err = np.linalg.norm(A.toarray() - (Ud * S) @ Vd, 2)
+    assert err <= 1.5 * s_ref[k] + 10 * rel * s_ref[0]
+
+
+def test_estimate_rank_rule(torch_dev, crater32):
+    """Rank = first r with s_r / s_0 <= tol (linalg.py:42-46), compared with
+    LAPACK's singular values away from the threshold; None when the factors
+    would take max_nbytes; 0 / 1 for empty / 1 x 1 input."""
+    import scipy.sparse
+    from fluxpy_amd import linalg
+    sm, FF = crater32
+    A = _far_block(sm, FF)
+    s_ref = np.linalg.svd(A.toarray(), compute_uv=False)
+    ratio = s_ref / s_ref[0]
+    for tol in (1e-2, 1e-3, 1e-4):
+        out = linalg.estimate_rank(A, tol)
+        assert out is not None
+        U, S, Vt, thr = out
+        r = S.size
+        r_ref = int(np.where(ratio <= tol)[0][0])
+        near = np.any(abs(np.log(ratio[max(0, r_ref - 1):r_ref + 2] / tol)) < np.log(1.2))
+        assert r == r_ref or near, (tol, r, r_ref)
+        assert thr <= tol
+        assert U.shape == (A.shape[0], r) and Vt.shape == (r, A.shape[1])
+        assert scipy.sparse.issparse(U) and scipy.sparse.issparse(Vt)
+        approx = (U.toarray() * S) @ Vt.toarray()
+        assert np.linalg.norm(A.toarray() - approx, 2) <= 3 * tol * s_ref[0]
+    # byte cap: the rank-r factors of a 160 x 160 block cannot fit in 100 bytes
+    assert linalg.estimate_rank(A, 1e-4, max_nbytes=100) is None
+    assert linalg.estimate_rank(scipy.sparse.csr_matrix((0, 5)), 1e-3) == 0
+    assert linalg.estimate_rank(scipy.sparse.csr_matrix(np.ones((1, 1))), 1e-3) == 1
+
+
+def test_sketch_reports_unsorted_rows(torch_dev):
+    """A CSR whose row is not sorted by column holds ids outside the block
+    between the two binary-search bounds; the sketch kernel skips them and
+    the build raises instead of reading outside the block."""
+    from fluxpy_amd import _hbuild, _kernels, _lib
+    t = torch_dev
+    n = 64
+    rng = np.random.default_rng(0)
+    indptr = np.arange(0, n * n + 1, n, dtype=np.int64)
+    indices = np.tile(np.arange(n, dtype=np.int32), n)
+    indices[5 * n:6 * n] = indices[5 * n:6 * n][::-1].copy()     # row 5 descending
+    data = rng.random(n * n)
+    dev = _lib.device()
+    F = _kernels.DeviceCSR(t.tensor(indptr, device=dev), t.tensor(indices, device=dev),
+                           t.tensor(data, device=dev), (n, n))
+    FT = _kernels.csr_transpose(F)
+    with pytest.raises(_lib.FluxError, match="sorted"):
+        _hbuild._SvdBatch(F, FT, [(0, 32, 32, 64)], [16], 1)
+    # the same block of a sorted matrix passes
+    indices[5 * n:6 * n] = np.arange(n, dtype=np.int32)
+    F = _kernels.DeviceCSR(t.tensor(indptr, device=dev), t.tensor(indices, device=dev),
+                           t.tensor(data, device=dev), (n, n))
+    sb = _hbuild._SvdBatch(F, _kernels.csr_transpose(F), [(0, 32, 32, 64)], [16], 1)
+    assert sb.sigma[0].size == 16
+
+
+# --------------------------------------------------------------------------
+# the compressed operator in each stored precision, and (I - rho F)
+# --------------------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def compressed64(crater32):
+    from fluxpy_amd.compressed_form_factors import CompressedFormFactorMatrix
+    sm, FF = crater32
+    H = CompressedFormFactorMatrix(sm, tol=1e-2, min_size=1024)
+    assert H.build_stats["svd_leaves"] > 0
+    return H
+
+
+@pytest.mark.parametrize("values,tol", [("fp64", 1e-13), ("fp32", 2e-6), ("fp16", 2e-3),
+                                        ("bf16", 1.6e-2)])
+def test_device_operator_precisions(torch_dev, crater32, compressed64, values, tol):
+    """Near-field values and far-field factors stored in `values` (products
+    accumulated in FP32, FP64 for 'fp64') against the same compressed
+    operator in FP64; the stated bound is a few units of the storage
+    format's rounding relative to max |y|."""
+    from fluxpy_amd import _lib
+    sm, FF = crater32
+    H = compressed64
+    rng = np.random.default_rng(5)
+    X = np.abs(rng.standard_normal((sm.num_faces, 3)))
+    op64 = H.device_operator("fp64")
+    opv = H.device_operator(values)
+    assert opv.nfar == op64.nfar > 0
+    Xd = _lib.to_dev(X)
+    Xi = op64.to_internal(Xd)
+    Y64 = op64.apply(Xi).cpu().numpy()
+    Xv = Xi if values == "fp64" else Xi.to(torch_dev.float32)
+    Yv = opv.apply(Xv).double().cpu().numpy()
+    err = abs(Yv - Y64).max() / abs(Y64).max()
+    assert err <= tol, (values, err)
+    # and the FP64 operator against the dense F within the compression bound
+    Yd = FF @ X
+    Y = H.apply(Xd).cpu().numpy()
+    assert abs(Y - Yd).max() / abs(Yd).max() <= 50 * 1e-2
+
+
+@pytest.mark.parametrize("place", ["left", "right"])
+def test_compressed_kernel_matrix(torch_dev, crater32, compressed64, place):
+    """K x = x - rho F x / x - F (rho x) (compressed_form_factors.py:875-913)
+    against the dense (I - rho F) x with the compressed F's error bound, and
+    exactly against the same expression through the compressed product."""
+    from fluxpy_amd.compressed_form_factors import CompressedKernelMatrix
+    sm, FF = crater32
+    H = compressed64
+    K = CompressedKernelMatrix(H, 0.3, place)
+    x = np.random.default_rng(6).standard_normal(sm.num_faces)
+    y = K @ x
+    Hx = H @ (0.3 * x) if place == "right" else 0.3 * (H @ x)
+    assert np.allclose(y, x - Hx, rtol=0, atol=1e-14 * abs(x).max())
+    yd = x - 0.3 * (FF @ x)
+    assert abs(y - yd).max() <= 0.3 * 50 * 1e-2 * abs(FF @ x).max()
+
+
+# --------------------------------------------------------------------------
+# plain configuration at its stated size
+# --------------------------------------------------------------------------
+
+def test_plain_configuration_full_size(torch_dev, oracle_mod, crater32):
+    """Spherical crater (32 x 32 jittered points, 1,922 facets, FP64), one
+    sun at 10 degrees elevation, rho = 0.12, eps = 0.95: F bit-exact, GMRES
+    to 1e-12 against the oracle's dense LU, both temperature forms against
+    the oracle."""
+    from fluxpy_amd.form_factors import FormFactorMatrix
+    from fluxpy_amd.model import compute_steady_state_temp, steady_state_temp_closed
+    from fluxpy_amd.solve import solve_radiosity
+    sm, FF = crater32
+    assert sm.num_faces == 1922
+    S = oracle_mod.OracleScene(sm.V, sm.F)
+    FFo = S.form_factor_matrix(sm.P, sm.N, sm.A)
+    assert np.array_equal(FF.indptr, FFo.indptr) and np.array_equal(FF.indices, FFo.indices)
+    assert np.array_equal(FF.data, FFo.data)
+    d = np.array([np.cos(np.deg2rad(10)), 0, np.sin(np.deg2rad(10))])
+    E = sm.get_direct_irradiance(1365.0, d)
+    I = np.arange(sm.num_faces)
+    Eo = oracle_mod.direct_irradiance(sm.N, S.occluded(I, d)[:, 0], 1365.0, d)
+    assert np.array_equal(E, Eo)
+    Fm = FormFactorMatrix.from_shape_model(sm)
+    B, it = solve_radiosity(Fm, E, 0.12, method="gmres", tol=1e-12)
+    Bo = oracle_mod.solve_radiosity_dense(FFo, Eo, 0.12)
+    assert abs(B - Bo).max() <= 1e-11 * abs(Bo).max()
+    # residual of the returned B in the oracle's F
+    r = Eo - (B - 0.12 * (FFo @ B))
+    assert np.linalg.norm(r) <= 1e-11 * np.linalg.norm(Eo)
+    T = compute_steady_state_temp(Fm, E, 0.12, 0.95)
+    To = oracle_mod.steady_state_temp(FFo, Eo, 0.12, 0.95)
+    assert abs(T - To).max() <= 1e-10 * To.max()
+    from scipy.constants import Stefan_Boltzmann
+    Tc = steady_state_temp_closed(Fm, E, B, 0.12, 0.95)
+    Tco = ((1 - 0.12) * (Eo + FFo @ Bo) / (0.95 * Stefan_Boltzmann)) ** 0.25
+    assert abs(Tc - Tco).max() <= 1e-10 * Tco.max()
+    # a planted change (emissivity 0.9) moves the closed form well outside
+    Tbad = ((1 - 0.12) * (Eo + FFo @ Bo) / (0.9 * Stefan_Boltzmann)) ** 0.25
+    assert abs(Tc - Tbad).max() > 1e-3 * Tco.max()

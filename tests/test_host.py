@@ -59,9 +59,9 @@ def test_byte_rules_match_reference_formulas():
     A = scipy.sparse.random(50, 70, density=0.2, format="csr", dtype=np.float32,
                             random_state=1)
     assert _csr_nbytes(50, A.nnz, 4) == A.data.nbytes + A.indices.nbytes + A.indptr.nbytes
-    # tall block with zero rows: U rows of zero block rows are zero
-    assert _svd_nbytes(100, 40, 90, 40, 5, 8) == (5 * 90 * 12 + 101 * 4) + 5 * 8 + \
-        (5 * 40 * 12 + 6 * 4)
+    # 90 nonzero rows, 33 nonzero columns: U holds 5 x 90 values, Vt 5 x 33
+    assert _svd_nbytes(100, 40, 90, 33, 5, 8) == (5 * 90 * 12 + 101 * 4) + 5 * 8 + \
+        (5 * 33 * 12 + 6 * 4)
 
 
 def test_fractal_mesh_counts():
@@ -84,3 +84,163 @@ def test_direct_instantiation_and_argument_errors():
         CompressedKernelMatrix(None, 1.5)
     with pytest.raises(ValueError):
         CompressedKernelMatrix(None, 0.5, "middle")
+
+
+# --------------------------------------------------------------------------
+# device pointers handed to a launch
+# --------------------------------------------------------------------------
+
+def _product_sources():
+    import glob
+    import os
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    return sorted(glob.glob(os.path.join(root, "fluxpy_amd", "*.py"))) + \
+        [os.path.join(root, "bench.py")]
+
+
+def test_argument_temporary_is_freed_before_the_call():
+    """A tensor created inside an argument list and reduced to data_ptr()
+    is released before the callee runs: its memory can then be handed to the
+    next allocation while the pointer is still in use."""
+    import weakref
+    import torch
+    freed = []
+
+    def make():
+        t = torch.empty(64)
+        weakref.finalize(t, freed.append, True)
+        return t
+
+    def callee(p0, p1):
+        return len(freed)
+
+    # (outside the assert statements: pytest's assertion rewriting keeps
+    # intermediate values alive)
+    n_inline = callee(make().data_ptr(), make().data_ptr())
+    keep = make()
+    freed.clear()
+    n_named = callee(keep.data_ptr(), keep.data_ptr())
+    assert n_inline == 2
+    assert n_named == 0
+
+
+def test_launch_arguments_hold_no_device_temporaries():
+    """No product source takes data_ptr() (or _lib.ptr()) of a call result:
+    every device array passed to a launch is bound to a name first."""
+    import ast
+    bad = []
+    for path in _product_sources():
+        tree = ast.parse(open(path).read(), path)
+        for node in ast.walk(tree):
+            if not isinstance(node, ast.Call) or not isinstance(node.func, ast.Attribute):
+                continue
+            if node.func.attr == "data_ptr" and isinstance(node.func.value, ast.Call):
+                bad.append(f"{path}:{node.lineno}")
+            if node.func.attr == "ptr" and node.args and isinstance(node.args[0], ast.Call):
+                bad.append(f"{path}:{node.lineno}")
+    assert not bad, f"device temporaries passed by pointer: {bad}"
+
+
+def test_sketch_descriptor_checks():
+    """The host-side checks of the batched sketch launches accept in-range
+    descriptors and reject every out-of-range one before a launch."""
+    from fluxpy_amd import _hbuild, _lib
+    l, r, m = np.array([8, 5]), np.array([3, 2]), np.array([10, 7])
+    offC = np.array([0, 24])
+    offY = np.array([0, 80])
+    z = np.zeros(2, dtype=np.int64)
+    ok = np.stack([m, r, l, offY, l, z, offC, r, z, [0, 30], r, z], 1)
+    _hbuild._check_gemm(ok, 80 + 35, 24 + 10, 30 + 14)
+    with pytest.raises(_lib.FluxError):
+        _hbuild._check_gemm(ok, 80 + 34, 24 + 10, 30 + 14)     # Y one short
+    with pytest.raises(_lib.FluxError):
+        _hbuild._check_gemm(ok, 80 + 35, 24 + 10, 30 + 13)     # C one short
+    bad = ok.copy()
+    bad[1, 7] = 1                                              # ldb below N
+    with pytest.raises(_lib.FluxError):
+        _hbuild._check_gemm(bad, 10 ** 6, 10 ** 6, 10 ** 6)
+    # C^T Y^T with both operands stored transposed (the Vt product's shape)
+    tr = np.stack([r, m, l, offC, r, z + 1, offY, l, z + 1, [0, 30], m, z], 1)
+    _hbuild._check_gemm(tr, 24 + 10, 80 + 35, 30 + 14)
+    with pytest.raises(_lib.FluxError):
+        _hbuild._check_gemm(tr, 24 + 9, 80 + 35, 30 + 14)
+    _hbuild._check_extent("x", [0, 5], [5, 3], 8)
+    with pytest.raises(_lib.FluxError):
+        _hbuild._check_extent("x", [0, 5], [5, 4], 8)
+    with pytest.raises(_lib.FluxError):
+        _hbuild._check_extent("x", [-1], [1], 8)
+
+
+# --------------------------------------------------------------------------
+# the factor byte model of the rank rule
+# --------------------------------------------------------------------------
+
+def _svds_gram_side(A, k):
+    """The svds algorithm of the SciPy the reference pinned (scipy>=1.5.4,
+    requirements.txt:34; scipy/sparse/linalg/eigen/arpack/arpack.py, svds):
+    ARPACK eigenpairs of the Gram matrix of the shorter side, then the other
+    side's vectors as A v / s (tall) or A^T u / s (wide)."""
+    import scipy.sparse.linalg as sla
+    m, n = A.shape
+    if m >= n:
+        G = sla.LinearOperator((n, n), matvec=lambda x: A.T @ (A @ x), dtype=np.float64)
+        w, v = sla.eigsh(G, k=k, v0=np.ones(n))
+        s = np.sqrt(w)
+        return (A @ v) / s, s, v.T
+    G = sla.LinearOperator((m, m), matvec=lambda x: A @ (A.T @ x), dtype=np.float64)
+    w, u = sla.eigsh(G, k=k, v0=np.ones(m))
+    s = np.sqrt(w)
+    return u, s, ((A.T @ u) / s).T
+
+
+@pytest.mark.parametrize("shape", [(120, 60), (60, 120), (90, 90), (200, 41)])
+def test_factor_nnz_model_matches_svds(shape):
+    """oracle/hmatrix.py factor_nnz (and the product's _factor_nnz) against
+    the zero pattern of real ARPACK factors of blocks with zero rows and
+    columns; a swapped tall / wide rule fails."""
+    import scipy.sparse
+    from oracle import hmatrix as OH
+    from fluxpy_amd.compressed_form_factors import _factor_nnz
+    rng = np.random.default_rng(sum(shape))
+    m, n = shape
+    A = rng.standard_normal(shape) * (rng.random(shape) < 0.4)
+    A[rng.choice(m, m // 5, replace=False)] = 0
+    A[:, rng.choice(n, n // 4, replace=False)] = 0
+    A = scipy.sparse.csr_matrix(A)
+    nzr = int((np.diff(A.indptr) > 0).sum())
+    nzc = int(np.unique(A.indices).size)
+    for k in (4, 12):
+        U, s, Vt = _svds_gram_side(A, k)
+        got = (scipy.sparse.csr_matrix(U).nnz, scipy.sparse.csr_matrix(Vt).nnz)
+        assert OH.factor_nnz(A, k) == got
+        assert _factor_nnz(m, n, nzr, nzc, k) == got
+        # planted mistakes: zero lines counted on the longer side only, or
+        # on neither side
+        longer_only = (k * nzr, k * n) if m >= n else (k * m, k * nzc)
+        assert longer_only != got and (k * m, k * n) != got
+
+
+def test_current_svds_zero_pattern_is_documented():
+    """The svds of the SciPy installed here (1.15) orthogonalises the Gram
+    eigenvectors and takes a dense SVD of A V: the factor of the shorter
+    side keeps the exact zero lines, the other one is dense up to LAPACK
+    rounding there.  The byte model follows the pinned SciPy, which is a
+    lower bound here; this records the difference."""
+    import scipy.sparse
+    import scipy.sparse.linalg as sla
+    rng = np.random.default_rng(7)
+    for m, n in ((120, 60), (60, 120)):
+        A = rng.standard_normal((m, n)) * (rng.random((m, n)) < 0.4)
+        A[rng.choice(m, m // 5, replace=False)] = 0
+        A[:, rng.choice(n, n // 4, replace=False)] = 0
+        A = scipy.sparse.csr_matrix(A)
+        nzr = int((np.diff(A.indptr) > 0).sum())
+        nzc = int(np.unique(A.indices).size)
+        U, s, Vt = sla.svds(A, 6, random_state=1)
+        nu, nv = scipy.sparse.csr_matrix(U).nnz, scipy.sparse.csr_matrix(Vt).nnz
+        # the shorter side's factor keeps the exact zero lines; the longer
+        # side's holds the model's count plus LAPACK rounding on zero lines
+        if m >= n:
+            assert nv == 6 * nzc and 6 * nzr <= nu <= 6 * m
+        else:
+            assert nu == 6 * nzr and 6 * nzc <= nv <= 6 * n
