@@ -156,8 +156,22 @@ def _stats(F, FT, nodes):
 # --------------------------------------------------------------------------
 
 class _SvdBatch:
-    """Sketch + Gram + eigensolver for a batch of blocks; keeps the
-    intermediates so accepted blocks can be factorised afterwards.
+    """Randomized range sketch + Rayleigh-Ritz singular values for a batch
+    of blocks; keeps the intermediates so accepted blocks can be factorised
+    afterwards (once per batch).
+
+    For a block A (m x n) with sketch width l:
+      Y = A W                          (W: n x l Rademacher, flux_block_spmm)
+      Q = Y T1 T2                      (two Gram passes: T = V D^-1/2 from the
+                                        eigenpairs of the Gram matrix, columns
+                                        with D_i <= EIG_CUT * D_0 dropped)
+      Z = A^T Q                        (n x l)
+      Z^T Z = X diag(s^2) X^T          (s: singular values of Q^T A)
+      A ~ (Q X_r) diag(s_r) (diag(1/s_r) X_r^T Z^T).
+    One Gram pass leaves Q orthonormal only to eps * cond(Y)^2 (1e-2 at the
+    cut), which shows up as spurious singular values near 1e-2 s_0; the
+    second pass starts from a basis with condition ~1 and brings it to
+    rounding (CholeskyQR2).
 
     Every launch is preceded by a host-side check that the offsets and
     descriptors stay inside the buffers they address (``_check_extent``,
@@ -187,19 +201,20 @@ class _SvdBatch:
         offE = np.concatenate([[0], np.cumsum(l)])
         self.offY, self.offZ, self.offG, self.offE = offY, offZ, offG, offE
         f64 = t.float64
-        self.Y = t.empty(max(int(offY[-1]), 1), dtype=f64, device=dev)
+        self.Q = t.empty(max(int(offY[-1]), 1), dtype=f64, device=dev)
+        Q1 = t.empty_like(self.Q)
         self.Z = t.empty(max(int(offZ[-1]), 1), dtype=f64, device=dev)
         G = t.empty(max(int(offG[-1]), 1), dtype=f64, device=dev)
         Vw = t.empty_like(G)
-        Vm = t.empty_like(G)
-        d = t.empty(max(int(offE[-1]), 1), dtype=f64, device=dev)
-        self.Tm = t.empty_like(G)
+        self.Xe = t.empty_like(G)
+        T = t.empty_like(G)
+        self.lam = t.empty(max(int(offE[-1]), 1), dtype=f64, device=dev)
         kept = t.empty(nb, dtype=t.int64, device=dev)
         status = t.zeros(1, dtype=t.int32, device=dev)
-        for what, off, ext, buf in (("sketch Y", offY[:-1], m * l, self.Y),
+        for what, off, ext, buf in (("sketch Y", offY[:-1], m * l, self.Q),
                                     ("sketch Z", offZ[:-1], n * l, self.Z),
                                     ("Gram", offG[:-1], l * l, G),
-                                    ("eigenvalues", offE[:-1], l, d)):
+                                    ("eigenvalues", offE[:-1], l, self.lam)):
             _check_extent(what, off, ext, buf.numel())
         rowstart = np.concatenate([[0], np.cumsum(m)])[:-1]
         rowstartT = np.concatenate([[0], np.cumsum(n)])[:-1]
@@ -210,37 +225,34 @@ class _SvdBatch:
         d_m, d_n = _i64(m), _i64(n)
         d_rs, d_rsT = _i64(rowstart), _i64(rowstartT)
         code = _lib.dtype_code(F.data.dtype)
-        # Y = A W (W Rademacher), Z = A^T Y
-        _lib.call("flux_block_spmm", F.indptr.data_ptr(), F.indices.data_ptr(), F.data.data_ptr(),
-                  code, nb, d_blk.data_ptr(), d_ls.data_ptr(), d_rs.data_ptr(),
-                  int(m.sum()), None, None, self.Y.data_ptr(), d_offY.data_ptr(), seed,
-                  status.data_ptr(), st)
-        _lib.call("flux_block_spmm", FT.indptr.data_ptr(), FT.indices.data_ptr(),
-                  FT.data.data_ptr(), code, nb, d_blkT.data_ptr(), d_ls.data_ptr(),
-                  d_rsT.data_ptr(), int(n.sum()), self.Y.data_ptr(),
-                  d_offY.data_ptr(), self.Z.data_ptr(), d_offZ.data_ptr(), seed,
-                  status.data_ptr(), st)
         lmax = int(l.max())
-        _lib.call("flux_batched_gram", nb, d_m.data_ptr(), d_ls.data_ptr(), lmax,
-                  self.Y.data_ptr(), d_offY.data_ptr(), G.data_ptr(), d_offG.data_ptr(), st)
-        _lib.call("flux_batched_syevj", nb, d_ls.data_ptr(), lmax, G.data_ptr(),
-                  d_offG.data_ptr(), Vw.data_ptr(), Vm.data_ptr(), d.data_ptr(),
-                  d_offE.data_ptr(), 30, st)
-        _lib.call("flux_scale_basis", nb, d_ls.data_ptr(), Vm.data_ptr(), d_offG.data_ptr(),
-                  d.data_ptr(), d_offE.data_ptr(), EIG_CUT, self.Tm.data_ptr(),
-                  kept.data_ptr(), st)
-        _lib.call("flux_batched_gram", nb, d_n.data_ptr(), d_ls.data_ptr(), lmax,
-                  self.Z.data_ptr(), d_offZ.data_ptr(), G.data_ptr(), d_offG.data_ptr(), st)
-        # K = Tm^T (Z^T Z) Tm, through H1 = (Z^T Z) Tm
-        H1 = Vw
         og = offG[:-1]
         zeros = np.zeros(nb, dtype=np.int64)
-        _gemm(np.stack([l, l, l, og, l, zeros, og, l, zeros, og, l, zeros], 1),
-              G, self.Tm, H1, st)
-        _gemm(np.stack([l, l, l, og, l, zeros + 1, og, l, zeros, og, l, zeros], 1),
-              self.Tm, H1, G, st)
-        self.Xe = Vm
-        self.lam = d
+        # Y = A W  (into Q)
+        _lib.call("flux_block_spmm", F.indptr.data_ptr(), F.indices.data_ptr(), F.data.data_ptr(),
+                  code, nb, d_blk.data_ptr(), d_ls.data_ptr(), d_rs.data_ptr(),
+                  int(m.sum()), None, None, self.Q.data_ptr(), d_offY.data_ptr(), seed,
+                  status.data_ptr(), st)
+        # two Gram passes: Q1 = Y T1, Q = Q1 T2
+        for src, dst in ((self.Q, Q1), (Q1, self.Q)):
+            _lib.call("flux_batched_gram", nb, d_m.data_ptr(), d_ls.data_ptr(), lmax,
+                      src.data_ptr(), d_offY.data_ptr(), G.data_ptr(), d_offG.data_ptr(), st)
+            _lib.call("flux_batched_syevj", nb, d_ls.data_ptr(), lmax, G.data_ptr(),
+                      d_offG.data_ptr(), Vw.data_ptr(), self.Xe.data_ptr(),
+                      self.lam.data_ptr(), d_offE.data_ptr(), 30, st)
+            _lib.call("flux_scale_basis", nb, d_ls.data_ptr(), self.Xe.data_ptr(),
+                      d_offG.data_ptr(), self.lam.data_ptr(), d_offE.data_ptr(), EIG_CUT,
+                      T.data_ptr(), kept.data_ptr(), st)
+            _gemm(np.stack([m, l, l, offY[:-1], l, zeros, og, l, zeros, offY[:-1], l, zeros],
+                           1), src, T, dst, st)
+        # Z = A^T Q, then the eigenpairs of Z^T Z
+        _lib.call("flux_block_spmm", FT.indptr.data_ptr(), FT.indices.data_ptr(),
+                  FT.data.data_ptr(), code, nb, d_blkT.data_ptr(), d_ls.data_ptr(),
+                  d_rsT.data_ptr(), int(n.sum()), self.Q.data_ptr(),
+                  d_offY.data_ptr(), self.Z.data_ptr(), d_offZ.data_ptr(), seed,
+                  status.data_ptr(), st)
+        _lib.call("flux_batched_gram", nb, d_n.data_ptr(), d_ls.data_ptr(), lmax,
+                  self.Z.data_ptr(), d_offZ.data_ptr(), G.data_ptr(), d_offG.data_ptr(), st)
         _lib.call("flux_batched_syevj", nb, d_ls.data_ptr(), lmax, G.data_ptr(),
                   d_offG.data_ptr(), Vw.data_ptr(), self.Xe.data_ptr(), self.lam.data_ptr(),
                   d_offE.data_ptr(), 30, st)
@@ -250,17 +262,23 @@ class _SvdBatch:
             raise _lib.FluxError("sketch: a CSR row holds column ids outside its block "
                                  "(rows must be sorted)")
         self.sigma = [np.sqrt(np.maximum(lam[offE[b]:offE[b + 1]], 0.0)) for b in range(nb)]
+        self._factored = False
         del d_blk, d_blkT, d_ls, d_offY, d_offZ, d_offG, d_offE, d_m, d_n, d_rs, d_rsT
+        del Q1, G, Vw, T
 
     def factors(self, picks, out_dtype):
         """U (m x r), S (r), Vt (r x n) for (batch index, rank) pairs, as
-        device tensors in out_dtype."""
+        device tensors in out_dtype.  Called at most once per batch (it
+        scales the stored eigenvectors in place)."""
         t = _lib.torch()
         dev = _lib.device()
         st = _lib.stream()
         res = []
         if not picks:
             return res
+        if self._factored:
+            raise RuntimeError("_SvdBatch.factors may be called once")
+        self._factored = True
         bs = np.array([p[0] for p in picks], dtype=np.int64)
         rs = np.array([p[1] for p in picks], dtype=np.int64)
         if ((bs < 0) | (bs >= self.nb)).any():
@@ -270,33 +288,24 @@ class _SvdBatch:
             raise _lib.FluxError("sketch factors: rank outside [1, sketch width]")
         m, n = self.m[bs], self.n[bs]
         np_ = len(picks)
-        offC = np.concatenate([[0], np.cumsum(l * rs)])
         offU = np.concatenate([[0], np.cumsum(m * rs)])
         offV = np.concatenate([[0], np.cumsum(n * rs)])
-        C = t.empty(max(int(offC[-1]), 1), dtype=t.float64, device=dev)
-        C2 = t.empty_like(C)
         U = t.empty(max(int(offU[-1]), 1), dtype=t.float64, device=dev)
         Vt = t.empty(max(int(offV[-1]), 1), dtype=t.float64, device=dev)
-        oG = self.offG[bs]
-        oE = self.offE[bs]
+        oG, oE = self.offG[bs], self.offE[bs]
         zeros = np.zeros(np_, dtype=np.int64)
-        # C = Tm Xe[:, :r]  (l x r)
-        _gemm(np.stack([l, rs, l, oG, l, zeros, oG, l, zeros, offC[:-1], rs, zeros], 1),
-              self.Tm, self.Xe, C, st)
-        C2.copy_(C)
-        # C2[:, j] /= sqrt(lam_j)
-        dims = np.stack([l, rs], 1).astype(np.int64)
-        _check_extent("scaled basis", offC[:-1], l * rs, C2.numel())
-        _check_extent("eigenvalues", oE, rs, self.lam.numel())
-        d_dims, d_offC, d_oE = _i64(dims), _i64(offC[:-1]), _i64(oE)
-        _lib.call("flux_scale_cols_inv_sqrt", np_, d_dims.data_ptr(), C2.data_ptr(),
-                  d_offC.data_ptr(), self.lam.data_ptr(), d_oE.data_ptr(), st)
-        # U = Y C  (m x r)
-        _gemm(np.stack([m, rs, l, self.offY[bs], l, zeros, offC[:-1], rs, zeros, offU[:-1], rs,
-                        zeros], 1), self.Y, C, U, st)
-        # Vt = C2^T Z^T  (r x n)
-        _gemm(np.stack([rs, n, l, offC[:-1], rs, zeros + 1, self.offZ[bs], l, zeros + 1,
-                        offV[:-1], n, zeros], 1), C2, self.Z, Vt, st)
+        # U = Q X_r  (m x r)
+        _gemm(np.stack([m, rs, l, self.offY[bs], l, zeros, oG, l, zeros, offU[:-1], rs, zeros],
+                       1), self.Q, self.Xe, U, st)
+        # X[:, j] /= s_j, then Vt = X_r^T Z^T  (r x n)
+        dims = np.stack([l, l], 1).astype(np.int64)
+        _check_extent("eigenvectors", oG, l * l, self.Xe.numel())
+        _check_extent("eigenvalues", oE, l, self.lam.numel())
+        d_dims, d_oG, d_oE = _i64(dims), _i64(oG), _i64(oE)
+        _lib.call("flux_scale_cols_inv_sqrt", np_, d_dims.data_ptr(), self.Xe.data_ptr(),
+                  d_oG.data_ptr(), self.lam.data_ptr(), d_oE.data_ptr(), st)
+        _gemm(np.stack([rs, n, l, oG, l, zeros + 1, self.offZ[bs], l, zeros + 1, offV[:-1], n,
+                        zeros], 1), self.Xe, self.Z, Vt, st)
         lam = self.lam
         for q in range(np_):
             b, r = int(bs[q]), int(rs[q])
@@ -306,7 +315,7 @@ class _SvdBatch:
             res.append((_kernels.convert(u.contiguous(), out_dtype),
                         _kernels.convert(s.contiguous(), out_dtype),
                         _kernels.convert(vt.contiguous(), out_dtype)))
-        del C, C2, U, Vt, d_dims, d_offC, d_oE
+        del U, Vt, d_dims, d_oG, d_oE
         return res
 
 
