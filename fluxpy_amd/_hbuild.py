@@ -14,6 +14,7 @@ Build (called from CompressedFormFactorMatrix.__init__):
 5. device operator: one near-field CSR (F minus the SVD leaves) plus the
    far-field factors, all in cluster order.
 """
+import logging
 import time
 import warnings
 
@@ -28,7 +29,10 @@ from .compressed_form_factors import (
     _csr_nbytes, _svd_nbytes)
 
 K0 = 40               # linalg.py:20 (k0 = 40)
-L_MAX = 1024          # largest sketch the batched eigensolver accepts
+L_MAX = 256           # widest sketch: the rank search ends with its k = 160 round
+NS_STEPS = 4          # Newton-Schulz steps after the rank-revealing pass
+
+_log = logging.getLogger("fluxpy_amd.hbuild")
 EIG_CUT = 1e-14       # drop sketch directions with D_i <= EIG_CUT * D_0
 SKETCH_BYTES = 2 << 30
 # False: no block is offered to the sketch, so every leaf is dense / CSR /
@@ -162,16 +166,17 @@ class _SvdBatch:
 
     For a block A (m x n) with sketch width l:
       Y = A W                          (W: n x l Rademacher, flux_block_spmm)
-      Q = Y T1 T2                      (two Gram passes: T = V D^-1/2 from the
-                                        eigenpairs of the Gram matrix, columns
-                                        with D_i <= EIG_CUT * D_0 dropped)
+      Q = Y V D^-1/2                   (eigenpairs of Y^T Y; directions with
+                                        D_i <= EIG_CUT * D_0 dropped)
+      Q <- Q (3 I - Q^T Q) / 2         (NS_STEPS Newton-Schulz steps)
       Z = A^T Q                        (n x l)
       Z^T Z = X diag(s^2) X^T          (s: singular values of Q^T A)
       A ~ (Q X_r) diag(s_r) (diag(1/s_r) X_r^T Z^T).
-    One Gram pass leaves Q orthonormal only to eps * cond(Y)^2 (1e-2 at the
-    cut), which shows up as spurious singular values near 1e-2 s_0; the
-    second pass starts from a basis with condition ~1 and brings it to
-    rounding (CholeskyQR2).
+    The eigenvector basis is orthonormal only to eps * cond(Y)^2 (2e-2 at
+    the cut), which would show up as spurious singular values near that
+    fraction of s_0; each Newton-Schulz step squares the error, four bring
+    it to rounding.  Forming Z from the orthonormal Q (not A^T Y) keeps the
+    Gram matrix of the last step at cond(A)^2.
 
     Every launch is preceded by a host-side check that the offsets and
     descriptors stay inside the buffers they address (``_check_extent``,
@@ -228,23 +233,36 @@ class _SvdBatch:
         lmax = int(l.max())
         og = offG[:-1]
         zeros = np.zeros(nb, dtype=np.int64)
-        # Y = A W  (into Q)
+        # Y = A W  (into Q1)
         _lib.call("flux_block_spmm", F.indptr.data_ptr(), F.indices.data_ptr(), F.data.data_ptr(),
                   code, nb, d_blk.data_ptr(), d_ls.data_ptr(), d_rs.data_ptr(),
-                  int(m.sum()), None, None, self.Q.data_ptr(), d_offY.data_ptr(), seed,
+                  int(m.sum()), None, None, Q1.data_ptr(), d_offY.data_ptr(), seed,
                   status.data_ptr(), st)
-        # two Gram passes: Q1 = Y T1, Q = Q1 T2
-        for src, dst in ((self.Q, Q1), (Q1, self.Q)):
+        basis = np.stack([m, l, l, offY[:-1], l, zeros, og, l, zeros, offY[:-1], l, zeros], 1)
+        # rank-revealing pass: Q = Y V D^-1/2 from the eigenpairs of Y^T Y
+        _lib.call("flux_batched_gram", nb, d_m.data_ptr(), d_ls.data_ptr(), lmax,
+                  Q1.data_ptr(), d_offY.data_ptr(), G.data_ptr(), d_offG.data_ptr(), st)
+        _lib.call("flux_batched_syevj", nb, d_ls.data_ptr(), lmax, G.data_ptr(),
+                  d_offG.data_ptr(), Vw.data_ptr(), self.Xe.data_ptr(),
+                  self.lam.data_ptr(), d_offE.data_ptr(), 30, st)
+        _lib.call("flux_scale_basis", nb, d_ls.data_ptr(), self.Xe.data_ptr(),
+                  d_offG.data_ptr(), self.lam.data_ptr(), d_offE.data_ptr(), EIG_CUT,
+                  T.data_ptr(), kept.data_ptr(), st)
+        _gemm(basis, Q1, T, self.Q, st)
+        # Q^T Q = I + E with |E| <= eps cond(Y)^2 (about 2e-2 at the cut):
+        # Newton-Schulz steps Q <- Q (3 I - Q^T Q) / 2 square the error
+        # (2e-2 -> 7e-4 -> 5e-7 -> 3e-13 -> rounding) and keep the dropped
+        # (zero) columns zero
+        cur, other = self.Q, Q1
+        for _ in range(NS_STEPS):
             _lib.call("flux_batched_gram", nb, d_m.data_ptr(), d_ls.data_ptr(), lmax,
-                      src.data_ptr(), d_offY.data_ptr(), G.data_ptr(), d_offG.data_ptr(), st)
-            _lib.call("flux_batched_syevj", nb, d_ls.data_ptr(), lmax, G.data_ptr(),
-                      d_offG.data_ptr(), Vw.data_ptr(), self.Xe.data_ptr(),
-                      self.lam.data_ptr(), d_offE.data_ptr(), 30, st)
-            _lib.call("flux_scale_basis", nb, d_ls.data_ptr(), self.Xe.data_ptr(),
-                      d_offG.data_ptr(), self.lam.data_ptr(), d_offE.data_ptr(), EIG_CUT,
-                      T.data_ptr(), kept.data_ptr(), st)
-            _gemm(np.stack([m, l, l, offY[:-1], l, zeros, og, l, zeros, offY[:-1], l, zeros],
-                           1), src, T, dst, st)
+                      cur.data_ptr(), d_offY.data_ptr(), G.data_ptr(), d_offG.data_ptr(), st)
+            _lib.call("flux_batched_ns_factor", nb, d_ls.data_ptr(), G.data_ptr(),
+                      d_offG.data_ptr(), st)
+            _gemm(basis, cur, G, other, st)
+            cur, other = other, cur
+        if cur is not self.Q:
+            self.Q.copy_(cur)
         # Z = A^T Q, then the eigenpairs of Z^T Z
         _lib.call("flux_block_spmm", FT.indptr.data_ptr(), FT.indices.data_ptr(),
                   FT.data.data_ptr(), code, nb, d_blkT.data_ptr(), d_ls.data_ptr(),
@@ -357,8 +375,12 @@ def _run_svds(H, F, FT, cands, itemsize, tdt, far):
                 chunk.append(work[i])
                 mem += need
                 i += 1
+            t_sk = time.perf_counter()
             sb = _SvdBatch(F, FT, [c[0].rng for c in chunk], [c[2] for c in chunk],
                            H._seed + len(far))
+            _log.debug("  sketch of %d blocks, widths %d..%d: %.3f s", len(chunk),
+                       min(c[2] for c in chunk), max(c[2] for c in chunk),
+                       time.perf_counter() - t_sk)
             H._build_stats["svd_blocks"] = H._build_stats.get("svd_blocks", 0) + len(chunk)
             picks = []
             for q, (nd, k, l) in enumerate(chunk):
@@ -520,6 +542,7 @@ def build(H, dims):
             for nd in cands:
                 nd.svd = None
             cands = []
+        _log.debug("level %d: %d blocks, %d sketch candidates", nlev, len(level), len(cands))
         if cands:
             _run_svds(H, F, FT, cands, itemsize, tdt, far)
         level = nxt

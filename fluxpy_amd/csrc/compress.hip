@@ -190,6 +190,169 @@ __global__ void __launch_bounds__(256) k_batched_gemm(const int64_t *__restrict_
   }
 }
 
+// Tournament pairing of step `step` of a cyclic Jacobi sweep over npad
+// (even) indices: index 0 stays, the others rotate; pair k is (p, q), p < q.
+__device__ __forceinline__ void jacobi_pair(int k, int step, int npad, int &p, int &q) {
+  const int x = (k == 0) ? 0 : ((k - 1 + step) % (npad - 1)) + 1;
+  const int y = ((npad - 2 - k + step) % (npad - 1)) + 1;
+  p = min(x, y);
+  q = max(x, y);
+}
+
+// Rotation (c, s) that zeroes a_pq of the 2x2 symmetric block.
+__device__ __forceinline__ void jacobi_rotation(double app, double aqq, double apq, double &c,
+                                                double &s) {
+  c = 1.0;
+  s = 0.0;
+  if (apq != 0.0) {
+    const double tau = (aqq - app) / (2.0 * apq);
+    const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+    c = 1.0 / sqrt(1.0 + t * t);
+    s = t * c;
+  }
+}
+
+// Sum of v over the workgroup (blockDim.x == 256); every thread gets it.
+__device__ __forceinline__ double block_sum_256(double v, double *red) {
+  const int tid = threadIdx.x;
+  red[tid] = v;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (tid < w) red[tid] += red[tid + w];
+    __syncthreads();
+  }
+  const double out = red[0];
+  __syncthreads();
+  return out;
+}
+
+// Largest matrix the LDS-resident eigensolver holds (A with a row stride
+// of n + 1 doubles: 132 KiB at n = 128).
+constexpr int kEigLds = 128;
+
+// Same algorithm as k_batched_syevj for n <= kEigLds, with A in LDS and
+// the rotations accumulated in W = V^T (rows p and q of W are contiguous,
+// so the update is coalesced) at Ww + offA[b] in global memory.  A is read,
+// not overwritten.  Cost per sweep: n^3 LDS operations and 2 n^2 doubles of
+// W traffic per step instead of 4 n^2 doubles of global A and V traffic.
+__global__ void __launch_bounds__(256) k_batched_syevj_lds(
+    const int64_t *__restrict__ ns, const double *__restrict__ A,
+    const int64_t *__restrict__ offA, double *__restrict__ Ww, double *__restrict__ Vout,
+    double *__restrict__ evals, const int64_t *__restrict__ offE, int max_sweeps) {
+  __shared__ double a[kEigLds * (kEigLds + 1)];
+  __shared__ double cs[kEigLds / 2], sn[kEigLds / 2];
+  __shared__ int pp[kEigLds / 2], qq[kEigLds / 2];
+  __shared__ double red[256];
+  const int64_t b = blockIdx.x;
+  const int n = (int)ns[b];
+  const int s = n + 1;
+  const double *ag = A + offA[b];
+  double *w = Ww + offA[b];
+  double *vo = Vout + offA[b];
+  double *ev = evals + offE[b];
+  const int tid = threadIdx.x, nt = blockDim.x;
+  for (int k = tid; k < n * n; k += nt) {
+    const int i = k / n, j = k % n;
+    a[i * s + j] = ag[k];
+    w[k] = (i == j) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  if (n <= 1) {
+    if (tid == 0 && n == 1) {
+      ev[0] = a[0];
+      vo[0] = 1.0;
+    }
+    return;
+  }
+  const int npad = n + (n & 1);
+  const int half = npad / 2;
+  for (int sweep = 0; sweep < max_sweeps; ++sweep) {
+    double off = 0.0, dia = 0.0;
+    for (int k = tid; k < n * n; k += nt) {
+      const int i = k / n, j = k % n;
+      const double x = a[i * s + j];
+      if (i != j)
+        off += x * x;
+      else
+        dia += x * x;
+    }
+    const double off_all = block_sum_256(off, red);
+    const double dia_all = block_sum_256(dia, red);
+    if (off_all <= 1e-30 * dia_all || off_all == 0.0) break;
+    for (int step = 0; step < npad - 1; ++step) {
+      for (int k = tid; k < half; k += nt) {
+        int p, q;
+        jacobi_pair(k, step, npad, p, q);
+        double c = 1.0, sv = 0.0;
+        if (q < n) jacobi_rotation(a[p * s + p], a[q * s + q], a[p * s + q], c, sv);
+        pp[k] = p;
+        qq[k] = q;
+        cs[k] = c;
+        sn[k] = sv;
+      }
+      __syncthreads();
+      // columns of A and rows of W
+      for (int x = tid; x < half * n; x += nt) {
+        const int k = x / n, i = x % n;
+        const int p = pp[k], q = qq[k];
+        if (q >= n || sn[k] == 0.0) continue;
+        const double c = cs[k], sv = sn[k];
+        const double aip = a[i * s + p], aiq = a[i * s + q];
+        a[i * s + p] = c * aip - sv * aiq;
+        a[i * s + q] = sv * aip + c * aiq;
+        const double wpi = w[(int64_t)p * n + i], wqi = w[(int64_t)q * n + i];
+        w[(int64_t)p * n + i] = c * wpi - sv * wqi;
+        w[(int64_t)q * n + i] = sv * wpi + c * wqi;
+      }
+      __syncthreads();
+      // rows of A
+      for (int x = tid; x < half * n; x += nt) {
+        const int k = x / n, j = x % n;
+        const int p = pp[k], q = qq[k];
+        if (q >= n || sn[k] == 0.0) continue;
+        const double c = cs[k], sv = sn[k];
+        const double apj = a[p * s + j], aqj = a[q * s + j];
+        a[p * s + j] = c * apj - sv * aqj;
+        a[q * s + j] = sv * apj + c * aqj;
+      }
+      __syncthreads();
+      for (int k = tid; k < half; k += nt) {
+        const int p = pp[k], q = qq[k];
+        if (q < n && sn[k] != 0.0) {
+          a[p * s + q] = 0.0;
+          a[q * s + p] = 0.0;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  // sort descending (rank by counting; ties by index); eigenvector i is row
+  // i of W
+  for (int i = tid; i < n; i += nt) {
+    const double di = a[i * s + i];
+    int rank = 0;
+    for (int j = 0; j < n; ++j) {
+      const double dj = a[j * s + j];
+      rank += (dj > di) || (dj == di && j < i);
+    }
+    ev[rank] = di;
+    for (int r = 0; r < n; ++r) vo[(int64_t)r * n + rank] = w[(int64_t)i * n + r];
+  }
+}
+
+// G_b <- 1.5 I - 0.5 G_b (n_b x n_b): the factor of one Newton-Schulz step
+// Q <- Q (3 I - Q^T Q) / 2 towards orthonormal columns.
+__global__ void k_ns_factor(const int64_t *__restrict__ ns, double *__restrict__ G,
+                            const int64_t *__restrict__ offG) {
+  const int64_t b = blockIdx.y;
+  const int64_t n = ns[b];
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n * n;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    const double g = G[offG[b] + k];
+    G[offG[b] + k] = ((k / n == k % n) ? 1.5 : 0.0) - 0.5 * g;
+  }
+}
+
 // Batched symmetric eigensolver: cyclic Jacobi with the round-robin
 // (tournament) ordering, one workgroup per matrix, matrices in global memory
 // (n x n row-major, overwritten).  Outputs eigenvalues sorted descending and
@@ -431,9 +594,25 @@ extern "C" int flux_batched_syevj(int64_t nb, const int64_t *ns, int64_t nmax, d
                                   void *stream) {
   FLUX_CHECK(nb >= 0 && nmax >= 0, "flux_batched_syevj: bad sizes");
   FLUX_CHECK(nmax <= 1024, "flux_batched_syevj: matrices above 1024 x 1024 are not supported");
+  FLUX_CHECK(nb <= 0x7fffffff, "flux_batched_syevj: too many matrices");
   if (nb == 0) return 0;
-  hipLaunchKernelGGL(k_batched_syevj, dim3((unsigned)nb), dim3(256), 0, as_stream(stream), ns,
-                     A, offA, Vwork, Vout, evals, offE, max_sweeps);
+  if (nmax <= kEigLds)
+    hipLaunchKernelGGL(k_batched_syevj_lds, dim3((unsigned)nb), dim3(256), 0,
+                       as_stream(stream), ns, A, offA, Vwork, Vout, evals, offE, max_sweeps);
+  else
+    hipLaunchKernelGGL(k_batched_syevj, dim3((unsigned)nb), dim3(256), 0, as_stream(stream),
+                       ns, A, offA, Vwork, Vout, evals, offE, max_sweeps);
+  FLUX_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int flux_batched_ns_factor(int64_t nb, const int64_t *ns, double *G,
+                                      const int64_t *offG, void *stream) {
+  FLUX_CHECK(nb >= 0 && nb <= 65535, "flux_batched_ns_factor: bad sizes");
+  if (nb == 0) return 0;
+  FLUX_CHECK(ns && G && offG, "flux_batched_ns_factor: NULL array");
+  hipLaunchKernelGGL(k_ns_factor, dim3(64, (unsigned)nb), dim3(256), 0, as_stream(stream), ns,
+                     G, offG);
   FLUX_LAUNCH_CHECK();
   return 0;
 }

@@ -1,26 +1,38 @@
-"""Timing and byte-count helpers (drop-in for flux.util, util.py:1-24)."""
+"""Timing and byte-count helpers (drop-in for flux.util, util.py:1-24).
+
+``tic()`` / ``toc()`` share one module-level stopwatch, as the reference's
+do; ``nbytes`` counts what an array holds, and for a SciPy CSR matrix the
+three arrays it is stored in (the byte rule the compressed tree build
+compares blocks with).
+"""
 import time
-from functools import singledispatch
 
 import scipy.sparse
 
-t0 = None
+
+class _Stopwatch:
+    """The module's stopwatch: the perf_counter value of the last tic()."""
+    started = None
+
+
+_watch = _Stopwatch()
 
 
 def tic():
-    global t0
-    t0 = time.perf_counter()
+    """Start, or restart, the stopwatch."""
+    _watch.started = time.perf_counter()
 
 
 def toc():
-    return time.perf_counter() - t0
+    """Seconds since the last tic()."""
+    if _watch.started is None:
+        raise RuntimeError("toc() called before tic()")
+    return time.perf_counter() - _watch.started
 
 
-@singledispatch
 def nbytes(arg):
+    """Bytes of an array; a CSR matrix counts its values, column indices and
+    row pointers."""
+    if isinstance(arg, scipy.sparse.csr_matrix):
+        return arg.data.nbytes + arg.indices.nbytes + arg.indptr.nbytes
     return arg.nbytes
-
-
-@nbytes.register
-def _(arg: scipy.sparse.csr_matrix):
-    return arg.data.nbytes + arg.indices.nbytes + arg.indptr.nbytes

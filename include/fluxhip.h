@@ -245,6 +245,13 @@ int flux_batched_syevj(int64_t nb, const int64_t *ns, int64_t nmax, double *A,
                        double *evals, const int64_t *offE, int max_sweeps,
                        void *stream);
 
+/* G_b <- 1.5 I - 0.5 G_b (n_b x n_b at G + offG[b]): with G_b = Q_b^T Q_b,
+ * the factor of one Newton-Schulz step Q_b <- Q_b G_b towards orthonormal
+ * columns (the second orthonormalisation pass of the rank sketch that
+ * replaces linalg.py:12's ARPACK call). */
+int flux_batched_ns_factor(int64_t nb, const int64_t *ns, double *G,
+                           const int64_t *offG, void *stream);
+
 /* T_b[:, j] = V_b[:, j] / sqrt(d_j) where d_j > cut * d_0, else 0;
  * kept[b] = number of kept columns. */
 int flux_scale_basis(int64_t nb, const int64_t *ns, const double *V,
