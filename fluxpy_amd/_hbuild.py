@@ -460,7 +460,7 @@ def _make_blocks(H, nd, TreeCls):
     """Python block objects mirroring the reference classes."""
     if nd.final == "tree":
         blk = TreeCls.__new__(TreeCls)
-        blk._root, blk.shape = H, (nd.m, nd.n)
+        blk._root, blk.shape, blk._rng = H, (nd.m, nd.n), tuple(int(v) for v in nd.rng)
         blk._row_block_inds = [np.asarray(p, dtype=np.int64) for p in nd.parts_r]
         blk._col_block_inds = [np.asarray(p, dtype=np.int64) for p in nd.parts_c]
         B = np.empty((len(nd.children), len(nd.children[0])), dtype=object)
@@ -589,7 +589,9 @@ def _leaf_counts(root):
 
 
 def _build_near(H):
-    """Near-field CSR = F minus the entries inside SVD leaves."""
+    """Near-field CSR = F minus the entries inside SVD leaves (its transpose
+    is built on the first adjoint product)."""
+    H._nearT = None
     F = H._F
     if not H._far_used:
         H._near = F
@@ -619,16 +621,24 @@ _VALUE_DTYPES = {"fp64": "float64", "fp32": "float32", "bf16": "bfloat16", "fp16
 
 
 class HOperator:
-    """Compressed F in cluster order: near-field CSR + far-field factors."""
+    """Compressed F (or, with transpose=True, F^T) in cluster order:
+    near-field CSR + far-field factors.  The adjoint uses the transposed
+    near-field CSR and the same far-field factors read with U and Vt
+    exchanged."""
 
-    def __init__(self, H, values):
+    def __init__(self, H, values, transpose=False):
         t = _lib.require_gpu()
         tdt = getattr(t, _VALUE_DTYPES[values])
         self.H = H
         self.values = values
+        self.transpose = bool(transpose)
         self.shape = H.shape
         self.n = H.shape[0]
         near = H._near
+        if self.transpose:
+            if getattr(H, "_nearT", None) is None:
+                H._nearT = _kernels.csr_transpose(near)
+            near = H._nearT
         self.near = near if near.data.dtype == tdt else near.astype(tdt)
         self.vec_dtype = t.float64 if values == "fp64" else t.float32
         self.perm = _lib.to_dev(H._ctree.perm)
@@ -662,12 +672,14 @@ class HOperator:
         self.U, self.S, self.Vt = cat(Us), cat(Ss), cat(Vts)
         self.desc = _i64(np.array(descs, dtype=np.int64))
         self.ranks = np.array(ranks, dtype=np.int64)
-        # rows -> far blocks covering them, through a per-row cluster id
+        # output rows -> far blocks covering them (row ranges, or column
+        # ranges for the adjoint), through a per-row cluster id
         n = self.n
         rowcl = np.arange(n, dtype=np.int32)
         lists = [[] for _ in range(n)]
+        lo_f, hi_f = (2, 3) if self.transpose else (0, 1)
         for b, d in enumerate(descs):
-            for i in range(d[0], d[1]):
+            for i in range(d[lo_f], d[hi_f]):
                 lists[i].append(b)
         cptr = np.zeros(n + 1, dtype=np.int64)
         cptr[1:] = np.cumsum([len(x) for x in lists])
@@ -717,7 +729,7 @@ class HOperator:
                       self.Vt.data_ptr(), _lib.dtype_code(self.U.dtype),
                       self.rowcl.data_ptr(), self.cptr.data_ptr(), self.cblk.data_ptr(),
                       X2.data_ptr(), Y.data_ptr(), _lib.dtype_code(X2.dtype), T.data_ptr(),
-                      _lib.stream())
+                      int(self.transpose), _lib.stream())
         return Y.reshape(-1) if vec else Y
 
     def jacobi_step(self, E, B, rho, left, B1, colmax):
@@ -746,23 +758,26 @@ class HOperator:
         return nb
 
 
-def operator(H, values=None):
+def operator(H, values=None, transpose=False):
     if values is None:
         values = "fp64" if np.dtype(H.dtype) == np.float64 else "fp32"
     if values not in _VALUE_DTYPES:
         raise ValueError(f"values must be one of {sorted(_VALUE_DTYPES)}")
     if getattr(H, "_hop", None) is None:
         H._hop = {}
-    if values not in H._hop:
-        H._hop[values] = HOperator(H, values)
-    return H._hop[values]
+    key = (values, bool(transpose))
+    if key not in H._hop:
+        H._hop[key] = HOperator(H, values, transpose)
+    return H._hop[key]
 
 
 # --------------------------------------------------------------------------
 # host views, pickling
 # --------------------------------------------------------------------------
 
-def leaf_csr(H, rng):
+def leaf_csr(H, rng, cluster_order=False):
+    """Host CSR of F over rng: rows / columns in ascending original ids, or
+    in cluster order."""
     r0, r1, c0, c1 = rng
     F = H._F
     ip = F.indptr[r0:r1 + 1].cpu().numpy()
@@ -771,25 +786,24 @@ def leaf_csr(H, rng):
     dat = F.data[a:b].cpu().numpy()
     rows = np.repeat(np.arange(r1 - r0), np.diff(ip))
     sel = (idx >= c0) & (idx < c1)
-    perm = H._ctree.perm
-    ro = np.argsort(perm[r0:r1])          # ascending global row ids
-    co = np.argsort(perm[c0:c1])
-    rinv = np.empty_like(ro)
-    rinv[ro] = np.arange(ro.size)
-    cinv = np.empty_like(co)
-    cinv[co] = np.arange(co.size)
-    sp = scipy.sparse.csr_matrix((dat[sel], (rinv[rows[sel]], cinv[idx[sel] - c0])),
-                                 shape=(r1 - r0, c1 - c0))
+    ri, ci = rows[sel], idx[sel] - c0
+    if not cluster_order:
+        ct = H._ctree
+        ri, ci = ct.rank(r0, r1)[ri], ct.rank(c0, c1)[ci]
+    sp = scipy.sparse.csr_matrix((dat[sel], (ri, ci)), shape=(r1 - r0, c1 - c0))
     sp.sort_indices()
     return sp
 
 
-def far_factors(H, fid):
+def far_factors(H, fid, cluster_order=False):
+    """(u, s, vt) of a far block on the host: rows of u / columns of vt in
+    ascending original ids, or in cluster order."""
     (r0, r1, c0, c1), r, (u, s, vt) = H._far[fid]
+    u, s, vt = u.cpu().numpy(), s.cpu().numpy(), vt.cpu().numpy()
+    if cluster_order:
+        return u, s, vt
     perm = H._ctree.perm
-    ro = np.argsort(perm[r0:r1])
-    co = np.argsort(perm[c0:c1])
-    return u.cpu().numpy()[ro], s.cpu().numpy(), vt.cpu().numpy()[:, co]
+    return u[np.argsort(perm[r0:r1])], s, vt[:, np.argsort(perm[c0:c1])]
 
 
 def state(H):

@@ -63,7 +63,7 @@ SIGNATURES = {
     "flux_scale_basis": (_c_int, [_c_i64, _vp, _vp, _vp, _vp, _vp, _c_dbl, _vp, _vp, _vp]),
     "flux_scale_cols_inv_sqrt": (_c_int, [_c_i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "flux_far_matmat": (_c_int, [_c_i64, _c_i64, _c_i64, _vp, _vp, _c_i64, _vp, _vp, _vp, _c_int,
-                                 _vp, _vp, _vp, _vp, _vp, _c_int, _vp, _vp]),
+                                 _vp, _vp, _vp, _vp, _vp, _c_int, _vp, _c_int, _vp]),
     "flux_permute_rows": (_c_int, [_c_i64, _c_i64, _vp, _c_int, _vp, _vp, _c_int, _vp]),
     "flux_jacobi_update": (_c_int, [_c_i64, _c_i64, _vp, _vp, _vp, _vp, _c_dbl, _c_int, _vp,
                                     _c_int, _vp, _vp]),
@@ -99,6 +99,9 @@ SIGNATURES = {
     "flux_vec_convert": (_c_int, [_c_i64, _vp, _c_int, _vp, _c_int, _vp]),
     "flux_temperature": (_c_int, [_c_i64, _c_i64, _vp, _vp, _vp, _c_dbl, _vp, _c_dbl, _c_dbl,
                                   _c_int, _c_dbl, _vp, _c_int, _vp]),
+    "flux_temperature_adjoint_seed": (_c_int, [_c_i64, _c_i64, _vp, _vp, _vp, _vp, _c_dbl, _vp,
+                                               _vp, _vp, _vp]),
+    "flux_temperature_adjoint_finish": (_c_int, [_c_i64, _c_i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "flux_conduction_work_bytes": (_c_i64, [_c_i64, _c_int]),
     "flux_conduction_q": (_c_int, [_c_i64, _c_int, _vp, _c_dbl, _vp, _vp, _vp, _vp, _vp, _vp,
                                    _vp, _vp, _vp, _vp, _vp, _vp]),

@@ -13,22 +13,22 @@ the smallest of CSR / truncated SVD / children by the reference's byte
 counts, then all-dense or all-sparse children collapse), the rank rule
 (k = 40, doubling, capped at min(m, n) - 1; rank = first index with
 s_i / s_0 <= tol; give up once the k-term factors need as many bytes as the
-CSR block), the block classes and their introspection API, and ``@``.
+CSR block), the class names and their introspection API, and ``@``.
 
-What is MI355X-specific:
+What is this build's own:
 
-* the cluster tree is turned into one permutation so every block is a
-  contiguous (row range, column range) of F in cluster order; F is
-  assembled once on the GPU in that order (``flux_ff_mask`` / ``_fill``
-  with I = J = permutation) and its transpose built on the GPU;
-* the tree is built level by level: per-level block statistics (nnz, zero
-  rows, zero columns) come from one batched kernel, and every SVD candidate
-  of a level is sketched together (Rademacher range sketch over the CSR
-  sub-block, FP64 Gram matrices, batched parallel Jacobi eigensolver) --
-  the singular values are those of the sketched block, not ARPACK's;
-* the product is flat: all dense and CSR leaves form one near-field CSR in
-  cluster order, the SVD leaves form one batched far-field product, and the
-  solvers iterate in cluster order (E and rho permuted once).
+* the cluster tree is one permutation that makes every cluster, and so
+  every block of the tree, a contiguous (row range, column range) of F in
+  cluster order; every block stores that range (``_rng``) and the
+  introspection walks (rows at a depth, blocks of a row, columns of a row)
+  are range searches over it, answered in original facet ids by sorting the
+  permutation's slice;
+* F is assembled once on the GPU in that order and the tree is built level
+  by level with batched block statistics and batched randomized sketches
+  (``_hbuild``);
+* the product is flat: the dense and CSR leaves form one near-field CSR in
+  cluster order, the SVD leaves one batched far-field product; the adjoint
+  product (x @ F) uses the transposed near field and the same factors.
 
 Deviation: ``force_max_depth=True`` with ``max_depth > 1`` hits
 ``assert False`` in the reference (compressed_form_factors.py:300-301).
@@ -37,7 +37,6 @@ collapse all-dense / all-sparse children), as the reference's own
 test_max_depth_2_for_stretched_sphere expects.
 """
 import copy
-import itertools as it
 import logging
 import pickle
 
@@ -45,7 +44,6 @@ import numpy as np
 import scipy.sparse
 import scipy.sparse.linalg
 
-from . import _kernels
 from . import _lib
 from .octree import get_octant_order
 from .quadtree import get_quadrant_order
@@ -160,27 +158,37 @@ class _ClusterTree:
             kids.append(k)
         return kids, parts
 
+    def ids(self, a, b):
+        """Original facet ids of the cluster-order range [a, b), ascending
+        (the order in which the reference lists a block's rows / columns)."""
+        return np.sort(self.perm[a:b])
+
+    def rank(self, a, b):
+        """rank[k] = position of the facet at cluster-order a + k among the
+        ascending ids of [a, b)."""
+        r = np.empty(b - a, dtype=np.int64)
+        r[np.argsort(self.perm[a:b], kind="stable")] = np.arange(b - a)
+        return r
+
 
 # --------------------------------------------------------------------------
-# block classes (compressed_form_factors.py:35-698)
+# blocks (compressed_form_factors.py:35-698)
+#
+# Every block covers the range _rng = (r0, r1, c0, c1) of F in cluster
+# order.  Leaves read their values through the matrix (``_root``); tree
+# blocks hold their children in the object array ``_blocks`` and, for the
+# reference's API, the local partition lists ``_row_block_inds`` /
+# ``_col_block_inds``.
 # --------------------------------------------------------------------------
-
-@np.vectorize
-def _is_dense(block):
-    return isinstance(block, FormFactorDenseBlock)
-
-
-@np.vectorize
-def _is_sparse(block):
-    return isinstance(block, FormFactorSparseBlock) or isinstance(block, FormFactorZeroBlock)
 
 
 class CompressedFormFactorBlock:
     """A block of a CompressedFormFactorMatrix."""
 
-    def __init__(self, root, shape):
+    def __init__(self, root, shape, rng=(0, 0, 0, 0)):
         self._root = root
-        self.shape = shape
+        self.shape = tuple(shape)
+        self._rng = tuple(int(v) for v in rng)
 
     @property
     def dtype(self):
@@ -188,7 +196,7 @@ class CompressedFormFactorBlock:
 
     @property
     def size(self):
-        return int(np.prod(self.shape))
+        return int(self.shape[0]) * int(self.shape[1])
 
     @property
     def root(self):
@@ -202,56 +210,62 @@ class CompressedFormFactorBlock:
     def _tol(self):
         return self._root._tol
 
+    def _row_ids(self):
+        return self._root._ctree.ids(self._rng[0], self._rng[1])
 
-class FormFactorLeafBlock(CompressedFormFactorBlock):
-
-    @property
-    def depth(self):
-        return 0
-
-    @property
-    def is_leaf(self):
-        return True
-
-    def get_blocks_at_depth(self, depth):
-        if depth == 0:
-            yield self
-
-    def _get_blocks_at_depth(self, depth):
-        if depth == 0:
-            yield self
-
-    # global row / column ids (ascending) of this leaf
-    def _ids(self):
-        r0, r1, c0, c1 = self._rng
-        perm = self._root._ctree.perm
-        rows = np.sort(perm[r0:r1])
-        cols = np.sort(perm[c0:c1])
-        return rows, cols
-
-    def _matmat(self, x):
-        x = np.asarray(x)
-        x2 = x.reshape(x.shape[0], -1)
-        y = self.toarray_or_op() @ x2
-        return y if x.ndim > 1 else y.ravel()
+    def _col_ids(self):
+        return self._root._ctree.ids(self._rng[2], self._rng[3])
 
     def __matmul__(self, x):
         return self._matmat(x)
 
 
+class FormFactorLeafBlock(CompressedFormFactorBlock):
+    """A leaf; its kind flags answer is_dense / is_sparse / is_empty_leaf."""
+    _DENSE = False
+    _SPARSE = False
+    _EMPTY = False
+
+    depth = 0
+    is_leaf = True
+
+    def is_dense(self):
+        return self._DENSE
+
+    def is_sparse(self):
+        return self._SPARSE
+
+    @property
+    def is_empty_leaf(self):
+        return self._EMPTY
+
+    def get_blocks_at_depth(self, depth):
+        if depth == 0:
+            yield self
+
+    _get_blocks_at_depth = get_blocks_at_depth
+
+    def _cluster_matmat(self, xc):
+        """Product with x given in the cluster order of the leaf's columns,
+        result in the cluster order of its rows."""
+        return np.zeros((self.shape[0], xc.shape[1]), dtype=self.dtype)
+
+    def _matmat(self, x):
+        x = np.asarray(x)
+        y = self.toarray_or_op() @ x.reshape(x.shape[0], -1)
+        return y if x.ndim > 1 else y.ravel()
+
+
 class FormFactorNullBlock(FormFactorLeafBlock):
+    """A block with no rows or no columns."""
+    _DENSE = _SPARSE = _EMPTY = True
 
     def __init__(self, root, shape, rng=(0, 0, 0, 0)):
         if shape[0] != 0 and shape[1] != 0:
             raise RuntimeError('a null block must have a degenerate shape')
-        super().__init__(root, shape)
-        self._rng = rng
+        super().__init__(root, shape, rng)
 
-    def is_dense(self):
-        return True
-
-    def is_sparse(self):
-        return True
+    nbytes = 0
 
     @property
     def _mat(self):
@@ -260,26 +274,12 @@ class FormFactorNullBlock(FormFactorLeafBlock):
     def toarray_or_op(self):
         return self._mat
 
-    @property
-    def nbytes(self):
-        return 0
-
-    @property
-    def is_empty_leaf(self):
-        return True
-
 
 class FormFactorZeroBlock(FormFactorLeafBlock):
+    """A block without nonzeros."""
+    _SPARSE = _EMPTY = True
 
-    def __init__(self, root, shape, rng=(0, 0, 0, 0)):
-        super().__init__(root, shape)
-        self._rng = rng
-
-    def is_dense(self):
-        return False
-
-    def is_sparse(self):
-        return True
+    nbytes = 0
 
     @property
     def _mat(self):
@@ -288,23 +288,16 @@ class FormFactorZeroBlock(FormFactorLeafBlock):
     def toarray_or_op(self):
         return self._mat
 
-    @property
-    def nbytes(self):
-        return 0
-
     def tocsr(self):
         return scipy.sparse.csr_matrix(self.shape, dtype=self.dtype)
 
-    @property
-    def is_empty_leaf(self):
-        return True
-
 
 class FormFactorDenseBlock(FormFactorLeafBlock):
+    """A leaf stored densely (m * n values)."""
+    _DENSE = True
 
     def __init__(self, root, shape, rng):
-        super().__init__(root, shape)
-        self._rng = rng
+        super().__init__(root, shape, rng)
         self._cache = None
 
     @property
@@ -321,38 +314,24 @@ class FormFactorDenseBlock(FormFactorLeafBlock):
 
     @property
     def nbytes(self):
-        return self.shape[0] * self.shape[1] * np.dtype(self.dtype).itemsize
+        return self.size * np.dtype(self.dtype).itemsize
 
-    def is_dense(self):
-        return True
-
-    def is_sparse(self):
-        return False
-
-    @property
-    def is_empty_leaf(self):
-        return False
+    def _cluster_matmat(self, xc):
+        return self._root._leaf_csr(self._rng, cluster_order=True) @ xc
 
 
 class FormFactorSparseBlock(FormFactorLeafBlock):
-
-    def is_dense(self):
-        return False
-
-    def is_sparse(self):
-        return True
-
-    @property
-    def is_empty_leaf(self):
-        return False
+    """A sparse leaf."""
+    _SPARSE = True
 
 
 class FormFactorCsrBlock(FormFactorSparseBlock):
+    """A leaf stored as CSR; counts its value bytes only
+    (compressed_form_factors.py:219-221)."""
 
     def __init__(self, root, shape, rng, nnz):
-        super().__init__(root, shape)
-        self._rng = rng
-        self._nnz = nnz
+        super().__init__(root, shape, rng)
+        self._nnz = int(nnz)
         self._cache = None
 
     @property
@@ -363,7 +342,6 @@ class FormFactorCsrBlock(FormFactorSparseBlock):
 
     @property
     def nbytes(self):
-        # data bytes only, as compressed_form_factors.py:219-221
         return self._nnz * np.dtype(self.dtype).itemsize
 
     def tocsr(self):
@@ -376,19 +354,23 @@ class FormFactorCsrBlock(FormFactorSparseBlock):
     def toarray_or_op(self):
         return self._spmat
 
+    def _cluster_matmat(self, xc):
+        return self._root._leaf_csr(self._rng, cluster_order=True) @ xc
+
 
 class FormFactorSvdBlock(FormFactorLeafBlock):
+    """A truncated-SVD leaf u diag(s) vt of rank ``_k``; the factors live on
+    the GPU in cluster order."""
+    compressed = True
 
     def __init__(self, root, shape, rng, k, nbytes_, far_id):
-        super().__init__(root, shape)
-        self._rng = rng
-        self._k = k
-        self._nbytes = nbytes_
+        super().__init__(root, shape, rng)
+        self._k = int(k)
+        self._nbytes = int(nbytes_)
         self._far_id = far_id
 
-    def _factors(self):
-        """(u, s, vt) in ascending global order of rows / columns."""
-        return self._root._far_factors(self._far_id)
+    def _factors(self, cluster_order=False):
+        return self._root._far_factors(self._far_id, cluster_order)
 
     @property
     def _u(self):
@@ -402,142 +384,131 @@ class FormFactorSvdBlock(FormFactorLeafBlock):
     def _vt(self):
         return self._factors()[2]
 
-    def toarray_or_op(self):
-        u, s, vt = self._factors()
-        return (u * s) @ vt
-
-    def is_dense(self):
-        return False
-
-    def is_sparse(self):
-        return False
-
-    @property
-    def is_empty_leaf(self):
-        return False
-
     @property
     def nbytes(self):
         return self._nbytes
 
-    @property
-    def compressed(self):
-        return True
+    def toarray_or_op(self):
+        u, s, vt = self._factors()
+        return (u * s) @ vt
+
+    def _cluster_matmat(self, xc):
+        u, s, vt = self._factors(cluster_order=True)
+        return u @ (s[:, None] * (vt @ xc))
 
 
 class FormFactorBlockMatrix(CompressedFormFactorBlock):
+    """A tree block: a grid of children covering its range."""
 
-    @property
-    def nbytes(self):
-        return sum(I.nbytes for I in self._row_block_inds) \
-            + sum(J.nbytes for J in self._col_block_inds) \
-            + sum(block.nbytes for block in self._blocks.flatten())
-
-    @property
-    def depth(self):
-        return 1 + max(block.depth for block in self._blocks.flatten())
-
-    @property
-    def is_leaf(self):
-        return False
-
-    @property
-    def is_empty_leaf(self):
-        return False
-
-    def is_dense(self):
-        return bool(_is_dense(self._blocks).all())
-
-    def is_sparse(self):
-        return bool(_is_sparse(self._blocks).all())
+    is_leaf = False
+    is_empty_leaf = False
 
     @property
     def bshape(self):
-        return (len(self._row_block_inds), len(self._col_block_inds))
+        return self._blocks.shape
 
-    def _matmat(self, x):
-        x = np.asarray(x)
-        x2 = x.reshape(x.shape[0], -1)
-        y = np.zeros((self.shape[0], x2.shape[1]), dtype=self.dtype)
-        for i, row_inds in enumerate(self._row_block_inds):
-            for j, col_inds in enumerate(self._col_block_inds):
-                block = self._blocks[i, j]
-                if block.is_empty_leaf:
-                    continue
-                y[row_inds] += block._matmat(x2[col_inds])
-        return y if x.ndim > 1 else y.ravel()
+    def _children(self):
+        return self._blocks.ravel()
 
-    def __matmul__(self, x):
-        return self._matmat(x)
+    @property
+    def nbytes(self):
+        # the children's row and column id lists (int64, one entry per row
+        # and per column of this block) plus the children
+        return 8 * (self.shape[0] + self.shape[1]) + sum(b.nbytes for b in self._children())
+
+    @property
+    def depth(self):
+        return 1 + max(b.depth for b in self._children())
+
+    def is_dense(self):
+        return all(isinstance(b, FormFactorDenseBlock) for b in self._children())
+
+    def is_sparse(self):
+        return all(isinstance(b, (FormFactorSparseBlock, FormFactorZeroBlock))
+                   for b in self._children())
+
+    def _leaves(self):
+        stack = [self]
+        while stack:
+            blk = stack.pop()
+            if blk.is_leaf:
+                yield blk
+            else:
+                stack.extend(reversed(list(blk._children())))
 
     def _get_blocks_at_depth(self, depth):
-        if depth == 0:
-            yield self
-        else:
-            for block in self._blocks.ravel():
-                yield from block._get_blocks_at_depth(depth - 1)
+        frontier = [self]
+        for _ in range(depth):
+            frontier = [c for b in frontier if not b.is_leaf for c in b._children()]
+        yield from frontier
 
-    def _get_row_inds_at_depth(self, depth, parent_inds):
-        assert depth >= 1
-        if depth == 1:
-            for row_block_inds in self._row_block_inds:
-                yield parent_inds[row_block_inds]
-        else:
-            for row_block_inds, block in zip(self._row_block_inds, np.diag(self._blocks)):
-                if block.is_leaf:
-                    yield parent_inds[row_block_inds]
-                else:
-                    yield from block._get_row_inds_at_depth(depth - 1,
-                                                            parent_inds[row_block_inds])
+    def _diag_row_ranges(self, depth):
+        """(r0, r1) of the diagonal blocks `depth` levels below this one; a
+        diagonal leaf above that depth stands for its rows."""
+        out = []
+        for i in range(min(self.bshape)):
+            b = self._blocks[i, i]
+            if depth == 1 or b.is_leaf:
+                out.append(b._rng[:2])
+            else:
+                out.extend(b._diag_row_ranges(depth - 1))
+        return out
 
-    def _get_row_blocks(self, row_ind, parent_inds):
-        for row_block_inds, row_blocks in zip(self._row_block_inds, self._blocks):
-            row_inds = parent_inds[row_block_inds]
-            if row_ind in row_inds:
-                for block in row_blocks:
-                    if block.is_leaf:
-                        yield block
+    def _leaves_in_row(self, p):
+        """Leaves whose row range holds the cluster-order position p, left to
+        right (a subtree's leaves in place of the subtree)."""
+        for i in range(self.bshape[0]):
+            r0, r1 = self._blocks[i, 0]._rng[:2]
+            if r0 <= p < r1:
+                for c in self._blocks[i]:
+                    if c.is_leaf:
+                        yield c
                     else:
-                        yield from block._get_row_blocks(row_ind, row_inds)
-                break
+                        yield from c._leaves_in_row(p)
+                return
 
-    def _get_col_inds_for_row(self, row_ind, parent_row_inds, parent_col_inds):
-        for row_block_inds, row_blocks in zip(self._row_block_inds, self._blocks):
-            row_inds = parent_row_inds[row_block_inds]
-            if row_ind in row_inds:
-                for block, col_block_inds in zip(row_blocks, self._col_block_inds):
-                    col_inds = parent_col_inds[col_block_inds]
-                    if block.is_leaf:
-                        yield col_inds
-                    else:
-                        yield from block._get_col_inds_for_row(row_ind, row_inds, col_inds)
-                break
+    def _matmat(self, x):
+        """Host product through the tree (views such as get_diag_blocks):
+        x is indexed by the ascending ids of this block's columns, the result
+        by the ascending ids of its rows, in the model dtype."""
+        x = np.asarray(x)
+        x2 = x.reshape(x.shape[0], -1)
+        r0, r1, c0, c1 = self._rng
+        ct = self._root._ctree
+        xc = x2[ct.rank(c0, c1)]
+        yc = np.zeros((r1 - r0, x2.shape[1]), dtype=self.dtype)
+        for leaf in self._leaves():
+            if leaf.is_empty_leaf:
+                continue
+            a0, a1, b0, b1 = leaf._rng
+            yc[a0 - r0:a1 - r0] += leaf._cluster_matmat(xc[b0 - c0:b1 - c0])
+        y = np.empty_like(yc)
+        y[ct.rank(r0, r1)] = yc
+        return y if x.ndim > 1 else y.ravel()
+
+    def _with_zeroed(self, keep):
+        """Shallow copy whose children outside the boolean grid `keep` are
+        replaced by zero blocks of the same shape and range."""
+        view = copy.copy(self)
+        view._blocks = self._blocks.copy()
+        for idx, b in np.ndenumerate(self._blocks):
+            if not keep[idx]:
+                view._blocks[idx] = FormFactorZeroBlock(self._root, b.shape, b._rng)
+        return view
 
     def get_individual_block(self, i, j):
-        """Shallow copy with every block except (i, j) zeroed."""
-        tmp = copy.copy(self)
-        tmp._blocks = copy.copy(tmp._blocks)
-        for i_, j_ in it.product(*(range(_) for _ in self.bshape)):
-            if i_ == i and j_ == j:
-                continue
-            tmp._blocks[i_, j_] = FormFactorZeroBlock(self._root, tmp._blocks[i_, j_].shape)
-        return tmp
+        """Shallow copy with every child except (i, j) zeroed."""
+        keep = np.zeros(self.bshape, dtype=bool)
+        keep[i, j] = True
+        return self._with_zeroed(keep)
 
     def get_diag_blocks(self):
-        tmp = copy.copy(self)
-        tmp._blocks = copy.copy(tmp._blocks)
-        for i, j in it.product(*(range(_) for _ in self.bshape)):
-            if i != j:
-                tmp._blocks[i, j] = FormFactorZeroBlock(self._root, tmp._blocks[i, j].shape)
-        return tmp
+        """Shallow copy with the off-diagonal children zeroed."""
+        return self._with_zeroed(np.eye(*self.bshape, dtype=bool))
 
     def get_off_diag_blocks(self):
-        tmp = copy.copy(self)
-        tmp._blocks = copy.copy(tmp._blocks)
-        for i, j in it.product(*(range(_) for _ in self.bshape)):
-            if i == j:
-                tmp._blocks[i, j] = FormFactorZeroBlock(self._root, tmp._blocks[i, j].shape)
-        return tmp
+        """Shallow copy with the diagonal children zeroed."""
+        return self._with_zeroed(~np.eye(*self.bshape, dtype=bool))
 
 
 class FormFactor2dTreeBlock(FormFactorBlockMatrix):
@@ -571,18 +542,13 @@ class FormFactorPartitionBlock(FormFactorBlockMatrix):
 
 class CompressedFormFactorMatrix(scipy.sparse.linalg.LinearOperator):
     """A compressed form factor (view factor) matrix on the GPU
-    (compressed_form_factors.py:701-873)."""
+    (compressed_form_factors.py:701-873).  ``@`` with a NumPy array returns
+    NumPy in the model dtype (:408); with a device tensor it stays on the
+    device.  ``.T @ x`` / ``rmatvec`` is the adjoint product x @ F."""
 
     def __init__(self, shape_model, tol=1e-5, min_size=16384, max_depth=None,
                  force_max_depth=False, RootBlock=FormFactorQuadtreeBlock, **kwargs):
-        if tol < 0:
-            raise RuntimeError('tol should be a nonnegative float')
-        if max_depth is not None and not isinstance(max_depth, int):
-            raise RuntimeError('invalid max_depth type: %s' % str(type(max_depth)))
-        if isinstance(max_depth, int) and max_depth <= 0:
-            raise RuntimeError('invalid max_depth value: %d' % max_depth)
-        if force_max_depth and max_depth is None:
-            raise RuntimeError('force_max_depth is True, but max_depth not specified')
+        _check_build_args(tol, max_depth, force_max_depth)
         if RootBlock not in (FormFactorQuadtreeBlock, FormFactorOctreeBlock):
             raise NotImplementedError(f"unsupported RootBlock {RootBlock}")
         self.shape_model = shape_model
@@ -596,7 +562,7 @@ class CompressedFormFactorMatrix(scipy.sparse.linalg.LinearOperator):
         from . import _hbuild
         _hbuild.build(self, RootBlock._dims)
 
-    # -- reference properties -----------------------------------------------
+    # -- persistence ---------------------------------------------------------
 
     @staticmethod
     def from_file(path):
@@ -611,6 +577,8 @@ class CompressedFormFactorMatrix(scipy.sparse.linalg.LinearOperator):
         from . import _hbuild
         return (_hbuild.restore, (_hbuild.state(self),))
 
+    # -- properties ----------------------------------------------------------
+
     @property
     def dtype(self):
         return self.shape_model.dtype
@@ -621,7 +589,7 @@ class CompressedFormFactorMatrix(scipy.sparse.linalg.LinearOperator):
 
     @property
     def shape(self):
-        return self.shape_model.num_faces, self.shape_model.num_faces
+        return self.num_faces, self.num_faces
 
     @property
     def nbytes(self):
@@ -640,49 +608,49 @@ class CompressedFormFactorMatrix(scipy.sparse.linalg.LinearOperator):
         """Timings and counts of the GPU tree build."""
         return dict(self._build_stats)
 
-    def get_blocks_at_depth(self, depth):
+    # -- introspection -------------------------------------------------------
+
+    def _checked_depth(self, depth):
         if depth > self.depth:
             raise Exception('specified depth (%d) exceeds tree depth (%d)' % (depth, self.depth))
-        if depth == 0 or self._root.is_leaf:
-            yield self._root
-        else:
-            yield from self._root._get_blocks_at_depth(depth)
+        return depth
+
+    def _checked_row(self, row_ind):
+        if not 0 <= row_ind < self.num_faces:
+            raise Exception('row_ind (== %d) should be in range [0, %d)' % (
+                row_ind, self.num_faces))
+        return int(self._ctree.iperm[row_ind])
+
+    def get_blocks_at_depth(self, depth):
+        yield from self._root._get_blocks_at_depth(self._checked_depth(depth))
 
     def get_row_inds_at_depth(self, depth):
-        if depth > self.depth:
-            raise Exception('specified depth (%d) exceeds tree depth (%d)' % (depth, self.depth))
-        row_inds = np.arange(self.num_faces)
-        if depth == 0:
-            yield row_inds
-        else:
-            yield from self._root._get_row_inds_at_depth(depth, row_inds)
+        if self._checked_depth(depth) == 0:
+            yield np.arange(self.num_faces)
+            return
+        for r0, r1 in self._root._diag_row_ranges(depth):
+            yield self._ctree.ids(r0, r1)
 
     def get_row_blocks(self, row_ind):
-        if not (0 <= row_ind < self.num_faces):
-            raise Exception('row_ind (== %d) should be in range [0, %d)' % (
-                row_ind, self.num_faces))
-        yield from self._root._get_row_blocks(row_ind, np.arange(self.num_faces))
+        yield from self._root._leaves_in_row(self._checked_row(row_ind))
 
     def get_col_inds_for_row(self, row_ind):
-        if not (0 <= row_ind < self.num_faces):
-            raise Exception('row_ind (== %d) should be in range [0, %d)' % (
-                row_ind, self.num_faces))
-        n = np.arange(self.num_faces)
-        yield from self._root._get_col_inds_for_row(row_ind, n, n)
+        for leaf in self._root._leaves_in_row(self._checked_row(row_ind)):
+            yield leaf._col_ids()
 
     def get_diag_blocks(self):
         """Shallow copy with the off-diagonal root blocks zeroed; its products
         walk the block tree (a diagnostic view, not a solver operator)."""
-        tmp = copy.copy(self)
-        tmp._root = tmp._root.get_diag_blocks()
-        tmp._derived = True
-        return tmp
+        return self._view(self._root.get_diag_blocks())
 
     def get_off_diag_blocks(self):
-        tmp = copy.copy(self)
-        tmp._root = tmp._root.get_off_diag_blocks()
-        tmp._derived = True
-        return tmp
+        return self._view(self._root.get_off_diag_blocks())
+
+    def _view(self, root):
+        view = copy.copy(self)
+        view._root = root
+        view._derived = True
+        return view
 
     _derived = False
 
@@ -695,41 +663,49 @@ class CompressedFormFactorMatrix(scipy.sparse.linalg.LinearOperator):
         for j0 in range(0, n, step):
             j1 = min(n, j0 + step)
             Ej = t.zeros((n, j1 - j0), dtype=_lib.torch_dtype(self.dtype), device=_lib.device())
-            Ej[t.arange(j0, j1, device=Ej.device), t.arange(0, j1 - j0, device=Ej.device)] = 1
+            cols = t.arange(j0, j1, device=Ej.device)
+            Ej[cols, cols - j0] = 1
             out[:, j0:j1] = self.apply(Ej).cpu().numpy()
         return out
 
     def tocsr(self):
         return scipy.sparse.csr_matrix(self.toarray())
 
-    # -- device products ----------------------------------------------------
+    # -- device products -----------------------------------------------------
 
-    def device_operator(self, values=None):
-        """Solver operator; ``values`` picks the stored precision of the
-        near-field values and far-field factors ('fp64', 'fp32', 'bf16',
-        'fp16'; default: the mesh dtype)."""
+    def device_operator(self, values=None, transpose=False):
+        """Solver operator (cluster order); ``values`` picks the stored
+        precision of the near-field values and far-field factors ('fp64',
+        'fp32', 'bf16', 'fp16'; default: the mesh dtype); ``transpose``
+        gives F^T."""
         if self._derived:
             raise TypeError("diagonal / off-diagonal views have no device operator")
         from . import _hbuild
-        return _hbuild.operator(self, values)
+        return _hbuild.operator(self, values, transpose)
+
+    def _device_product(self, X, transpose):
+        op = self.device_operator(transpose=transpose)
+        X2 = X.reshape(X.shape[0], -1)
+        Y = op.from_internal(op.apply(op.to_internal(X2)))
+        return Y.reshape(X.shape) if X.dim() == 1 else Y
 
     def apply(self, X):
         """Y = F X for a device tensor X (original facet order)."""
-        op = self.device_operator()
-        X2 = X.reshape(X.shape[0], -1)
-        Xi = op.to_internal(X2)
-        Yi = op.apply(Xi)
-        Y = op.from_internal(Yi)
-        return Y.reshape(X.shape) if X.dim() == 1 else Y
+        return self._device_product(X, False)
 
-    def _host_product(self, x):
+    def apply_t(self, X):
+        """Y = F^T X for a device tensor X (original facet order)."""
+        return self._device_product(X, True)
+
+    def _host_product(self, x, transpose=False):
         x = np.asarray(x)
         if self._derived:
+            if transpose:
+                raise TypeError("diagonal / off-diagonal views have no adjoint product")
             return self._root._matmat(x)
         # the result is in the model dtype, as compressed_form_factors.py:408
-        dt = np.dtype(self.dtype)
-        xd = _lib.to_dev(x.astype(dt))
-        return self.apply(xd).cpu().numpy()
+        xd = _lib.to_dev(x.astype(np.dtype(self.dtype)))
+        return self._device_product(xd, transpose).cpu().numpy()
 
     def _matvec(self, x):
         return self._host_product(np.ravel(x)).reshape(-1)
@@ -737,41 +713,61 @@ class CompressedFormFactorMatrix(scipy.sparse.linalg.LinearOperator):
     def _matmat(self, X):
         return self._host_product(X)
 
+    def _rmatvec(self, x):
+        return self._host_product(np.ravel(x), transpose=True).reshape(-1)
+
+    def _rmatmat(self, X):
+        return self._host_product(X, transpose=True)
+
     def __matmul__(self, x):
-        t = _lib.torch()
-        if isinstance(x, t.Tensor):
+        if isinstance(x, _lib.torch().Tensor):
             return self.apply(x)
         return super().__matmul__(x)
 
     def dot(self, x):
-        t = _lib.torch()
-        if isinstance(x, t.Tensor):
+        if isinstance(x, _lib.torch().Tensor):
             return self.apply(x)
         return super().dot(x)
 
-    # -- leaf data (host views for the block API) ---------------------------
+    # -- leaf data (host views for the block API) ----------------------------
 
-    def _leaf_csr(self, rng):
-        """SciPy CSR of F[I, J] (ascending global ids) for a leaf range."""
+    def _leaf_csr(self, rng, cluster_order=False):
+        """SciPy CSR of F over a leaf range: rows and columns in ascending
+        original ids (or in cluster order)."""
         from . import _hbuild
-        return _hbuild.leaf_csr(self, rng)
+        return _hbuild.leaf_csr(self, rng, cluster_order)
 
-    def _far_factors(self, far_id):
+    def _far_factors(self, far_id, cluster_order=False):
         from . import _hbuild
-        return _hbuild.far_factors(self, far_id)
+        return _hbuild.far_factors(self, far_id, cluster_order)
+
+
+def _check_build_args(tol, max_depth, force_max_depth):
+    """The argument checks of the reference constructor (:710-725)."""
+    if tol < 0:
+        raise RuntimeError('tol should be a nonnegative float')
+    if max_depth is not None:
+        if not isinstance(max_depth, int):
+            raise RuntimeError('invalid max_depth type: %s' % str(type(max_depth)))
+        if max_depth <= 0:
+            raise RuntimeError('invalid max_depth value: %d' % max_depth)
+    elif force_max_depth:
+        raise RuntimeError('force_max_depth is True, but max_depth not specified')
 
 
 class CompressedKernelMatrix(scipy.sparse.linalg.LinearOperator):
-    """K x = x - rho (F x) ('left') or x - F (rho x) ('right')
-    (compressed_form_factors.py:875-913)."""
+    """K = I - rho F (compressed_form_factors.py:875-913): K x = x - rho (F x)
+    ('left') or x - F (rho x) ('right').  The adjoint K^T x = x - F^T (rho x)
+    ('left') or x - rho (F^T x) ('right') runs on the compressed F's adjoint
+    product.  Device tensors stay on the device."""
 
     def __init__(self, form_factor_matrix, rho, albedo_placement='right'):
-        self.form_factor_matrix = form_factor_matrix
-        if not (0 < rho <= 1):
+        if not 0 < rho <= 1:
             raise ValueError('need 0 < rho <= 1')
-        self.rho = rho
-        if albedo_placement not in {'left', 'right'}:
+        if albedo_placement not in ('left', 'right'):
             raise ValueError('albedo_placement must be "left" or "right"')
+        self.form_factor_matrix = form_factor_matrix
+        self.rho = rho
         self.albedo_placement = albedo_placement
 
     @property
@@ -782,21 +778,35 @@ class CompressedKernelMatrix(scipy.sparse.linalg.LinearOperator):
     def dtype(self):
         return self.form_factor_matrix.dtype
 
-    def _matvec(self, x):
-        y = x.copy()
-        if self.albedo_placement == 'left':
-            y -= self.rho * (self.form_factor_matrix @ x)
+    def _apply(self, x, adjoint):
+        F = self.form_factor_matrix
+        if isinstance(x, _lib.torch().Tensor):
+            prod = F.apply_t if adjoint else F.apply
         else:
-            y -= self.form_factor_matrix @ (self.rho * x)
-        return y
+            x = np.asarray(x)
+            prod = (lambda v: F.T @ v) if adjoint else (lambda v: F @ v)
+        # rho multiplies the input of F for 'right' (F^T for 'left' on the
+        # adjoint), its output otherwise
+        rho_first = (self.albedo_placement == 'right') != adjoint
+        Fx = prod(self.rho * x) if rho_first else self.rho * prod(x)
+        return x - Fx
+
+    def _matvec(self, x):
+        return self._apply(x, False)
+
+    def _matmat(self, X):
+        return self._apply(X, False)
 
     def _rmatvec(self, x):
-        y = x.copy()
-        if self.albedo_placement == 'left':
-            y -= self.form_factor_matrix.T @ (self.rho * x)
-        else:
-            y -= self.rho * (self.form_factor_matrix.T @ x)
-        return y
+        return self._apply(x, True)
+
+    def _rmatmat(self, X):
+        return self._apply(X, True)
+
+    def __matmul__(self, x):
+        if isinstance(x, _lib.torch().Tensor):
+            return self._apply(x, False)
+        return super().__matmul__(x)
 
 
 logging.getLogger(__name__).addHandler(logging.NullHandler())

@@ -27,12 +27,17 @@ __device__ __forceinline__ double wide(double v) { return v; }
 // T_b (rank x nrhs, accumulation type) starts at tstart[b] = sum of
 // rank * nrhs over the blocks before b.
 
+// The transposed product (trans != 0) is Y[c0:c1] += Vt_b^T (S_b o (U_b^T
+// X[r0:r1])): the same kernels with the roles of U and Vt (and of the row
+// and column ranges) exchanged, reading the factors in place.
+
 // One thread per (block, j, c).
 template <typename TF, typename TX, typename TAcc>
 __global__ void k_far_project(int64_t nfar, const int64_t *__restrict__ desc,
                               const int64_t *__restrict__ tstart, int64_t total,
-                              const TF *__restrict__ Vt, const TF *__restrict__ S,
-                              const TX *__restrict__ X, int64_t nrhs, TAcc *__restrict__ T) {
+                              const TF *__restrict__ U, const TF *__restrict__ Vt,
+                              const TF *__restrict__ S, const TX *__restrict__ X, int64_t nrhs,
+                              int trans, TAcc *__restrict__ T) {
   const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= total) return;
   int64_t lo = 0, hi = nfar;
@@ -45,13 +50,18 @@ __global__ void k_far_project(int64_t nfar, const int64_t *__restrict__ desc,
   }
   const int64_t b = lo;
   const int64_t *d = desc + b * 8;
-  const int64_t c0 = d[2], c1 = d[3];
   const int64_t loc = w - tstart[b];
   const int64_t j = loc / nrhs, c = loc % nrhs;
-  const int64_t n = c1 - c0;
-  const TF *v = Vt + d[6] + j * n;
   TAcc acc = 0;
-  for (int64_t i = 0; i < n; ++i) acc += (TAcc)wide(v[i]) * (TAcc)X[(c0 + i) * nrhs + c];
+  if (!trans) {  // Vt_b X[c0:c1]: row j of Vt
+    const int64_t c0 = d[2], n = d[3] - d[2];
+    const TF *v = Vt + d[6] + j * n;
+    for (int64_t i = 0; i < n; ++i) acc += (TAcc)wide(v[i]) * (TAcc)X[(c0 + i) * nrhs + c];
+  } else {       // U_b^T X[r0:r1]: column j of U
+    const int64_t r0 = d[0], m = d[1] - d[0], r = d[4];
+    const TF *u = U + d[5] + j;
+    for (int64_t i = 0; i < m; ++i) acc += (TAcc)wide(u[i * r]) * (TAcc)X[(r0 + i) * nrhs + c];
+  }
   T[w] = acc * (TAcc)wide(S[d[7] + j]);
 }
 
@@ -63,7 +73,8 @@ __global__ void k_far_expand(int64_t n, int64_t nrhs, const int64_t *__restrict_
                              const int64_t *__restrict__ tstart,
                              const int32_t *__restrict__ rowcl, const int64_t *__restrict__ cptr,
                              const int32_t *__restrict__ cblk, const TF *__restrict__ U,
-                             const TAcc *__restrict__ T, TX *__restrict__ Y) {
+                             const TF *__restrict__ Vt, int trans, const TAcc *__restrict__ T,
+                             TX *__restrict__ Y) {
   const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= n * nrhs) return;
   const int64_t i = w / nrhs, c = w % nrhs;
@@ -72,11 +83,20 @@ __global__ void k_far_expand(int64_t n, int64_t nrhs, const int64_t *__restrict_
   for (int64_t q = cptr[cl]; q < cptr[cl + 1]; ++q) {
     const int64_t b = cblk[q];
     const int64_t *d = desc + b * 8;
-    const int64_t r0 = d[0], r1 = d[1], r = d[4];
-    if (i < r0 || i >= r1) continue;
-    const TF *u = U + d[5] + (i - r0) * r;
+    const int64_t r = d[4];
     const TAcc *t = T + tstart[b];
-    for (int64_t j = 0; j < r; ++j) acc += (TAcc)wide(u[j]) * t[j * nrhs + c];
+    if (!trans) {  // row i - r0 of U_b
+      const int64_t r0 = d[0], r1 = d[1];
+      if (i < r0 || i >= r1) continue;
+      const TF *u = U + d[5] + (i - r0) * r;
+      for (int64_t j = 0; j < r; ++j) acc += (TAcc)wide(u[j]) * t[j * nrhs + c];
+    } else {       // column i - c0 of Vt_b
+      const int64_t c0 = d[2], c1 = d[3];
+      if (i < c0 || i >= c1) continue;
+      const TF *v = Vt + d[6] + (i - c0);
+      const int64_t nb = c1 - c0;
+      for (int64_t j = 0; j < r; ++j) acc += (TAcc)wide(v[j * nb]) * t[j * nrhs + c];
+    }
   }
   Y[w] = (TX)((TAcc)Y[w] + acc);
 }
@@ -85,17 +105,18 @@ template <typename TF, typename TX>
 static int far_launch(int64_t n, int64_t nrhs, int64_t nfar, const int64_t *desc,
                       const int64_t *tstart, int64_t total, const void *U, const void *S,
                       const void *Vt, const int32_t *rowcl, const int64_t *cptr,
-                      const int32_t *cblk, const void *X, void *Y, void *T, hipStream_t st) {
+                      const int32_t *cblk, const void *X, void *Y, void *T, int trans,
+                      hipStream_t st) {
   using TAcc = typename std::conditional<std::is_same<TX, double>::value ||
                                              std::is_same<TF, double>::value,
                                          double, float>::type;
   hipLaunchKernelGGL((k_far_project<TF, TX, TAcc>), dim3(ceil_div_u(total, 256)), dim3(256), 0,
-                     st, nfar, desc, tstart, total, (const TF *)Vt, (const TF *)S,
-                     (const TX *)X, nrhs, (TAcc *)T);
+                     st, nfar, desc, tstart, total, (const TF *)U, (const TF *)Vt,
+                     (const TF *)S, (const TX *)X, nrhs, trans, (TAcc *)T);
   FLUX_LAUNCH_CHECK();
   hipLaunchKernelGGL((k_far_expand<TF, TX, TAcc>), dim3(ceil_div_u(n * nrhs, 256)), dim3(256),
                      0, st, n, nrhs, desc, tstart, rowcl, cptr, cblk, (const TF *)U,
-                     (const TAcc *)T, (TX *)Y);
+                     (const TF *)Vt, trans, (const TAcc *)T, (TX *)Y);
   FLUX_LAUNCH_CHECK();
   return 0;
 }
@@ -144,7 +165,7 @@ extern "C" int flux_far_matmat(int64_t n, int64_t nrhs, int64_t nfar, const int6
                                const int64_t *tstart, int64_t total, const void *U,
                                const void *S, const void *Vt, int fdtype, const int32_t *rowcl,
                                const int64_t *cptr, const int32_t *cblk, const void *X, void *Y,
-                               int xdtype, void *T, void *stream) {
+                               int xdtype, void *T, int trans, void *stream) {
   FLUX_CHECK(n >= 0 && nrhs >= 0 && nfar >= 0 && total >= 0, "flux_far_matmat: bad sizes");
   if (nfar == 0 || n * nrhs == 0) return 0;
   FLUX_CHECK(desc && tstart && U && S && Vt && rowcl && cptr && cblk && X && Y && T,
@@ -152,7 +173,7 @@ extern "C" int flux_far_matmat(int64_t n, int64_t nrhs, int64_t nfar, const int6
   hipStream_t st = as_stream(stream);
 #define FAR(TF, TX)                                                                      \
   return far_launch<TF, TX>(n, nrhs, nfar, desc, tstart, total, U, S, Vt, rowcl, cptr, \
-                            cblk, X, Y, T, st)
+                            cblk, X, Y, T, trans, st)
   if (fdtype == FLUX_F64 && xdtype == FLUX_F64) FAR(double, double);
   if (fdtype == FLUX_F32 && xdtype == FLUX_F64) FAR(float, double);
   if (fdtype == FLUX_F32 && xdtype == FLUX_F32) FAR(float, float);

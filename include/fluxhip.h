@@ -264,7 +264,10 @@ int flux_scale_cols_inv_sqrt(int64_t nb, const int64_t *dims, double *C,
                              const int64_t *offL, void *stream);
 
 /* Far-field part of the H-matvec (SVD leaves, compressed_form_factors.py:
- * 247-251): Y[r0:r1] += U_b (S_b o (Vt_b X[c0:c1])) for every far block b.
+ * 247-251): Y[r0:r1] += U_b (S_b o (Vt_b X[c0:c1])) for every far block b;
+ * with trans != 0 the adjoint Y[c0:c1] += Vt_b^T (S_b o (U_b^T X[r0:r1]))
+ * (the x @ F of compressed_form_factors.py:905-913, which the reference's
+ * compressed blocks cannot serve), the lists then covering columns.
  * desc: nfar rows of 8 int64 (r0, r1, c0, c1, rank, offU, offVt, offS);
  * tstart[b] = sum of rank*nrhs before b, total = the full sum; T: work of
  * `total` elements in the accumulation type (FP64 if X or the factors are
@@ -275,7 +278,7 @@ int flux_far_matmat(int64_t n, int64_t nrhs, int64_t nfar, const int64_t *desc,
                     const void *S, const void *Vt, int fdtype,
                     const int32_t *rowcl, const int64_t *cptr,
                     const int32_t *cblk, const void *X, void *Y, int xdtype,
-                    void *T, void *stream);
+                    void *T, int trans, void *stream);
 
 /* Gather (scatter == 0: out[k] = X[perm[k]]) or scatter (out[perm[k]] =
  * X[k]) of the rows of an n x nrhs block: the cluster ordering. */
@@ -487,6 +490,28 @@ int flux_temperature(int64_t n, int64_t nrhs, const void *B, const void *Q,
                      const void *rho, double rho_s, const void *emiss,
                      double emiss_s, double fsurf, int clamp, double sigma,
                      void *T, int dtype, void *stream);
+
+/* ---- Reverse mode of the temperature map --------------------------------
+ * No counterpart in the reference, which has no derivatives; these are the
+ * element-wise parts of the gradient of a loss L(T) through
+ *   (I - diag(rho) F) B = E,  Phi = E + F B,
+ *   T = ((1 - rho) Phi / (emiss sigma))^(1/4)
+ * (the steady state of model.py:8-42 in its single-solve form).  All
+ * arrays FP64, n x k row-major (facet-major) or per facet (rho, emiss,
+ * grho, gemiss).
+ * seed:   from T and gT = dL/dT: gPhi = gT (1 - rho) / (4 T^3 emiss sigma)
+ *         (0 where T = 0), grho = -sum_s gT T / (4 (1 - rho)),
+ *         gemiss = -sum_s gT T / (4 emiss).
+ * finish: after lam solves (I - F^T diag(rho)) lam = F^T gPhi:
+ *         gE = gPhi + lam, grho += sum_s lam (F B). */
+int flux_temperature_adjoint_seed(int64_t n, int64_t k, const double *T,
+                                  const double *gT, const double *rho,
+                                  const double *emiss, double sigma,
+                                  double *gPhi, double *grho, double *gemiss,
+                                  void *stream);
+int flux_temperature_adjoint_finish(int64_t n, int64_t k, const double *gPhi,
+                                    const double *lam, const double *FB,
+                                    double *gE, double *grho, void *stream);
 
 /* ---- Time-dependent thermal model --------------------------------------
  * One step of 1-D subsurface heat conduction for nf facets at once.  It

@@ -266,3 +266,84 @@ def test_compressed_solves_and_pickle(torch_dev, oracle_mod):
     assert sorted(np.concatenate(cols).tolist()) == list(range(sm.num_faces))
     Hd = H.get_diag_blocks()
     assert np.isfinite(Hd @ x).all()
+
+
+def _oracle_rows_at_depth(t, depth):
+    """The reference's _get_row_inds_at_depth walk (compressed_form_factors.py
+    :467-480) over the oracle's tree (global ids)."""
+    out = []
+    for i, ri in enumerate(t.row_inds):
+        b = t.blocks[i, i]
+        if depth == 1 or b.is_leaf:
+            out.append(np.sort(t.I[ri]))
+        else:
+            out.extend(_oracle_rows_at_depth(b, depth - 1))
+    return out
+
+
+def _oracle_row_leaves(t, row):
+    """(leaf kind, row ids, col ids) of the leaves of the block row holding
+    `row` (the walks of :482-509)."""
+    out = []
+    for i, ri in enumerate(t.row_inds):
+        if row in t.I[ri]:
+            for j, cj in enumerate(t.col_inds):
+                b = t.blocks[i, j]
+                if b.is_leaf:
+                    out.append((b.kind, tuple(np.sort(t.I[ri])), tuple(np.sort(t.J[cj]))))
+                else:
+                    out.extend(_oracle_row_leaves(b, row))
+            break
+    return out
+
+
+def _oracle_blocks_at_depth(t, depth):
+    if depth == 0:
+        return [t]
+    if t.is_leaf:
+        return []
+    return [x for b in t.blocks.ravel() for x in _oracle_blocks_at_depth(b, depth - 1)]
+
+
+def test_introspection_matches_reference_walks(torch_dev, oracle_mod):
+    """get_row_inds_at_depth, get_row_blocks, get_col_inds_for_row,
+    get_blocks_at_depth and the diagonal / off-diagonal / individual views
+    against the reference's recursive walks restated over the oracle tree."""
+    from oracle import hmatrix as OH
+    from fluxpy_amd.compressed_form_factors import CompressedFormFactorMatrix
+    from fluxpy_amd.form_factors import get_form_factor_matrix
+    from fluxpy_amd.shape import CgalTrimeshShapeModel
+    V, F = meshes.spherical_crater(npts=32, seed=1001)
+    sm = CgalTrimeshShapeModel(V, F)
+    H = CompressedFormFactorMatrix(sm, tol=1e-2, min_size=1024)
+    FF = get_form_factor_matrix(sm)
+    Ho = OH.HMatrix(FF, sm.P, tol=1e-2, min_size=1024)
+    assert H.depth == Ho.root.depth
+    for d in range(1, H.depth + 1):
+        got = [tuple(r) for r in H.get_row_inds_at_depth(d)]
+        ref = [tuple(r) for r in _oracle_rows_at_depth(Ho.root, d)]
+        assert got == ref, d
+        shapes = [b.shape for b in H.get_blocks_at_depth(d)]
+        assert shapes == [tuple(b.shape) for b in _oracle_blocks_at_depth(Ho.root, d)]
+    kind = lambda b: type(b).__name__.replace("FormFactor", "").replace("Block", "").lower()
+    for row in np.random.default_rng(8).choice(sm.num_faces, 25, replace=False):
+        ref = _oracle_row_leaves(Ho.root, row)
+        leaves = list(H.get_row_blocks(row))
+        got = [(kind(b), tuple(b._row_ids()), tuple(b._col_ids())) for b in leaves]
+        assert got == ref
+        cols = [tuple(c) for c in H.get_col_inds_for_row(row)]
+        assert cols == [r[2] for r in ref]
+    # views: diagonal + off-diagonal = whole, individual blocks sum to it too
+    x = np.random.default_rng(9).standard_normal(sm.num_faces)
+    y = H @ x
+    yd, yo = H.get_diag_blocks() @ x, H.get_off_diag_blocks() @ x
+    assert np.allclose(yd + yo, y, rtol=1e-12, atol=1e-14 * abs(y).max())
+    root = H._root
+    acc = np.zeros_like(y)
+    for i, j in it.product(range(root.bshape[0]), range(root.bshape[1])):
+        acc += root.get_individual_block(i, j)._matmat(x)
+    assert np.allclose(acc, y, rtol=1e-12, atol=1e-14 * abs(y).max())
+    with pytest.raises(Exception):
+        next(H.get_blocks_at_depth(H.depth + 1))
+    with pytest.raises(Exception):
+        next(H.get_row_blocks(sm.num_faces))
