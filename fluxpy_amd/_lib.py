@@ -92,6 +92,7 @@ SIGNATURES = {
                                 _c_dbl, _c_int, _vp, _vp, _vp, _c_i64, _vp, _vp, _vp, _c_i64,
                                 _vp, _c_int, _vp, _vp]),
     "flux_vec_reduce":(_c_int, [_c_i64, _c_i64, _vp, _vp, _c_int, _c_int, _vp, _vp, _vp]),
+    "flux_vec_reduce_noalloc": (_c_int, [_c_i64, _c_i64, _vp, _vp, _c_int, _c_int, _vp, _vp]),
     "flux_vec_axpby": (_c_int, [_c_i64, _c_i64, _vp, _vp, _vp, _vp, _c_int, _vp]),
     "flux_vec_rowscale": (_c_int, [_c_i64, _c_i64, _vp, _c_dbl, _c_int, _c_dbl, _vp, _vp,
                                    _c_int, _vp]),

@@ -5,6 +5,8 @@
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
 
+#include <mutex>
+
 #include "common.h"
 
 namespace flux {
@@ -153,6 +155,57 @@ extern "C" int flux_vec_reduce(int64_t n, int64_t nrhs, const void *x, const voi
                      out, mode);
   FLUX_LAUNCH_CHECK();
   return 0;
+}
+
+// ---- the same reduction without a caller workspace --------------------
+// The 64 partial results per column live in `scratch`, a file-scope device
+// array: one copy per GPU, held until the process ends.  In host code the
+// name `scratch` is a host-side stand-in, not a device address; the device
+// address is looked up once per device with hipGetSymbolAddress and only
+// that address is passed to kernels.  Every call on one GPU shares the
+// array: calls that overlap on one GPU (other streams, other threads) race.
+// Nothing in it has to start at zero: k_dot_partial writes every partial
+// that k_dot_final reads.
+namespace flux {
+
+constexpr int64_t kReduceScratchCols = 4096;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[64 * kReduceScratchCols];
+
+static std::mutex g_scratch_mutex;
+static double *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device.
+static int scratch_address(double **out) {
+  int dev = 0;
+  FLUX_TRY_HIP(hipGetDevice(&dev));
+  FLUX_CHECK(dev >= 0 && dev < kMaxDevices, "flux_vec_reduce_noalloc: device %d out of range",
+             dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void *p = nullptr;
+    FLUX_TRY_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    FLUX_CHECK(p != nullptr, "flux_vec_reduce_noalloc: no device address for the scratch array");
+    g_scratch_ptr[dev] = (double *)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+}  // namespace flux
+
+extern "C" int flux_vec_reduce_noalloc(int64_t n, int64_t nrhs, const void *x, const void *y,
+                                       int dtype, int mode, double *out, void *stream) {
+  using namespace flux;
+  FLUX_CHECK(nrhs >= 0 && nrhs <= kReduceScratchCols,
+             "flux_vec_reduce_noalloc: %lld columns exceed the %lld of the built-in scratch "
+             "(use flux_vec_reduce with a workspace)",
+             (long long)nrhs, (long long)kReduceScratchCols);
+  if (nrhs == 0) return 0;
+  double *work = nullptr;
+  const int rc = scratch_address(&work);
+  if (rc != 0) return rc;
+  return flux_vec_reduce(n, nrhs, x, y, dtype, mode, out, work, stream);
 }
 
 extern "C" int flux_vec_axpby(int64_t n, int64_t nrhs, const double *alpha, const void *x,

@@ -461,6 +461,18 @@ int flux_vec_reduce(int64_t n, int64_t nrhs, const void *x, const void *y,
                     int dtype, int mode, double *out, double *work,
                     void *stream);
 
+/* flux_vec_reduce without the `work` argument: allocates nothing.  No
+ * counterpart in the reference; it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The partial results live in a
+ * library-owned device array of 64 x 4096 doubles (2 MiB per GPU, held until
+ * the process ends); nrhs above 4096 is rejected.  Every call on one GPU
+ * shares that array, so calls that overlap on one GPU race.  Results are
+ * identical to flux_vec_reduce's. */
+int flux_vec_reduce_noalloc(int64_t n, int64_t nrhs, const void *x,
+                            const void *y, int dtype, int mode, double *out,
+                            void *stream);
+
 /* y[:,c] = alpha[c] x[:,c] + beta[c] y[:,c]; alpha, beta: device FP64
  * arrays of nrhs, NULL meaning all ones. */
 int flux_vec_axpby(int64_t n, int64_t nrhs, const double *alpha,
