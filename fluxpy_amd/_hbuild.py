@@ -626,7 +626,7 @@ class HOperator:
     near-field CSR and the same far-field factors read with U and Vt
     exchanged."""
 
-    def __init__(self, H, values, transpose=False):
+    def __init__(self, H, values, transpose=False, far_only=False):
         t = _lib.require_gpu()
         tdt = getattr(t, _VALUE_DTYPES[values])
         self.H = H
@@ -639,7 +639,8 @@ class HOperator:
             if getattr(H, "_nearT", None) is None:
                 H._nearT = _kernels.csr_transpose(near)
             near = H._nearT
-        self.near = near if near.data.dtype == tdt else near.astype(tdt)
+        # far_only: the far-field factors alone (far_apply), no near-field copy
+        self.near = None if far_only else (near if near.data.dtype == tdt else near.astype(tdt))
         self.vec_dtype = t.float64 if values == "fp64" else t.float32
         self.perm = _lib.to_dev(H._ctree.perm)
         self.nfar = len(H._far_used)
@@ -670,7 +671,8 @@ class HOperator:
                 x if x.dtype == t.float32 else _kernels.convert(x, t.float32), tdt)
 
         self.U, self.S, self.Vt = cat(Us), cat(Ss), cat(Vts)
-        self.desc = _i64(np.array(descs, dtype=np.int64))
+        self.desc_host = np.array(descs, dtype=np.int64)
+        self.desc = _i64(self.desc_host)
         self.ranks = np.array(ranks, dtype=np.int64)
         # output rows -> far blocks covering them (row ranges, or column
         # ranges for the adjoint), through a per-row cluster id
@@ -729,8 +731,26 @@ class HOperator:
                       self.Vt.data_ptr(), _lib.dtype_code(self.U.dtype),
                       self.rowcl.data_ptr(), self.cptr.data_ptr(), self.cblk.data_ptr(),
                       X2.data_ptr(), Y.data_ptr(), _lib.dtype_code(X2.dtype), T.data_ptr(),
-                      int(self.transpose), _lib.stream())
+                      int(self.transpose), None, _lib.stream())
         return Y.reshape(-1) if vec else Y
+
+    def far_apply(self, X, Y, rscale=None):
+        """Y += rscale o (F_far X) in place (X, Y: n x k, cluster order; the
+        accumulation type follows X and the factors as in apply)."""
+        t = _lib.torch()
+        if not self.nfar:
+            return Y
+        k = X.shape[1]
+        tst, total = self._tst(k)
+        acc = t.float64 if (X.dtype == t.float64 or self.U.dtype == t.float64) else t.float32
+        T = t.empty(max(total, 1), dtype=acc, device=X.device)
+        _lib.call("flux_far_matmat", self.n, k, self.nfar, self.desc.data_ptr(),
+                  tst.data_ptr(), total, self.U.data_ptr(), self.S.data_ptr(),
+                  self.Vt.data_ptr(), _lib.dtype_code(self.U.dtype), self.rowcl.data_ptr(),
+                  self.cptr.data_ptr(), self.cblk.data_ptr(), X.data_ptr(), Y.data_ptr(),
+                  _lib.dtype_code(X.dtype), T.data_ptr(), int(self.transpose),
+                  _lib.ptr(rscale), _lib.stream())
+        return Y
 
     def jacobi_step(self, E, B, rho, left, B1, colmax):
         t = _lib.torch()
@@ -752,10 +772,131 @@ class HOperator:
                   colmax.data_ptr(), _lib.stream())
 
     def nbytes_device(self):
-        nb = self.near.nbytes
+        nb = self.near.nbytes if self.near is not None else 0
         if self.nfar:
             nb += (self.U.numel() + self.S.numel() + self.Vt.numel()) * self.U.element_size()
         return nb
+
+
+class FarField16:
+    """The far field of a compressed matrix packed for the MFMA products
+    (flux_far_pack16 / flux_far_mm16): 16-bit U and diag(S) Vt with the
+    rank padded to a multiple of 16, and the expansion scheduled by
+    containment level of the row ranges (the ranges of one level are
+    disjoint, so each launch writes every output row once)."""
+
+    def __init__(self, master, values="bf16"):
+        t = _lib.require_gpu()
+        if master.transpose or not master.nfar:
+            raise ValueError("FarField16 packs the far field of a forward operator")
+        if values not in ("bf16", "fp16"):
+            raise ValueError("values must be 'bf16' or 'fp16'")
+        self.half = values == "fp16"
+        self.dtype = t.float16 if self.half else t.bfloat16
+        self.n, self.nfar = master.n, master.nfar
+        dev = master.U.device
+        src = master.desc_host
+        r0, r1, c0, c1, r = (src[:, i] for i in range(5))
+        m, nc = r1 - r0, c1 - c0
+        rp = 16 * ((r + 15) // 16)
+        if (rp > 256).any():
+            raise _lib.FluxError("FarField16: ranks above 256 are not supported")
+        offU = np.concatenate([[0], np.cumsum(m * rp)])
+        offV = np.concatenate([[0], np.cumsum(rp * nc)])
+        self.rp_off = np.concatenate([[0], np.cumsum(rp)])[:-1]
+        self.base = np.stack([r0, r1, c0, c1, rp, offU[:-1], offV[:-1], 0 * r0], 1)
+        self.U16 = t.empty(max(int(offU[-1]), 1), dtype=self.dtype, device=dev)
+        self.V16 = t.empty(max(int(offV[-1]), 1), dtype=self.dtype, device=dev)
+        self.rp_total = int(rp.sum())
+        ddesc = _i64(self.base)
+        _lib.call("flux_far_pack16", self.nfar, master.desc.data_ptr(), ddesc.data_ptr(),
+                  master.U.data_ptr(), master.S.data_ptr(), master.Vt.data_ptr(),
+                  _lib.dtype_code(master.U.dtype), int(self.half), self.U16.data_ptr(),
+                  self.V16.data_ptr(), _lib.stream())
+        del ddesc
+        # expansion schedule: distinct row ranges by containment depth
+        ranges = sorted(set(zip(r0.tolist(), r1.tolist())), key=lambda a: (a[0], -a[1]))
+        depth, stack = {}, []
+        for a, b in ranges:
+            while stack and stack[-1] <= a:
+                stack.pop()
+            depth[(a, b)] = len(stack)
+            stack.append(b)
+        nlev = max(depth.values()) + 1
+        by_range = {}
+        for b in range(self.nfar):
+            by_range.setdefault((int(r0[b]), int(r1[b])), []).append(b)
+        groups, wgstart, blist = [], [], []
+        self.lvl_first, self.lvl_ngroups, self.lvl_nwg = [], [], []
+        for lv in range(nlev):
+            first, nwg = len(groups), 0
+            for rng_ in (g for g in ranges if depth[g] == lv):
+                members = by_range[rng_]
+                groups.append([rng_[0], rng_[1], len(blist), len(blist) + len(members)])
+                blist.extend(members)
+                wgstart.append(nwg)
+                nwg += (rng_[1] - rng_[0] + 63) // 64
+            self.lvl_first.append(first)
+            self.lvl_ngroups.append(len(groups) - first)
+            self.lvl_nwg.append(nwg)
+        for g in groups:
+            if g[1] <= g[0]:
+                raise _lib.FluxError("FarField16: empty row range")
+        self.nlev = nlev
+        self.groups = _i64(np.array(groups, dtype=np.int64))
+        self.wgstart = _i64(np.array(wgstart, dtype=np.int64))
+        self.blist = _lib.to_dev(np.array(blist, dtype=np.int32))
+        c64 = _lib.ctypes.c_int64 * nlev
+        self._lvl = (c64(*self.lvl_first), c64(*self.lvl_ngroups), c64(*self.lvl_nwg))
+        self._per_k = {}
+
+    def _buffers(self, k):
+        if k not in self._per_k:
+            t = _lib.torch()
+            d = self.base.copy()
+            d[:, 7] = k * self.rp_off
+            T = t.empty(max(self.rp_total * k, 1), dtype=self.dtype, device=self.U16.device)
+            self._per_k[k] = (_i64(d), T)
+        return self._per_k[k]
+
+    @property
+    def nbytes(self):
+        return (self.U16.numel() + self.V16.numel()) * 2
+
+    def mm(self, X16, Y, rscale=None):
+        """Y (n x k FP32) += rscale o (F_far X16), X16 n x k in the packed
+        16-bit type, both in cluster order."""
+        if X16.dtype != self.dtype or Y.dtype != _lib.torch().float32:
+            raise TypeError("FarField16.mm: X in the packed type, Y FP32")
+        n, k = X16.shape
+        if n != self.n or tuple(Y.shape) != (n, k):
+            raise ValueError("FarField16.mm: shape mismatch")
+        desc, T = self._buffers(k)
+        lf, lg, lw = self._lvl
+        _lib.call("flux_far_mm16", self.nfar, desc.data_ptr(), self.U16.data_ptr(),
+                  self.V16.data_ptr(), int(self.half), X16.data_ptr(), k, T.data_ptr(),
+                  self.nlev, lf, lg, lw, self.groups.data_ptr(), self.wgstart.data_ptr(),
+                  self.blist.data_ptr(), _lib.ptr(rscale), Y.data_ptr(), _lib.stream())
+        return Y
+
+
+def far_master(H):
+    """Far-field factors alone in the model dtype (FP32 or FP64 masters)."""
+    if getattr(H, "_hop", None) is None:
+        H._hop = {}
+    key = ("master", False)
+    if key not in H._hop:
+        values = "fp64" if np.dtype(H.dtype) == np.float64 else "fp32"
+        H._hop[key] = HOperator(H, values, far_only=True)
+    return H._hop[key]
+
+
+def far16(H, values="bf16"):
+    """The packed 16-bit far field of H (built once)."""
+    key = ("far16", values)
+    if key not in H._hop:
+        H._hop[key] = FarField16(far_master(H), values)
+    return H._hop[key]
 
 
 def operator(H, values=None, transpose=False):

@@ -285,12 +285,20 @@ class PanelRefinedSolver:
     with the residuals; a round that fails it recomputes its residual in full.
     """
 
-    def __init__(self, P_lo, P_hi, n, k, rho=1.0, left=False):
+    def __init__(self, P_lo, P_hi, n, k, rho=1.0, left=False, far_lo=None, far_hi=None):
         t = _lib.require_gpu()
         dev = P_lo.panel.device
         self.P_lo, self.P_hi = P_lo, P_hi
+        # far field (low-rank leaves) of a compressed operator: 16-bit MFMA
+        # product in the sweeps, FP32 / FP64 master factors with FP64 vectors
+        # in the residual
+        if (far_lo is None) != (far_hi is None):
+            raise ValueError("far_lo and far_hi go together")
+        self.far_lo, self.far_hi = far_lo, far_hi
         self.n, self.k, self.left = n, k, left
         self.set_rho(rho)
+        if far_hi is not None:
+            self.Etmp = t.empty((n, k), dtype=t.float64, device=dev)
         f32, f64 = t.float32, t.float64
         self.Bs = [t.empty((n, k), dtype=f64, device=dev) for _ in range(2)]
         self.Rs = [t.empty((n, k), dtype=f64, device=dev) for _ in range(2)]
@@ -325,6 +333,15 @@ class PanelRefinedSolver:
         else:
             self.rho64 = self.rho32 = None
             self.rs = float(rho)
+        # row factors of the far field for the albedo on the left
+        self.fscale32 = self.fscale64 = None
+        if self.left and self.far_lo is not None:
+            dev = self.P_lo.panel.device
+            if self.rho64 is not None:
+                self.fscale64, self.fscale32 = self.rho64, self.rho32
+            else:
+                self.fscale64 = t.full((self.n,), self.rs, dtype=t.float64, device=dev)
+                self.fscale32 = t.full((self.n,), self.rs, dtype=t.float32, device=dev)
 
     # slices of the X of an update residual: the update X is at most 2^-7 of
     # the first round's in every column (checked after the fact), so four
@@ -350,8 +367,18 @@ class PanelRefinedSolver:
         for it in range(sweeps):
             # the last sweep's next iterate would go unread
             xn = self.Xb[1 - cur] if it + 1 < sweeps else None
-            self.P_lo.mm_bf16(self.Xb[cur], self.D[cur], base=self.R32, rho=self.rho32,
-                              rho_s=self.rs, left=self.left, xnext=xn)
+            if self.far_lo is None:
+                self.P_lo.mm_bf16(self.Xb[cur], self.D[cur], base=self.R32, rho=self.rho32,
+                                  rho_s=self.rs, left=self.left, xnext=xn)
+            else:
+                # near field, then the far field into the same FP32 output,
+                # then the next BF16 iterate from the sum
+                self.P_lo.mm_bf16(self.Xb[cur], self.D[cur], base=self.R32, rho=self.rho32,
+                                  rho_s=self.rs, left=self.left, xnext=None)
+                self.far_lo.mm(self.Xb[cur], self.D[cur], rscale=self.fscale32)
+                if xn is not None:
+                    to_bf16_scaled(self.D[cur], rho=self.rho32, rho_s=self.rs,
+                                   scale=not self.left, out=xn)
             cur = 1 - cur
         D = self.D[1 - cur]
         update = r_prev is not None and isinstance(self.P_hi, SlicedPanelMatrix)
@@ -361,22 +388,35 @@ class PanelRefinedSolver:
         rs = self.rho64 if self.rho64 is not None else self.rs
         xs = None if self.left else rs
         if update:
-            self.P_hi.mm_f64(self.D64, r_out, E=r_prev, B=self.D64, rho=self.rho64,
+            Eu = self._with_far(r_prev, self.D64)
+            self.P_hi.mm_f64(self.D64, r_out, E=Eu, B=self.D64, rho=self.rho64,
                              rho_s=self.rs, left=self.left, colmax=colmax, xscale=xs,
                              nslices=self.UPDATE_SLICES, slot=slot)
         else:
             self._full_residual(E, b_out, r_out, colmax, slot)
         return update
 
+    def _with_far(self, E, B):
+        """E + F_far (rho B) (right) / E + rho F_far B (left) in FP64 with
+        the master factors, or E itself without a far field."""
+        if self.far_hi is None:
+            return E
+        rs = self.rho64 if self.rho64 is not None else self.rs
+        X = B if self.left else _kernels.rowscale(rs, B, out=self.Xr)
+        self.Etmp.copy_(E)
+        self.far_hi.far_apply(X, self.Etmp, rscale=self.fscale64)
+        return self.Etmp
+
     def _full_residual(self, E, B, r_out, colmax, slot):
         rs = self.rho64 if self.rho64 is not None else self.rs
+        Ef = self._with_far(E, B)
         if isinstance(self.P_hi, SlicedPanelMatrix):
-            self.P_hi.mm_f64(B, r_out, E=E, B=B, rho=self.rho64, rho_s=self.rs,
+            self.P_hi.mm_f64(B, r_out, E=Ef, B=B, rho=self.rho64, rho_s=self.rs,
                              left=self.left, colmax=colmax, xscale=None if self.left else rs,
                              slot=slot)
         else:
             Xr = B if self.left else _kernels.rowscale(rs, B, out=self.Xr)
-            self.P_hi.mm_f64(Xr, r_out, E=E, B=B, rho=self.rho64, rho_s=self.rs,
+            self.P_hi.mm_f64(Xr, r_out, E=Ef, B=B, rho=self.rho64, rho_s=self.rs,
                              left=self.left, colmax=colmax)
 
     def _readback(self, parts):

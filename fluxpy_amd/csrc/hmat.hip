@@ -74,7 +74,7 @@ __global__ void k_far_expand(int64_t n, int64_t nrhs, const int64_t *__restrict_
                              const int32_t *__restrict__ rowcl, const int64_t *__restrict__ cptr,
                              const int32_t *__restrict__ cblk, const TF *__restrict__ U,
                              const TF *__restrict__ Vt, int trans, const TAcc *__restrict__ T,
-                             TX *__restrict__ Y) {
+                             const double *__restrict__ rscale, TX *__restrict__ Y) {
   const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= n * nrhs) return;
   const int64_t i = w / nrhs, c = w % nrhs;
@@ -98,6 +98,7 @@ __global__ void k_far_expand(int64_t n, int64_t nrhs, const int64_t *__restrict_
       for (int64_t j = 0; j < r; ++j) acc += (TAcc)wide(v[j * nb]) * t[j * nrhs + c];
     }
   }
+  if (rscale) acc *= (TAcc)rscale[i];
   Y[w] = (TX)((TAcc)Y[w] + acc);
 }
 
@@ -106,7 +107,7 @@ static int far_launch(int64_t n, int64_t nrhs, int64_t nfar, const int64_t *desc
                       const int64_t *tstart, int64_t total, const void *U, const void *S,
                       const void *Vt, const int32_t *rowcl, const int64_t *cptr,
                       const int32_t *cblk, const void *X, void *Y, void *T, int trans,
-                      hipStream_t st) {
+                      const double *rscale, hipStream_t st) {
   using TAcc = typename std::conditional<std::is_same<TX, double>::value ||
                                              std::is_same<TF, double>::value,
                                          double, float>::type;
@@ -116,7 +117,7 @@ static int far_launch(int64_t n, int64_t nrhs, int64_t nfar, const int64_t *desc
   FLUX_LAUNCH_CHECK();
   hipLaunchKernelGGL((k_far_expand<TF, TX, TAcc>), dim3(ceil_div_u(n * nrhs, 256)), dim3(256),
                      0, st, n, nrhs, desc, tstart, rowcl, cptr, cblk, (const TF *)U,
-                     (const TF *)Vt, trans, (const TAcc *)T, (TX *)Y);
+                     (const TF *)Vt, trans, (const TAcc *)T, rscale, (TX *)Y);
   FLUX_LAUNCH_CHECK();
   return 0;
 }
@@ -165,7 +166,8 @@ extern "C" int flux_far_matmat(int64_t n, int64_t nrhs, int64_t nfar, const int6
                                const int64_t *tstart, int64_t total, const void *U,
                                const void *S, const void *Vt, int fdtype, const int32_t *rowcl,
                                const int64_t *cptr, const int32_t *cblk, const void *X, void *Y,
-                               int xdtype, void *T, int trans, void *stream) {
+                               int xdtype, void *T, int trans, const double *rscale,
+                               void *stream) {
   FLUX_CHECK(n >= 0 && nrhs >= 0 && nfar >= 0 && total >= 0, "flux_far_matmat: bad sizes");
   if (nfar == 0 || n * nrhs == 0) return 0;
   FLUX_CHECK(desc && tstart && U && S && Vt && rowcl && cptr && cblk && X && Y && T,
@@ -173,7 +175,7 @@ extern "C" int flux_far_matmat(int64_t n, int64_t nrhs, int64_t nfar, const int6
   hipStream_t st = as_stream(stream);
 #define FAR(TF, TX)                                                                      \
   return far_launch<TF, TX>(n, nrhs, nfar, desc, tstart, total, U, S, Vt, rowcl, cptr, \
-                            cblk, X, Y, T, trans, st)
+                            cblk, X, Y, T, trans, rscale, st)
   if (fdtype == FLUX_F64 && xdtype == FLUX_F64) FAR(double, double);
   if (fdtype == FLUX_F32 && xdtype == FLUX_F64) FAR(float, double);
   if (fdtype == FLUX_F32 && xdtype == FLUX_F32) FAR(float, float);

@@ -291,16 +291,18 @@ def _gmres(op, E, rho, left, rtol, maxiter, restart, x0):
 
 
 def _panel_ready(FF):
-    """True for a compressed matrix whose operator is all near field (no
-    low-rank leaves): its batched solves run on the row-block panels."""
+    """True for a compressed matrix: its batched solves run on the row-block
+    panels of its near field, plus its low-rank far field if it has one."""
     from .compressed_form_factors import CompressedFormFactorMatrix
     return (isinstance(FF, CompressedFormFactorMatrix) and not getattr(FF, "_derived", False)
-            and getattr(FF, "_near", None) is not None
-            and not getattr(FF, "_far_used", [None]))
+            and getattr(FF, "_near", None) is not None)
 
 
 def _panel_engine(FF, residual):
-    """(BF16 panels, residual panels) of FF, built once and kept on FF."""
+    """(BF16 panels, residual panels, far field for the sweeps, far field
+    for the residual) of FF, built once and kept on FF; the far-field parts
+    are None without low-rank leaves."""
+    from . import _hbuild
     from ._panel import PanelMatrix, SlicedPanelMatrix
     cache = FF.__dict__.setdefault("_panel_cache", {})
     if "bf16" not in cache:
@@ -309,7 +311,11 @@ def _panel_engine(FF, residual):
         lo = cache["bf16"]
         cache[residual] = (SlicedPanelMatrix(FF._near, lo) if residual == "int8"
                            else lo.sibling(FF._near, "fp32"))
-    return cache["bf16"], cache[residual]
+    far_lo = far_hi = None
+    if FF._far_used:
+        far_hi = _hbuild.far_master(FF)
+        far_lo = _hbuild.far16(FF, "bf16")
+    return cache["bf16"], cache[residual], far_lo, far_hi
 
 
 def _solve_panels(FF, E2, rho, left, rtol, inner_iters, max_refine, residual):
@@ -330,12 +336,13 @@ def _solve_panels(FF, E2, rho, left, rtol, inner_iters, max_refine, residual):
         if rkey is not None:
             cache.update(key=rkey, rho64=rho64)
     Ei = _to_internal(op, E2)
-    P_lo, P_hi = _panel_engine(FF, residual)
+    P_lo, P_hi, far_lo, far_hi = _panel_engine(FF, residual)
     solvers = FF.__dict__.setdefault("_panel_solvers", {})
     key = (residual, k, bool(left))
     S = solvers.get(key)
     if S is None:
-        S = solvers[key] = PanelRefinedSolver(P_lo, P_hi, n, k, rho64, left=left)
+        S = solvers[key] = PanelRefinedSolver(P_lo, P_hi, n, k, rho64, left=left,
+                                              far_lo=far_lo, far_hi=far_hi)
     else:
         S.set_rho(rho64)
     _, info = S.solve(Ei, rtol=rtol, inner_iters=inner_iters, max_rounds=max_refine,

@@ -278,7 +278,33 @@ int flux_far_matmat(int64_t n, int64_t nrhs, int64_t nfar, const int64_t *desc,
                     const void *S, const void *Vt, int fdtype,
                     const int32_t *rowcl, const int64_t *cptr,
                     const int32_t *cblk, const void *X, void *Y, int xdtype,
-                    void *T, int trans, void *stream);
+                    void *T, int trans, const double *rscale, void *stream);
+/* (rscale: NULL, or n FP64 row factors applied to the far-field sum before
+ * it is added to Y -- the albedo of the 'left' placement.) */
+
+/* The far field on MFMA with 16-bit factors (the inner sweeps of the refined
+ * solve; same product as flux_far_matmat, trans = 0).
+ * pack16: factors (F32/F64, descriptors as flux_far_matmat) -> padded
+ *   16-bit copies (half != 0: FP16, else BF16) with S folded into Vt:
+ *   ddesc rows of 8 int64 (r0, r1, c0, c1, rp, offU16, offV16, toff),
+ *   rp = 16 ceil(rank / 16) <= 256, U16 m x rp and V16 rp x n row-major.
+ * mm16: Y (n x k FP32) += rscale o (U16 (V16 X)) for X (n x k, the 16-bit
+ *   type of the factors).  T: work of sum_b rp_b k 16-bit values (toff).
+ *   The expansion runs one launch per level: level lv covers groups
+ *   lvl_first[lv] .. + lvl_ngroups[lv] (host arrays) of `groups` (device,
+ *   int64 rows r0, r1, first, end into blist), with lvl_nwg[lv] workgroups
+ *   of 64 rows (wgstart: device, first workgroup of each group within its
+ *   level).  The row ranges of one level must be disjoint. */
+int flux_far_pack16(int64_t nfar, const int64_t *sdesc, const int64_t *ddesc,
+                    const void *U, const void *S, const void *Vt, int fdtype,
+                    int half, void *U16, void *V16, void *stream);
+int flux_far_mm16(int64_t nfar, const int64_t *desc, const void *U16,
+                  const void *V16, int half, const void *X, int64_t k, void *T,
+                  int64_t nlevels, const int64_t *lvl_first,
+                  const int64_t *lvl_ngroups, const int64_t *lvl_nwg,
+                  const int64_t *groups, const int64_t *wgstart,
+                  const int32_t *blist, const float *rscale, float *Y,
+                  void *stream);
 
 /* Gather (scatter == 0: out[k] = X[perm[k]]) or scatter (out[perm[k]] =
  * X[k]) of the rows of an n x nrhs block: the cluster ordering. */

@@ -183,6 +183,35 @@ def test_visibility_bit_exact_crater(torch_dev, oracle_mod, crater, mode):
     assert bad.size == 0, f"{len(bad)} mismatched pairs, first: {bad[:10].tolist()}"
 
 
+def test_oriented_visibility_subsets(torch_dev, oracle_mod, crater):
+    """get_visibility(I, J, oriented=True) with I != J: the oracle's mask
+    with every pair whose facet-id direction P[J[q]] - P[I[p]] is not in
+    front of N[I[p]] cleared; rows equal get_visibility_1_to_N; on I = J =
+    arange it equals the reference's positional expression."""
+    V, F = crater
+    sm = _model(V, F)
+    n = sm.num_faces
+    S = oracle_mod.OracleScene(V, F)
+    rng = np.random.default_rng(17)
+    I = np.sort(rng.choice(n, 120, replace=False))
+    J = rng.permutation(n)[:300]
+    vis = sm.get_visibility(I, J, oriented=True)
+    ref = S.vis_mn(I, J).astype(bool)
+    front = ((sm.P[J][None, :, :] - sm.P[I][:, None, :]) * sm.N[I][:, None, :]).sum(-1) > 0
+    assert np.array_equal(vis, ref & front)
+    assert (ref & ~front).any()          # the orientation test removed pairs
+    for p in (0, 17, 119):
+        assert np.array_equal(sm.get_visibility_1_to_N(I[p], J, oriented=True), vis[p])
+    # full index sets: identical to the reference's positional form
+    A = np.arange(n)
+    full = sm.get_visibility_matrix(oriented=True)
+    base = S.vis_mn(A, A).astype(bool)
+    I_, J_ = np.where(base)
+    drop = ((sm.P[J_] - sm.P[I_]) * sm.N[I_]).sum(1) <= 0
+    base[I_[drop], J_[drop]] = False
+    assert np.array_equal(full, base)
+
+
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
 @pytest.mark.parametrize("mode", ["cgal", "embree"])
 def test_form_factor_bit_exact(torch_dev, oracle_mod, crater, dt, mode):

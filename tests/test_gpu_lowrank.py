@@ -240,3 +240,60 @@ def test_plain_configuration_full_size(torch_dev, oracle_mod, crater32):
     # a planted change (emissivity 0.9) moves the closed form well outside
     Tbad = ((1 - 0.12) * (Eo + FFo @ Bo) / (0.9 * Stefan_Boltzmann)) ** 0.25
     assert abs(Tc - Tbad).max() > 1e-3 * Tco.max()
+
+
+# --------------------------------------------------------------------------
+# the far field on MFMA (16-bit factors) and the refined solve with it
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("values,tol", [("bf16", 1e-2), ("fp16", 1.5e-3)])
+@pytest.mark.parametrize("left", [False, True])
+def test_far_field_mfma(torch_dev, compressed64, values, tol, left):
+    """FarField16.mm (v_mfma_f32_16x16x16 bf16 / f16) against the FP64
+    far-field product of the same 16-bit X with the master factors; the
+    bound is three roundings of the storage format (factors, S folded into
+    Vt, the projected T) relative to max |y|."""
+    from fluxpy_amd import _hbuild, _lib
+    t = torch_dev
+    H = compressed64
+    master = _hbuild.far_master(H)
+    far = _hbuild.far16(H, values)
+    n = H.shape[0]
+    gen = t.Generator(device="cuda").manual_seed(3)
+    for k in (64, 16, 3):
+        X16 = t.rand((n, k), generator=gen, device="cuda").to(far.dtype)
+        rs32 = t.rand(n, generator=gen, device="cuda") if left else None
+        Y = t.zeros((n, k), dtype=t.float32, device="cuda")
+        far.mm(X16, Y, rscale=rs32)
+        Yref = t.zeros((n, k), dtype=t.float64, device="cuda")
+        master.far_apply(X16.to(t.float64), Yref,
+                         rscale=None if rs32 is None else rs32.to(t.float64))
+        err = float((Y.double() - Yref).abs().max() / Yref.abs().max())
+        assert err <= tol, (values, k, left, err)
+        # repeated calls give identical bits (fixed summation order)
+        Y2 = t.zeros_like(Y)
+        far.mm(X16, Y2, rscale=rs32)
+        assert t.equal(Y, Y2)
+
+
+@pytest.mark.parametrize("left", [False, True])
+def test_refined_solve_with_far_field(torch_dev, oracle_mod, crater32, compressed64, left):
+    """The batched BF16 + refinement solve on a compressed matrix with
+    low-rank leaves (near-field panels + MFMA far field in the sweeps, int8
+    residual + FP64 far field) against GMRES on the same compressed operator
+    (FP64, 1e-13): within 1e-6 relative per sun."""
+    from fluxpy_amd.solve import solve_radiosity, solve_radiosity_refined
+    sm, FF = crater32
+    H = compressed64
+    rho = np.random.default_rng(12).uniform(0.1, 0.3, sm.num_faces)
+    E = sm.get_direct_irradiance_multi(1365.0, meshes.sun_directions(8, seed=13))
+    place = "left" if left else "right"
+    B, info = solve_radiosity_refined(H, H, E, rho, albedo_placement=place, rtol=1e-7,
+                                      inner_iters=2, return_info=True)
+    Bg, _ = solve_radiosity(H, E, rho, place, method="gmres", tol=1e-13)
+    err = abs(B - Bg).max(0) / abs(Bg).max(0)
+    assert err.max() <= 1e-6, (err.max(), info)
+    assert info["residual_history"][-1] <= 1e-7
+    # and within the compression error of the exact (dense) solution
+    Bd = oracle_mod.solve_radiosity_dense(FF, E, rho, place)
+    assert abs(B - Bd).max() <= 0.05 * abs(Bd).max()
