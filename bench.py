@@ -13,10 +13,15 @@ One step = one batched solve of all 64 suns to <= 1e-6 relative error (the
 refinement stops at max|r_c| <= 1e-7 max|E_c|): BF16 near-field panels on
 MFMA for the inner sweeps (one sweep per round by default), and the residual
 of every round on int8-sliced panels with int8 MFMA, FP64-accurate (or,
-with --residual fp64, FP32 master panels on FP64 MFMA).  The plain FP64 path solves one sun at a time with FP64 values
-and vectors on the same compressed matrix, reference Jacobi stopping rule
-(solve.py:6-8), no batching.  Inputs are resident in HBM before the timed
-region; visibility, assembly and compression are timed separately.
+with --residual fp64, FP32 master panels on FP64 MFMA).  The headline tree
+has its low-rank leaves off (the operator is the exact F, more work than the
+compressed one); the same solve on the tree WITH low-rank leaves (far field
+on BF16 MFMA in the sweeps) is reported beside it under "lowrank".  The
+plain FP64 path solves one sun at a time with FP64 values and vectors on the
+same matrix, reference Jacobi stopping rule (solve.py:6-8), no batching; the
+FP64 path with all suns as one block is reported as "fp64_batched".  Inputs
+are resident in HBM before the timed region; visibility, assembly and
+compression are timed separately.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid 145]
 Prints one JSON line on stdout (progress on stderr).
@@ -34,6 +39,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "radiosity solves/s (40k tris, 64 suns) BF16+refine vs FP64 @1e-6; Mrays/s"
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0
+INT8_PEAK_TOPS = 5000.0
 # HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes
 # on this workload (each scaled by the factor a plain 512 MiB copy shows:
 # FETCH_SIZE x 2, WRITE_SIZE x 1); the numbers and the commands are in
@@ -49,6 +56,26 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+class Timer:
+    """HIP-event timing of launches on the current stream."""
+
+    def __init__(self, torch):
+        self.t = torch
+        self.e0 = torch.cuda.Event(enable_timing=True)
+        self.e1 = torch.cuda.Event(enable_timing=True)
+
+    def us(self, fn, reps=20, warm=3):
+        for _ in range(warm):
+            fn()
+        self.t.cuda.synchronize()
+        self.e0.record()
+        for _ in range(reps):
+            fn()
+        self.e1.record()
+        self.t.cuda.synchronize()
+        return self.e0.elapsed_time(self.e1) / reps * 1e3
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -56,7 +83,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--grid", type=int, default=145)
     ap.add_argument("--suns", type=int, default=64)
-    ap.add_argument("--fp64-suns", type=int, default=4)
+    ap.add_argument("--fp64-suns", type=int, default=4,
+                    help="suns timed one at a time on the plain FP64 path")
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--min-size", type=int, default=4096)
     ap.add_argument("--inner", type=int, default=1,
@@ -67,11 +95,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-thermal", action="store_true",
                     help="skip the time-dependent thermal model measurement")
+    ap.add_argument("--no-lowrank", action="store_true",
+                    help="skip the solve on the tree with low-rank leaves")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the adjoint gradient measurement")
     ap.add_argument("--calib", action="store_true",
                     help="end with a 512 MiB device copy (counter calibration)")
-    ap.add_argument("--lowrank", action="store_true",
-                    help="offer blocks to the randomized sketch (low-rank leaves); off by "
-                         "default, so every leaf is dense / CSR and the operator is exact")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: the solve runs on one GPU; --gpus {args.gpus} is not supported "
@@ -79,15 +108,14 @@ def main():
         sys.exit(2)
 
     import torch
-    from fluxpy_amd import _kernels, _lib, meshes
-    from fluxpy_amd.compressed_form_factors import CompressedFormFactorMatrix
+    from fluxpy_amd import _hbuild, _kernels, _lib, meshes
+    from fluxpy_amd.compressed_form_factors import CompressedFormFactorMatrix, _ClusterTree
     from fluxpy_amd.shape import CgalTrimeshShapeModel
     from fluxpy_amd.solve import _panel_engine, solve_radiosity, solve_radiosity_refined
 
     torch.cuda.set_device(0)
     dev = torch.device("cuda", 0)
-    st = _lib.stream()
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    tm = Timer(torch)
 
     # ---------------- workload ----------------
     V, F = meshes.fractal_terrain(grid=args.grid)
@@ -98,40 +126,19 @@ def main():
     D = meshes.sun_directions(args.suns, seed=1002)
     log(f"[bench] facets={nf} suns={args.suns}")
 
-    # ---------------- visibility: rays and Mrays/s ----------------
-    P = _lib.to_dev(sm.P)
-    N = _lib.to_dev(sm.N)
-    I = _lib.to_dev(np.arange(nf, dtype=np.int64))
-    W = (nf + 63) // 64
-    mask = torch.empty((nf, W), dtype=torch.int64, device=dev)
-    cnt = torch.empty(nf, dtype=torch.int64, device=dev)
-    _lib.call("flux_ff_candidates", sm._scene.handle, P.data_ptr(), N.data_ptr(), _lib.F32,
-              I.data_ptr(), nf, I.data_ptr(), nf, 1e-5, mask.data_ptr(), cnt.data_ptr(), st)
-    rays = int(cnt.sum().item())
-    torch.cuda.synchronize()
-    ev0.record()
-    _lib.call("flux_ff_mask", sm._scene.handle, P.data_ptr(), N.data_ptr(), _lib.F32,
-              I.data_ptr(), nf, I.data_ptr(), nf, 1e-5, _lib.MODE_CGAL, mask.data_ptr(),
-              cnt.data_ptr(), st)
-    ev1.record()
-    torch.cuda.synchronize()
-    vis_ms = ev0.elapsed_time(ev1)
-    mrays = rays / (vis_ms * 1e-3) / 1e6
-    del mask
-    log(f"[bench] rays={rays} visibility kernel {vis_ms:.1f} ms -> {mrays:.0f} Mrays/s")
+    vis = visibility(torch, _lib, sm, _ClusterTree, tm)
 
-    # ---------------- compressed matrix ----------------
+    # ---------------- compressed matrix (exact F: low-rank leaves off) ----------------
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    from fluxpy_amd import _hbuild
-    _hbuild.LOWRANK = args.lowrank
+    _hbuild.LOWRANK = False
     H = CompressedFormFactorMatrix(sm, tol=args.tol, min_size=args.min_size)
+    _hbuild.LOWRANK = True
     torch.cuda.synchronize()
     t_build = time.perf_counter() - t0
     bs = H.build_stats
     op64 = H.device_operator("fp64")
-    near = H._near
-    nnz = near.nnz
+    nnz = H._near.nnz
     log(f"[bench] compressed build {t_build:.2f} s (assembly {bs['assembly_s']:.2f} s, "
         f"tree {bs['tree_s']:.2f} s) nbytes={H.nbytes} leaves={bs['leaf_counts']} "
         f"svd_leaves={bs['svd_leaves']} near nnz={nnz}")
@@ -140,15 +147,25 @@ def main():
     E64 = _kernels.convert(E32, torch.float64)
     rho_d = _lib.to_dev(rho)
 
-    # ---------------- plain FP64 path (one sun at a time) ----------------
-    ns64 = min(args.fp64_suns, args.suns)
-    solve_radiosity(op64, E64[:, 0].contiguous(), rho, 'right')  # warm-up
+    # ---------------- FP64 reference solution of every sun (one block) --------
+    solve_radiosity(op64, E64, rho, 'right')                              # warm-up
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    B64, it64 = [], []
+    B64_all, it64b = solve_radiosity(op64, E64, rho, 'right')
+    torch.cuda.synchronize()
+    t64b = time.perf_counter() - t0
+    fp64b_sps = args.suns / t64b
+    log(f"[bench] FP64 batched ({args.suns} suns, one block, {it64b} Jacobi iters): "
+        f"{t64b * 1e3:.1f} ms -> {fp64b_sps:.1f} solves/s")
+
+    # ---------------- plain FP64 path (one sun at a time) ----------------
+    ns64 = min(args.fp64_suns, args.suns)
+    solve_radiosity(op64, E64[:, 0].contiguous(), rho, 'right')           # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    it64 = []
     for s in range(ns64):
-        b, it = solve_radiosity(op64, E64[:, s].contiguous(), rho, 'right')
-        B64.append(b)
+        _, it = solve_radiosity(op64, E64[:, s].contiguous(), rho, 'right')
         it64.append(it)
     torch.cuda.synchronize()
     t64 = (time.perf_counter() - t0) / ns64
@@ -156,29 +173,22 @@ def main():
     log(f"[bench] FP64 path: {t64 * 1e3:.2f} ms/sun ({np.mean(it64):.1f} Jacobi iters) "
         f"-> {fp64_sps:.1f} solves/s")
 
+    def rel_err(B, Bref):
+        """max over suns of max|B_s - Bref_s| / max|Bref_s| (all suns)."""
+        num = (B - Bref).abs().amax(0)
+        den = Bref.abs().amax(0).clamp_min(1e-300)
+        return float((num / den).max())
+
     # ---------------- BF16 + refinement, batched ----------------
-    pure_near = op64.nfar == 0
-    if pure_near:
-        # the panels are built on the first solve and kept on H
-        Pbf, Pres = _panel_engine(H, args.residual)
-        log(f"[bench] panels: fill {Pbf.fill:.3f}, bf16 {Pbf.nbytes / 1e6:.0f} MB, "
-            f"{args.residual} residual {Pres.nbytes / 1e6:.0f} MB")
+    Pbf, Pres, _, _ = _panel_engine(H, args.residual)
+    log(f"[bench] panels: fill {Pbf.fill:.3f}, bf16 {Pbf.nbytes / 1e6:.0f} MB, "
+        f"{args.residual} residual {Pres.nbytes / 1e6:.0f} MB")
 
-        def make_step(residual):
-            def step():
-                return solve_radiosity_refined(H, H, E64, rho_d, rtol=1e-7,
-                                               inner_iters=args.inner, return_info=True,
-                                               residual=residual)
-            return step
-    else:
-        opbf = H.device_operator("bf16")
-        op32 = H.device_operator("fp32")
-
-        def make_step(residual):
-            def step():
-                return solve_radiosity_refined(opbf, op32, E64, rho, rtol=1e-7,
-                                               inner_iters=args.inner, return_info=True)
-            return step
+    def make_step(M, residual):
+        def step():
+            return solve_radiosity_refined(M, M, E64, rho_d, rtol=1e-7, inner_iters=args.inner,
+                                           return_info=True, residual=residual)
+        return step
 
     def timed(step, steps, warmup):
         for _ in range(warmup):
@@ -188,94 +198,27 @@ def main():
         for _ in range(steps):
             B, info = step()
         torch.cuda.synchronize()
-        t = (time.perf_counter() - t0) / steps
-        errs = [float((B[:, s] - B64[s]).abs().max() / B64[s].abs().max())
-                for s in range(ns64)]
-        return t, B, info, max(errs)
+        return (time.perf_counter() - t0) / steps, B, info
 
-    t_step, B, info, err = timed(make_step(args.residual), args.steps, args.warmup)
+    t_step, B, info = timed(make_step(H, args.residual), args.steps, args.warmup)
+    err = rel_err(B, B64_all)
     sps = args.suns / t_step
     log(f"[bench] BF16+refine ({args.residual} residual): {t_step * 1e3:.2f} ms/step, "
-        f"{sps:.0f} solves/s, rounds={info['rounds']} max rel err vs FP64 {err:.2e}")
-    alt = None
-    if pure_near:
-        other = "fp64" if args.residual == "int8" else "int8"
-        t_alt, _, info_alt, err_alt = timed(make_step(other), 2, 1)
-        alt = {"residual": other, "solves_per_s": round(args.suns / t_alt, 2),
-               "rounds": info_alt["rounds"], "max_rel_err_vs_fp64": err_alt}
-        log(f"[bench] BF16+refine ({other} residual): {t_alt * 1e3:.2f} ms/step, "
-            f"{args.suns / t_alt:.0f} solves/s, rounds={info_alt['rounds']} "
-            f"max rel err vs FP64 {err_alt:.2e}")
+        f"{sps:.0f} solves/s, rounds={info['rounds']} max rel err vs FP64 "
+        f"(all {args.suns} suns) {err:.2e}")
+    other = "fp64" if args.residual == "int8" else "int8"
+    t_alt, B_alt, info_alt = timed(make_step(H, other), 2, 1)
+    alt = {"residual": other, "solves_per_s": round(args.suns / t_alt, 2),
+           "rounds": info_alt["rounds"], "max_rel_err_vs_fp64": rel_err(B_alt, B64_all)}
+    log(f"[bench] BF16+refine ({other} residual): {t_alt * 1e3:.2f} ms/step, "
+        f"{args.suns / t_alt:.0f} solves/s")
+    del B_alt
 
-    # ---------------- roofline of the dominant kernel ----------------
-    # Both product kernels of a round are timed with HIP events on the stream
-    # they run on; the roofline reports the one with the larger share of a
-    # solve step (launch time x launches per step), the other goes beside it.
-    # Algorithmic bytes per launch (stored element sizes; panel zeros and
-    # cache re-reads not counted):
-    #   BF16 sweep:    nnz (2 value + 4 index) + 4 (n + 1)
-    #                  + k n (2 X in + 4 out + 4 base in + 2 next X out)
-    #   int8 residual: nnz (5 slice bytes + 4 index) + 4 (n + 1)
-    #                  + k n (5 X slices + 8 E + 8 B + 8 out)
-    #   int8 update residual (rounds after the first, 4 slices):
-    #                  nnz (4 + 4) + 4 (n + 1) + k n (4 + 8 r_prev + 8 D + 8 out)
-    k = args.suns
-    reps = 20
-    kernels = {}
-
-    def time_launch(fn):
-        for _ in range(3):
-            fn()
-        torch.cuda.synchronize()
-        ev0.record()
-        for _ in range(reps):
-            fn()
-        ev1.record()
-        torch.cuda.synchronize()
-        return ev0.elapsed_time(ev1) / reps * 1e3          # us per launch
-
-    if pure_near:
-        Xb = torch.zeros((nf, k), dtype=torch.bfloat16, device=dev)
-        Xn = torch.empty_like(Xb)
-        out = torch.empty((nf, k), dtype=torch.float32, device=dev)
-        base = torch.zeros_like(out)
-        rho32 = rho_d.to(torch.float32)
-        us = time_launch(lambda: Pbf.mm_bf16(Xb, out, base=base, rho=rho32, xnext=Xn))
-        kernels["k_panel_mm_bf16 + k_panel_reduce_bf16"] = {
-            "us": us, "per_step": info["rounds"] * args.inner,
-            "bytes": nnz * (2 + 4) + 4 * (nf + 1) + k * nf * (2 + 4 + 4 + 2)}
-        del Xb, Xn, out, base
-        P8 = _panel_engine(H, "int8")[1]
-        X64 = torch.rand((nf, k), dtype=torch.float64, device=dev)
-        R64 = torch.empty_like(X64)
-        cmx = torch.empty(k, dtype=torch.float64, device=dev)
-        P8.slice_x(X64, xscale=rho_d)
-        us = time_launch(lambda: P8.product(k, R64, E=X64, B=X64, colmax=cmx))
-        kernels["k_panel8_mm<5> + k_panel8_reduce + k_colmax_blocks"] = {
-            "us": us, "per_step": 1 if args.residual == "int8" else 0,
-            "bytes": nnz * (5 + 4) + 4 * (nf + 1) + k * nf * (5 + 8 + 8 + 8)}
-        us = time_launch(lambda: P8.product(k, R64, E=X64, B=X64, colmax=cmx, nslices=4))
-        kernels["k_panel8_mm<4> + k_panel8_reduce + k_colmax_blocks"] = {
-            "us": us,
-            "per_step": (info["rounds"] - 1 - info.get("full_residual_fallbacks", 0)
-                         if args.residual == "int8" else 0),
-            "bytes": nnz * (4 + 4) + 4 * (nf + 1) + k * nf * (4 + 8 + 8 + 8)}
-        us_slice = time_launch(lambda: P8.slice_x(X64, xscale=rho_d))
-        log(f"[bench] X slicing (k_colmax_scaled + k_colmax_final + k_slice_x): {us_slice:.1f} us")
-        del X64, R64
-    else:
-        us = time_launch(lambda: opbf.apply(_kernels.convert(E64, torch.float32)))
-        kernels["k_csr_rows<bf16> + far field"] = {
-            "us": us, "per_step": info["rounds"] * args.inner,
-            "bytes": nnz * (2 + 4) + 4 * (nf + 1) + k * nf * (2 + 4)}
-    for name, kv in kernels.items():
-        kv["GB/s"] = kv["bytes"] / (kv["us"] * 1e-6) / 1e9
-        log(f"[bench] {name}: {kv['us']:.1f} us/launch, {kv['GB/s']:.0f} GB/s algorithmic, "
-            f"{kv['per_step']} launches per step")
+    # ---------------- kernels: roofline, MFMA utilisation ----------------
+    kernels = panel_kernels(torch, tm, H, Pbf, info, args, nf, nnz, rho_d, dev)
     kname = max(kernels, key=lambda n: kernels[n]["us"] * max(kernels[n]["per_step"], 1e-9))
-    k_ms = kernels[kname]["us"] * 1e-3
-    alg_bytes = kernels[kname]["bytes"]
     achieved = kernels[kname]["GB/s"]
+    hmv = hmatvec(torch, tm, H, nf, dev)
 
     if args.calib:
         # a plain device copy of known size: its FETCH_SIZE / WRITE_SIZE counters
@@ -289,11 +232,15 @@ def main():
             f"{dst.numel() * 4} bytes written per launch")
         del src, dst
 
-    thermal = None if args.no_thermal else thermal_steps(H, sm, rho, ev0, ev1)
+    # ---------------- the tree with low-rank leaves ----------------
+    lowrank = None
+    if not args.no_lowrank:
+        lowrank = lowrank_line(torch, sm, args, E64, rho, rho_d, B64_all, rel_err, timed,
+                               make_step, CompressedFormFactorMatrix, solve_radiosity)
 
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(sm, H, E64, rho, V, F)
+    adjoint = None if args.no_adjoint else adjoint_line(torch, sm, H, rho, D)
+    thermal = None if args.no_thermal else thermal_steps(H, sm, rho, tm)
+    cpu = None if args.no_cpu else cpu_baseline(sm, H, E64, rho, V, F)
 
     out = {
         "metric": METRIC,
@@ -310,42 +257,230 @@ def main():
         "data": "synthetic (seeded fractal terrain, seeded suns and albedo)",
         "config": {
             "workload": "fractal terrain 145^2 vertices (41,472 facets), 64 suns, compressed "
-                        "F tree (tol 1e-6, min_size 4096%s), refinement to max|r| <= 1e-7 "
-                        "max|E| per sun: %d BF16 panel sweep(s) per round, %s residual"
-                        % ("" if args.lowrank else ", low-rank leaves off: exact F", args.inner,
-                           "FP64-accurate int8-sliced" if args.residual == "int8"
+                        "F tree (tol 1e-6, min_size 4096, low-rank leaves off: exact F), "
+                        "refinement to max|r| <= 1e-7 max|E| per sun: %d BF16 panel sweep(s) "
+                        "per round, %s residual"
+                        % (args.inner, "FP64-accurate int8-sliced" if args.residual == "int8"
                            else "FP64-MFMA"),
             "inner_sweeps": args.inner,
             "facets": nf, "suns": args.suns, "near_nnz": nnz, "tol": args.tol,
             "min_size": args.min_size, "compressed_nbytes": int(H.nbytes),
             "leaves": bs["leaf_counts"], "svd_leaves": bs["svd_leaves"],
-            "lowrank": bool(args.lowrank),
         },
         "fp64_solves_per_s": round(fp64_sps, 2),
+        "fp64_batched_solves_per_s": round(fp64b_sps, 2),
         "speedup_vs_fp64": round(sps / fp64_sps, 2),
+        "speedup_vs_fp64_batched": round(sps / fp64b_sps, 2),
         "max_rel_err_vs_fp64": err,
+        "error_checked_suns": args.suns,
         "residual": args.residual,
         "other_residual": alt,
         "refine_rounds": info["rounds"],
-        "mrays_per_s": round(mrays, 1),
-        "rays": rays,
+        "mrays_per_s": vis["mrays_per_s"],
+        "visibility": vis,
+        "rays": vis["rays"],
         "compress_s": round(t_build, 3),
         "roofline": {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": MEASURED_TRAFFIC.get(kname),
-            "kernel": kname, "kernel_us": round(k_ms * 1e3, 2), "algorithmic_bytes": alg_bytes,
+            "kernel": kname, "kernel_us": round(kernels[kname]["us"], 2),
+            "algorithmic_bytes": kernels[kname]["bytes"],
         },
         "kernels": {n: {"us": round(v["us"], 2), "GB/s": round(v["GB/s"], 1),
-                        "launches_per_step": v["per_step"], "algorithmic_bytes": v["bytes"]}
+                        "launches_per_step": v["per_step"], "algorithmic_bytes": v["bytes"],
+                        "mfma": v.get("mfma")}
                     for n, v in kernels.items()},
+        "hmatvec_1rhs": hmv,
+        "lowrank": lowrank,
+        "adjoint": adjoint,
         "thermal": thermal,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 
 
-def thermal_steps(H, sm, rho, ev0, ev1, nsteps=20):
+def visibility(torch, _lib, sm, ClusterTree, tm):
+    """Rays of the form-factor pass and its mask kernel's rate, warmed, in
+    facet order and in cluster order (the order the compressed build casts
+    them in: neighbouring rows then trace neighbouring rays)."""
+    nf = sm.num_faces
+    dev = torch.device("cuda", 0)
+    st = _lib.stream()
+    P = _lib.to_dev(sm.P)
+    N = _lib.to_dev(sm.N)
+    W = (nf + 63) // 64
+    mask = torch.empty((nf, W), dtype=torch.int64, device=dev)
+    cnt = torch.empty(nf, dtype=torch.int64, device=dev)
+    out = {}
+    orders = {"facet": np.arange(nf, dtype=np.int64),
+              "cluster": ClusterTree(np.asarray(sm.P), 2).perm.astype(np.int64)}
+    for name, order in orders.items():
+        I = _lib.to_dev(order)
+        _lib.call("flux_ff_candidates", sm._scene.handle, P.data_ptr(), N.data_ptr(), _lib.F32,
+                  I.data_ptr(), nf, I.data_ptr(), nf, 1e-5, mask.data_ptr(), cnt.data_ptr(), st)
+        rays = int(cnt.sum().item())
+
+        def launch():
+            _lib.call("flux_ff_mask", sm._scene.handle, P.data_ptr(), N.data_ptr(), _lib.F32,
+                      I.data_ptr(), nf, I.data_ptr(), nf, 1e-5, _lib.MODE_CGAL,
+                      mask.data_ptr(), cnt.data_ptr(), st)
+        ms = tm.us(launch, reps=2, warm=1) * 1e-3
+        out[name] = {"ms": round(ms, 2), "mrays_per_s": round(rays / (ms * 1e-3) / 1e6, 1)}
+        log(f"[bench] visibility ({name} order, warmed): rays={rays} {ms:.1f} ms -> "
+            f"{out[name]['mrays_per_s']:.0f} Mrays/s")
+    out["rays"] = rays
+    out["mrays_per_s"] = max(out["facet"]["mrays_per_s"], out["cluster"]["mrays_per_s"])
+    del mask
+    return out
+
+
+def panel_kernels(torch, tm, H, Pbf, info, args, nf, nnz, rho_d, dev):
+    """Both product kernels of a round timed with HIP events on the stream
+    they run on, with algorithmic bytes per launch (stored element sizes;
+    panel zeros and cache re-reads not counted):
+       BF16 sweep:    nnz (2 value + 4 index) + 4 (n + 1)
+                      + k n (2 X in + 4 out + 4 base in + 2 next X out)
+       int8 residual: nnz (5 slice bytes + 4 index) + 4 (n + 1)
+                      + k n (5 X slices + 8 E + 8 B + 8 out)
+       int8 update residual (rounds after the first, 4 slices):
+                      nnz (4 + 4) + 4 (n + 1) + k n (4 + 8 r_prev + 8 D + 8 out)
+    and the MFMA work each launch executes: 2 x 64 rows x padded columns x k
+    (BF16, every subtile), 2 x 16 rows x 64 columns x k per nonzero 16-row
+    subtile and slice pair (int8: 15 pairs on 5 slices, 10 on 4)."""
+    from fluxpy_amd.solve import _panel_engine
+    k = args.suns
+    kernels = {}
+    Xb = torch.zeros((nf, k), dtype=torch.bfloat16, device=dev)
+    Xn = torch.empty_like(Xb)
+    out = torch.empty((nf, k), dtype=torch.float32, device=dev)
+    base = torch.zeros_like(out)
+    rho32 = rho_d.to(torch.float32)
+    us = tm.us(lambda: Pbf.mm_bf16(Xb, out, base=base, rho=rho32, xnext=Xn))
+    flops = 2.0 * 64 * Pbf.total_cols * k
+    kernels["k_panel_mm_bf16 + k_panel_reduce_bf16"] = {
+        "us": us, "per_step": info["rounds"] * args.inner,
+        "bytes": nnz * (2 + 4) + 4 * (nf + 1) + k * nf * (2 + 4 + 4 + 2),
+        "mfma": {"type": "bf16", "TFLOP/s": round(flops / us / 1e6, 1),
+                 "peak": BF16_PEAK_TFLOPS, "frac": round(flops / us / 1e6 / BF16_PEAK_TFLOPS, 4)}}
+    del Xb, Xn, out, base
+    P8 = _panel_engine(H, "int8")[1]
+    nz_sub = int(P8.qmask[:4 * P8.quads].to(torch.int64).sum().item())
+    X64 = torch.rand((nf, k), dtype=torch.float64, device=dev)
+    R64 = torch.empty_like(X64)
+    cmx = torch.empty(k, dtype=torch.float64, device=dev)
+    P8.slice_x(X64, xscale=rho_d)
+    for ns, pairs, per_step in ((5, 15, 1 if args.residual == "int8" else 0),
+                                (4, 10, (info["rounds"] - 1 - info.get("full_residual_fallbacks", 0)
+                                         if args.residual == "int8" else 0))):
+        us = tm.us(lambda: P8.product(k, R64, E=X64, B=X64, colmax=cmx, nslices=ns))
+        ops = 2.0 * 16 * 64 * k * nz_sub * pairs
+        kernels[f"k_panel8_mm<{ns}> + k_panel8_reduce + k_colmax_blocks"] = {
+            "us": us, "per_step": per_step,
+            "bytes": nnz * (ns + 4) + 4 * (nf + 1) + k * nf * (ns + 8 + 8 + 8),
+            "mfma": {"type": "int8", "TOP/s": round(ops / us / 1e6, 1), "peak": INT8_PEAK_TOPS,
+                     "frac": round(ops / us / 1e6 / INT8_PEAK_TOPS, 4)}}
+    us_slice = tm.us(lambda: P8.slice_x(X64, xscale=rho_d))
+    log(f"[bench] X slicing (k_colmax_scaled + k_colmax_final + k_slice_x): {us_slice:.1f} us")
+    del X64, R64
+    for name, kv in kernels.items():
+        kv["GB/s"] = kv["bytes"] / (kv["us"] * 1e-6) / 1e9
+        log(f"[bench] {name}: {kv['us']:.1f} us/launch, {kv['GB/s']:.0f} GB/s algorithmic, "
+            f"{kv['per_step']} launches per step, MFMA {kv['mfma']}")
+    return kernels
+
+
+def hmatvec(torch, tm, H, nf, dev):
+    """Single right-hand-side H-matvec (the near-field CSR of the exact tree)
+    in FP64 and FP32 values: algorithmic bytes nnz (value + 4 index) +
+    8 (n + 1) row pointers + n (x + y) per launch."""
+    out = {}
+    for values, vb in (("fp64", 8), ("fp32", 4)):
+        op = H.device_operator(values)
+        X = torch.rand((nf, 1), dtype=op.vec_dtype, device=dev)
+        Y = torch.empty_like(X)
+        us = tm.us(lambda: op.near.matmat(X, out=Y))
+        nnz = op.near.nnz
+        nbytes = nnz * (vb + 4) + 8 * (nf + 1) + nf * 2 * X.element_size()
+        gbs = nbytes / (us * 1e-6) / 1e9
+        out[values] = {"us": round(us, 2), "bytes": nbytes, "GB/s": round(gbs, 1),
+                       "frac": round(gbs / HBM_PEAK_GBS, 4)}
+        log(f"[bench] H-matvec 1 rhs ({values}): {us:.1f} us, {gbs:.0f} GB/s "
+            f"({gbs / HBM_PEAK_GBS:.2f} of HBM peak)")
+    return out
+
+
+def lowrank_line(torch, sm, args, E64, rho, rho_d, B64_all, rel_err, timed, make_step,
+                 CompressedFormFactorMatrix, solve_radiosity):
+    """The same batched solve on the tree WITH low-rank leaves: near-field
+    panels plus the far field (BF16 MFMA in the sweeps, FP32 master factors
+    with FP64 vectors in the residual)."""
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    H = CompressedFormFactorMatrix(sm, tol=args.tol, min_size=args.min_size)
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t0
+    bs = H.build_stats
+    op64 = H.device_operator("fp64")
+    Bc, _ = solve_radiosity(op64, E64, rho, 'right')                # FP64, same operator
+    t_step, B, info = timed(make_step(H, args.residual), max(5, args.steps // 5), 2)
+    from fluxpy_amd import _hbuild
+    far = _hbuild.far16(H, "bf16")
+    res = {"solves_per_s": round(args.suns / t_step, 2), "ms_per_step": round(t_step * 1e3, 3),
+           "rounds": info["rounds"], "build_s": round(t_build, 2),
+           "tree_s": round(bs["tree_s"], 2), "compressed_nbytes": int(H.nbytes),
+           "svd_leaves": bs["svd_leaves"], "leaves": bs["leaf_counts"],
+           "near_nnz": H._near.nnz, "far_factor_bytes_bf16": far.nbytes,
+           "max_rel_err_vs_fp64_same_operator": rel_err(B, Bc),
+           "max_rel_err_vs_fp64_exact_F": rel_err(B, B64_all)}
+    log(f"[bench] low-rank tree: build {t_build:.1f} s, {bs['svd_leaves']} SVD leaves, "
+        f"nbytes {H.nbytes / 1e6:.0f} MB, {res['solves_per_s']:.0f} solves/s, err vs FP64 "
+        f"same operator {res['max_rel_err_vs_fp64_same_operator']:.1e}, vs exact F "
+        f"{res['max_rel_err_vs_fp64_exact_F']:.1e}")
+    del H, op64, Bc, B
+    torch.cuda.empty_cache()
+    return res
+
+
+def adjoint_line(torch, sm, H, rho, D, nsuns=16, nfd=8):
+    """Gradient of L = sum (T - T_obs)^2 w.r.t. albedo, emissivity and
+    insolation for 16 suns by the adjoint solve on the exact F (the
+    differentiation configuration's shape), timed; central differences
+    (h = 1e-6 relative) on `nfd` random facets' albedo."""
+    from fluxpy_amd.adjoint import steady_state_temperature, temperature_loss_gradient
+    nf = sm.num_faces
+    rng = np.random.default_rng(1005)
+    r = rng.uniform(0.08, 0.16, nf)
+    e = rng.uniform(0.9, 0.98, nf)
+    E = sm.get_direct_irradiance_multi(1365.0, D[:nsuns]).astype(np.float64)
+    r_star = r * (1 + 0.1 * rng.standard_normal(nf))
+    T_obs = steady_state_temperature(H, r_star, e, E).cpu().numpy()
+    temperature_loss_gradient(H, r, e, E, T_obs)                       # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    L, gr, ge, gE = temperature_loss_gradient(H, r, e, E, T_obs)
+    torch.cuda.synchronize()
+    t_grad = time.perf_counter() - t0
+    gr = gr.cpu().numpy()
+    idx = rng.choice(np.where(E.max(1) > 0)[0], nfd, replace=False)
+    fd = []
+    for i in idx:
+        h = 1e-6 * r[i]
+        vals = []
+        for sgn in (1, -1):
+            rr = r.copy()
+            rr[i] += sgn * h
+            T = steady_state_temperature(H, rr, e, E).cpu().numpy()
+            vals.append(((T - T_obs) ** 2).sum())
+        fd.append((vals[0] - vals[1]) / (2 * h))
+    fd_err = float(np.max(np.abs(np.array(fd) - gr[idx])) / np.abs(gr).max())
+    log(f"[bench] adjoint gradient ({nsuns} suns): {t_grad * 1e3:.1f} ms, central-difference "
+        f"error on {nfd} facets {fd_err:.1e}")
+    return {"suns": nsuns, "gradient_ms": round(t_grad * 1e3, 2), "fd_facets": nfd,
+            "fd_max_rel_err": fd_err, "loss": L}
+
+
+def thermal_steps(H, sm, rho, tm, nsteps=20):
     """The time-dependent thermal model (model.py:108-234) on the same terrain
     and F: hourly steps of a sun circling at 10 degrees elevation, 60
     subsurface layers.  One step = device insolation, the two-column product
@@ -367,30 +502,27 @@ def thermal_steps(H, sm, rho, ev0, ev1, nsteps=20):
                    np.full(nt, np.sin(el))], 1)
     nz = 60
     z = np.concatenate([[0.0], setgrid(nz, 1.05, 2.5)])
-    tm = ThermalModel(H, t, Ds, np.full(nt, 1365.0), rho, device=True, z=z, T0=200.0,
-                      ti=120.0, rhoc=960000.0, emiss=0.95, Fgeotherm=0.2, shape_model=sm)
+    tmodel = ThermalModel(H, t, Ds, np.full(nt, 1365.0), rho, device=True, z=z, T0=200.0,
+                          ti=120.0, rhoc=960000.0, emiss=0.95, Fgeotherm=0.2, shape_model=sm)
     for _ in range(3):
-        next(tm)
+        next(tmodel)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(nsteps):
-        next(tm)
+        next(tmodel)
     torch.cuda.synchronize()
     step_ms = (time.perf_counter() - t0) / nsteps * 1e3
-    pcc = tm.pcc_thermal_model_1d
+    pcc = tmodel.pcc_thermal_model_1d
     nf = pcc.num_faces
-    Q = tm._Q.clone()
-    reps = 20
-    ev0.record()
-    for _ in range(reps):
+    Q = tmodel._Q.clone()
+
+    def launch():
         _lib.call("flux_conduction_q", nf, nz, pcc._z.data_ptr(), 3600.0, Q.data_ptr(),
                   Q.data_ptr(), pcc._T.data_ptr(), pcc._T_next.data_ptr(), pcc._ti.data_ptr(),
                   pcc._rhoc.data_ptr(), pcc._emiss.data_ptr(), pcc._Fgeotherm.data_ptr(),
                   pcc._Fsurf_next.data_ptr(), pcc._work.data_ptr(), pcc._status.data_ptr(),
                   _lib.stream())
-    ev1.record()
-    torch.cuda.synchronize()
-    us = ev0.elapsed_time(ev1) / reps * 1e3
+    us = tm.us(launch, warm=0)
     nbytes = nf * (2 * 8 * (nz + 1) + 2 * 8 * nz + 5 * 8)
     log(f"[bench] thermal model: {step_ms:.3f} ms/step; flux_conduction_q {us:.1f} us "
         f"({nbytes / us / 1e3:.0f} GB/s algorithmic), {nf} facets x {nz} layers")
@@ -400,29 +532,59 @@ def thermal_steps(H, sm, rho, ev0, ev1, nsteps=20):
 
 
 def cpu_baseline(sm, H, E64, rho, V, F):
-    """The oracle (CPU port) on a bounded sample: one sun's FP64 Jacobi with
-    SciPy CSR (one core) on the same near field, and the C BVH ray caster on
-    a row sample (all cores)."""
+    """The oracle (CPU port) on a bounded sample of the same workload:
+    FP64 Jacobi (reference stopping rule) with SciPy CSR on the same F, one
+    sun per thread (SciPy's CSR product releases the GIL), on one core and
+    on all cores; and the C BVH form-factor rows (OpenMP) on a row sample,
+    on all cores and on one core."""
+    import ctypes
+    from concurrent.futures import ThreadPoolExecutor
     from oracle import oracle as O
     sp = H._near.to_scipy().astype(np.float64)
     perm = H._ctree.perm
-    E = E64[:, 0].cpu().numpy()[perm]
+    Eh = E64.cpu().numpy()[perm]
+    rp = rho[perm]
+    cores = min(os.cpu_count() or 1, 16, Eh.shape[1])
+
+    def one(s):
+        O.solve_radiosity_jacobi(sp, np.ascontiguousarray(Eh[:, s]), rp, "right")
+
     t0 = time.perf_counter()
-    O.solve_radiosity_jacobi(sp, E, rho[perm], "right")
-    t_cpu = time.perf_counter() - t0
+    one(0)
+    t1 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(cores) as ex:
+        list(ex.map(one, range(cores)))
+    tall = time.perf_counter() - t0
+    # form-factor rows: C BVH on a row sample, all OpenMP threads, then one
     S = O.OracleScene(V, F)
     rng = np.random.default_rng(7)
-    nrows = 64
-    I = np.sort(rng.choice(sm.num_faces, nrows, replace=False))
+    I = np.sort(rng.choice(sm.num_faces, 32, replace=False))
     t0 = time.perf_counter()
     csr = S.form_factor_matrix(sm.P, sm.N, sm.A, I)
     t_rows = time.perf_counter() - t0
-    cores = os.cpu_count()
-    log(f"[bench] CPU: FP64 Jacobi {t_cpu:.2f} s/sun, {nrows} FF rows in {t_rows:.2f} s")
-    return {"value": round(1.0 / t_cpu, 4), "unit": "solves/s", "cores": 1, "kind": "port",
-            "sample": "1 sun, FP64 Jacobi (reference stopping rule) with SciPy CSR on the "
-                      "same near field; FF rows %d/%d with the C BVH on %d threads took "
-                      "%.2f s (%d nnz)" % (nrows, sm.num_faces, cores, t_rows, csr.nnz)}
+    t_rows1 = None
+    try:
+        gomp = ctypes.CDLL("libgomp.so.1")
+        nthr = gomp.omp_get_max_threads()
+        gomp.omp_set_num_threads(1)
+        t0 = time.perf_counter()
+        S.form_factor_matrix(sm.P, sm.N, sm.A, I[:8])
+        t_rows1 = (time.perf_counter() - t0) * 4          # scaled to the 32 rows
+        gomp.omp_set_num_threads(nthr)
+    except OSError:
+        nthr = None
+    log(f"[bench] CPU: FP64 Jacobi {t1:.2f} s/sun on 1 core, {cores} suns in {tall:.2f} s on "
+        f"{cores} threads; 32 FF rows in {t_rows:.2f} s ({nthr} threads)"
+        + (f", {t_rows1:.2f} s on 1 thread" if t_rows1 else ""))
+    return {"value": round(cores / tall, 4), "unit": "solves/s", "cores": cores, "kind": "port",
+            "sample": "%d suns (one per thread), FP64 Jacobi with the reference stopping rule "
+                      "and SciPy CSR on the same %d-nnz F; 1 core: %.4f solves/s"
+                      % (cores, sp.nnz, 1.0 / t1),
+            "one_core": {"value": round(1.0 / t1, 4), "unit": "solves/s", "cores": 1},
+            "ff_rows": {"rows": 32, "nnz": int(csr.nnz), "s_all_threads": round(t_rows, 3),
+                        "threads": nthr,
+                        "s_one_thread": None if t_rows1 is None else round(t_rows1, 3)}}
 
 
 if __name__ == "__main__":
