@@ -142,8 +142,18 @@ class PanelMatrix:
         _lib.call("flux_panel_mm_bf16", self.shape[0], self.colptr.data_ptr(),
                   self.cols.data_ptr(), self.panel.data_ptr(), Xb.data_ptr(), k, _lib.ptr(base),
                   _lib.ptr(rho), float(rho_s), int(left), out.data_ptr(), _lib.ptr(xnext),
-                  _lib.ptr(prev), _lib.ptr(colmax), *plan, _lib.stream())
+                  _lib.ptr(prev), _lib.ptr(colmax), *plan, int(self.stage), _lib.stream())
         return out
+
+    # 16-column chunks per pipeline stage of the BF16 product (0: library
+    # default); a per-matrix tuning choice that leaves the result unchanged
+    stage = 0
+
+    def replan(self, cap):
+        """A new work plan with pieces of at most `cap` quads (shared with
+        the siblings that hold this plan)."""
+        self.plan = PanelPlan(self.colptr, (self.shape[0] + self.ROWS - 1) // self.ROWS, cap)
+        return self.plan
 
     def mm_f64(self, X, out, E=None, B=None, rho=None, rho_s=1.0, left=False, colmax=None):
         """out = P X, or with E: out = E + s (P X) - B and colmax = max|out|."""
@@ -436,6 +446,37 @@ class PanelRefinedSolver:
         """True if the update X is at most 2^-7 of the first round's X in
         every column (then UPDATE_SLICES slices keep round 1's bound)."""
         return bool(np.all(xmax_d <= np.ldexp(xmax_b, -7)))
+
+    def issue(self, E, inner_iters=1):
+        """The first two refinement rounds with no host synchronisation (so
+        a HIP graph can capture them): round 1 from B = 0 with the full
+        residual, round 2 with the update residual.  Returns the device
+        buffer that receives B; ``verify`` reads the checks once the work
+        has run."""
+        sched = [int(inner_iters)] if np.ndim(inner_iters) == 0 else [int(v) for v in inner_iters]
+        if not sched or min(sched) < 1:
+            raise ValueError("inner_iters: at least one sweep per round")
+        Bs, Rs, cms = self.Bs, self.Rs, self.cms
+        self._round(E, E, None, Bs[0], Rs[0], cms[0], sched[0], 0, src_max=self.emax)
+        self._issued_update = self._round(E, Rs[0], Bs[0], Bs[1], Rs[1], cms[1],
+                                          sched[min(1, len(sched) - 1)], 1, r_prev=Rs[0])
+        return Bs[1]
+
+    def verify(self, rtol=1e-7):
+        """After an ``issue``: (ok, max over columns of max|r_c| / max|E_c|,
+        update residual valid).  ok requires both."""
+        t = _lib.torch()
+        k = self.k
+        parts = [self.cms[1], self.emax.reshape(-1)]
+        sliced = isinstance(self.P_hi, SlicedPanelMatrix)
+        if sliced:
+            parts += [self.P_hi.xmax(k, 0), self.P_hi.xmax(k, 1)]
+        got = t.cat(parts).cpu().numpy()
+        rmax, emax = got[:k], got[k:2 * k]
+        upd_ok = (not self._issued_update or
+                  self._update_ok(got[2 * k:3 * k], got[3 * k:4 * k]))
+        rel = float((rmax / np.where(emax > 0, emax, 1.0)).max())
+        return bool(upd_ok and rel <= rtol), rel, bool(upd_ok)
 
     def solve(self, E, rtol=1e-7, inner_iters=4, max_rounds=30, finish=None):
         """E: n x k FP64 device tensor in panel order.  Returns (B, info); B

@@ -974,8 +974,9 @@ extern "C" int flux_panel_mm_bf16(int64_t nrows, const int64_t *colptr, const in
                                   float *out, void *xnext, const float *prev, double *colmax,
                                   const int32_t *items, int64_t n_items, const int32_t *nsplit,
                                   const int32_t *slot0, const int32_t *split_blocks,
-                                  int64_t n_split, float *ws, void *stream) {
+                                  int64_t n_split, float *ws, int stage, void *stream) {
   FLUX_CHECK(nrows >= 0 && nrhs >= 0, "flux_panel_mm_bf16: bad sizes");
+  FLUX_CHECK(stage == 0 || stage == 4 || stage == 8, "flux_panel_mm_bf16: stage must be 0, 4 or 8");
   const int64_t nblk = (nrows + kPanelRows - 1) / kPanelRows;
   if (nblk == 0 || nrhs == 0) return 0;
   FLUX_CHECK(colptr && cols && panel && X && out, "flux_panel_mm_bf16: NULL array");
@@ -996,7 +997,7 @@ extern "C" int flux_panel_mm_bf16(int64_t nrows, const int64_t *colptr, const in
                      (const __hip_bfloat16 *)X, nrhs, base, rho, rho_s, left, out,                 \
                      (__hip_bfloat16 *)xnext, prev, colmax, items, nsplit, slot0, ws, ngrid)
   const bool vec = (nrhs & 3) == 0;
-  if (panel_stage() == 4) {
+  if ((stage ? stage : panel_stage()) == 4) {
     if (vec)
       MMB(4, true);
     else

@@ -116,6 +116,26 @@ def spherical_crater(npts=32, seed=1001, rim=0.5, depth=0.15, jitter=0.3):
     return V, _orient_up(V, F)
 
 
+def batch_instances(count=32, nmin=1000, nmax=20000, seed=1003):
+    """The batching configuration's meshes: `count` instances with facet
+    counts log-uniform in [nmin, nmax], alternately spherical craters and
+    fractal terrains (seeded shapes, depth / relief 0.05-0.3).  Returns a
+    list of (V, F) pairs; the facet counts land within a few percent of the
+    drawn target (the generators make 2 (g - 1)^2 facets on a g x g grid)."""
+    rng = np.random.default_rng(seed)
+    targets = np.exp(rng.uniform(np.log(nmin), np.log(nmax), count))
+    depths = rng.uniform(0.05, 0.3, count)
+    out = []
+    for i, (nt, dp) in enumerate(zip(targets, depths)):
+        g = max(4, int(round(np.sqrt(nt / 2.0))) + 1)
+        if i % 2 == 0:
+            out.append(spherical_crater(npts=g, seed=seed + 17 * i, depth=dp))
+        else:
+            out.append(fractal_terrain(grid=g, levels=max(4, int(np.ceil(np.log2(g - 1)))),
+                                       relief=dp / 2.0, n_craters=10, seed=seed + 17 * i))
+    return out
+
+
 def diamond_square(levels, hurst=0.8, seed=0):
     """(2^levels + 1)^2 height field, amplitude 2^-hurst per octave."""
     rng = np.random.default_rng(seed)

@@ -376,7 +376,9 @@ int flux_panel_plan(int64_t nblk, const int64_t *colptr, int64_t cap_quads,
  *   else 1; xnext may be NULL);  colmax[c] = max_i |out - prev| if colmax.
  * One Jacobi sweep of the correction equation per call.  items == NULL: one
  * workgroup per row block; with a plan, ws holds n_slots * ceil(nrhs / 64)
- * * 4096 floats. */
+ * * 4096 floats.  stage: 16-column chunks per pipeline stage (4 or 8; 0 =
+ * the library default, 4 unless FLUX_PANEL_STAGE=8), a tuning choice that
+ * leaves the result unchanged. */
 int flux_panel_mm_bf16(int64_t nrows, const int64_t *colptr, const int32_t *cols,
                        const void *panel, const void *X, int64_t nrhs,
                        const float *base, const float *rho, float rho_s,
@@ -384,7 +386,7 @@ int flux_panel_mm_bf16(int64_t nrows, const int64_t *colptr, const int32_t *cols
                        double *colmax, const int32_t *items, int64_t n_items,
                        const int32_t *nsplit, const int32_t *slot0,
                        const int32_t *split_blocks, int64_t n_split, float *ws,
-                       void *stream);
+                       int stage, void *stream);
 
 /* FP32 panels x FP64 X on v_mfma_f64_16x16x4_f64 (values widened):
  *   acc = P X; with E: out = E + s o acc - B and colmax[c] = max_i |out|
