@@ -620,6 +620,64 @@ def _build_near(H):
 _VALUE_DTYPES = {"fp64": "float64", "fp32": "float32", "bf16": "bfloat16", "fp16": "float16"}
 
 
+FAR_TILE = 128        # output rows per tile of the far-field block lists (hmat.hip)
+
+
+def tile_lists(lo, hi, n, tile=FAR_TILE):
+    """Per output tile of `tile` rows, the blocks whose output range
+    [lo_b, hi_b) meets it, in block order: (tptr int64 of ceil(n/tile)+1
+    entries, tblk int32), tile t's blocks being tblk[tptr[t]:tptr[t+1]]."""
+    lo = np.asarray(lo, dtype=np.int64)
+    hi = np.asarray(hi, dtype=np.int64)
+    if ((lo < 0) | (hi > n) | (hi <= lo)).any():
+        raise _lib.FluxError("far field: output range outside [0, n) or empty")
+    t_lo, t_hi = lo // tile, (hi - 1) // tile + 1
+    cnt = t_hi - t_lo
+    blk = np.repeat(np.arange(len(lo), dtype=np.int64), cnt)
+    first = np.repeat(np.cumsum(cnt) - cnt, cnt)
+    tl = np.repeat(t_lo, cnt) + (np.arange(int(cnt.sum()), dtype=np.int64) - first)
+    order = np.lexsort((blk, tl))
+    ntile = (n + tile - 1) // tile
+    tptr = np.zeros(ntile + 1, dtype=np.int64)
+    np.add.at(tptr, tl + 1, 1)
+    return np.cumsum(tptr), blk[order].astype(np.int32)
+
+
+FAR_STEPS = 8         # 16-rank steps per work item of the FP64 far expansion
+
+
+def far_plan(tptr, tblk, ranks, max_steps=FAR_STEPS):
+    """Work items of the FP64 far-field expansion: each tile's block list
+    tblk[tptr[t]:tptr[t+1]] cut greedily into runs of at most `max_steps`
+    16-rank steps (at least one block per run).  Returns (items int64 rows
+    (tile, q0, q1, slot), red int64 rows (tile, first slot, slot count),
+    nslots): a tile with one run adds into Y (slot -1); the runs of a tile
+    with several get consecutive slots, summed in slot order."""
+    ranks = np.asarray(ranks, dtype=np.int64)
+    steps = (ranks + 15) // 16
+    items, red = [], []
+    nslots = 0
+    for t in range(len(tptr) - 1):
+        q, q1 = int(tptr[t]), int(tptr[t + 1])
+        runs = []
+        while q < q1:
+            e, acc = q, 0
+            while e < q1 and (e == q or acc + steps[tblk[e]] <= max_steps):
+                acc += int(steps[tblk[e]])
+                e += 1
+            runs.append((q, e))
+            q = e
+        if len(runs) == 1:
+            items.append([t, runs[0][0], runs[0][1], -1])
+        elif runs:
+            red.append([t, nslots, len(runs)])
+            for a, b in runs:
+                items.append([t, a, b, nslots])
+                nslots += 1
+    return (np.array(items, dtype=np.int64).reshape(-1, 4),
+            np.array(red, dtype=np.int64).reshape(-1, 3), nslots)
+
+
 class HOperator:
     """Compressed F (or, with transpose=True, F^T) in cluster order:
     near-field CSR + far-field factors.  The adjoint uses the transposed
@@ -674,22 +732,23 @@ class HOperator:
         self.desc_host = np.array(descs, dtype=np.int64)
         self.desc = _i64(self.desc_host)
         self.ranks = np.array(ranks, dtype=np.int64)
-        # output rows -> far blocks covering them (row ranges, or column
-        # ranges for the adjoint), through a per-row cluster id
-        n = self.n
-        rowcl = np.arange(n, dtype=np.int32)
-        lists = [[] for _ in range(n)]
+        # output tiles of FAR_TILE rows -> far blocks whose output range (row
+        # range, or column range for the adjoint) meets them, in block order
         lo_f, hi_f = (2, 3) if self.transpose else (0, 1)
-        for b, d in enumerate(descs):
-            for i in range(d[lo_f], d[hi_f]):
-                lists[i].append(b)
-        cptr = np.zeros(n + 1, dtype=np.int64)
-        cptr[1:] = np.cumsum([len(x) for x in lists])
-        cblk = np.array([b for x in lists for b in x], dtype=np.int32)
-        self.rowcl = _lib.to_dev(rowcl)
-        self.cptr = _i64(cptr)
-        self.cblk = _lib.to_dev(cblk if cblk.size else np.zeros(1, np.int32))
+        tptr, tblk = tile_lists(self.desc_host[:, lo_f], self.desc_host[:, hi_f], self.n)
+        self.tptr = _i64(tptr)
+        self.tblk = _lib.to_dev(tblk if tblk.size else np.zeros(1, np.int32))
+        items, red, self.nslots = far_plan(tptr, tblk, self.ranks)
+        self.nitems, self.nred = len(items), len(red)
+        self.items = _i64(items if len(items) else np.zeros((1, 4), np.int64))
+        self.red = _i64(red if len(red) else np.zeros((1, 3), np.int64))
+        self.max_rank = int(self.ranks.max()) if self.nfar else 0
         self._tstart = {}
+
+    def _work(self, total, k, xdtype):
+        """Elements of the far-field work buffer for `total` projections."""
+        return int(_lib.load().flux_far_work_elems(int(total), int(k), int(self.nslots),
+                                                   _lib.dtype_code(xdtype)))
 
     def _tst(self, nrhs):
         if nrhs not in self._tstart:
@@ -725,11 +784,12 @@ class HOperator:
             tst, total = self._tst(k)
             acc = t.float64 if (X2.dtype == t.float64 or self.U.dtype == t.float64) \
                 else t.float32
-            T = t.empty(max(total, 1), dtype=acc, device=X2.device)
+            T = t.empty(self._work(total, k, X2.dtype), dtype=acc, device=X2.device)
             _lib.call("flux_far_matmat", self.n, k, self.nfar, self.desc.data_ptr(),
                       tst.data_ptr(), total, self.U.data_ptr(), self.S.data_ptr(),
-                      self.Vt.data_ptr(), _lib.dtype_code(self.U.dtype),
-                      self.rowcl.data_ptr(), self.cptr.data_ptr(), self.cblk.data_ptr(),
+                      self.Vt.data_ptr(), _lib.dtype_code(self.U.dtype), self.max_rank,
+                      self.tptr.data_ptr(), self.tblk.data_ptr(), self.nitems,
+                      self.items.data_ptr(), self.nred, self.red.data_ptr(), self.nslots,
                       X2.data_ptr(), Y.data_ptr(), _lib.dtype_code(X2.dtype), T.data_ptr(),
                       int(self.transpose), None, _lib.stream())
         return Y.reshape(-1) if vec else Y
@@ -743,11 +803,13 @@ class HOperator:
         k = X.shape[1]
         tst, total = self._tst(k)
         acc = t.float64 if (X.dtype == t.float64 or self.U.dtype == t.float64) else t.float32
-        T = t.empty(max(total, 1), dtype=acc, device=X.device)
+        T = t.empty(self._work(total, k, X.dtype), dtype=acc, device=X.device)
         _lib.call("flux_far_matmat", self.n, k, self.nfar, self.desc.data_ptr(),
                   tst.data_ptr(), total, self.U.data_ptr(), self.S.data_ptr(),
-                  self.Vt.data_ptr(), _lib.dtype_code(self.U.dtype), self.rowcl.data_ptr(),
-                  self.cptr.data_ptr(), self.cblk.data_ptr(), X.data_ptr(), Y.data_ptr(),
+                  self.Vt.data_ptr(), _lib.dtype_code(self.U.dtype), self.max_rank,
+                  self.tptr.data_ptr(), self.tblk.data_ptr(), self.nitems,
+                  self.items.data_ptr(), self.nred, self.red.data_ptr(), self.nslots,
+                  X.data_ptr(), Y.data_ptr(),
                   _lib.dtype_code(X.dtype), T.data_ptr(), int(self.transpose),
                   _lib.ptr(rscale), _lib.stream())
         return Y
@@ -781,9 +843,8 @@ class HOperator:
 class FarField16:
     """The far field of a compressed matrix packed for the MFMA products
     (flux_far_pack16 / flux_far_mm16): 16-bit U and diag(S) Vt with the
-    rank padded to a multiple of 16, and the expansion scheduled by
-    containment level of the row ranges (the ranges of one level are
-    disjoint, so each launch writes every output row once)."""
+    rank padded to a multiple of 16, and the expansion planned over 128-row
+    output tiles (``tile_lists`` / ``far_plan``)."""
 
     def __init__(self, master, values="bf16"):
         t = _lib.require_gpu()
@@ -814,40 +875,14 @@ class FarField16:
                   _lib.dtype_code(master.U.dtype), int(self.half), self.U16.data_ptr(),
                   self.V16.data_ptr(), _lib.stream())
         del ddesc
-        # expansion schedule: distinct row ranges by containment depth
-        ranges = sorted(set(zip(r0.tolist(), r1.tolist())), key=lambda a: (a[0], -a[1]))
-        depth, stack = {}, []
-        for a, b in ranges:
-            while stack and stack[-1] <= a:
-                stack.pop()
-            depth[(a, b)] = len(stack)
-            stack.append(b)
-        nlev = max(depth.values()) + 1
-        by_range = {}
-        for b in range(self.nfar):
-            by_range.setdefault((int(r0[b]), int(r1[b])), []).append(b)
-        groups, wgstart, blist = [], [], []
-        self.lvl_first, self.lvl_ngroups, self.lvl_nwg = [], [], []
-        for lv in range(nlev):
-            first, nwg = len(groups), 0
-            for rng_ in (g for g in ranges if depth[g] == lv):
-                members = by_range[rng_]
-                groups.append([rng_[0], rng_[1], len(blist), len(blist) + len(members)])
-                blist.extend(members)
-                wgstart.append(nwg)
-                nwg += (rng_[1] - rng_[0] + 63) // 64
-            self.lvl_first.append(first)
-            self.lvl_ngroups.append(len(groups) - first)
-            self.lvl_nwg.append(nwg)
-        for g in groups:
-            if g[1] <= g[0]:
-                raise _lib.FluxError("FarField16: empty row range")
-        self.nlev = nlev
-        self.groups = _i64(np.array(groups, dtype=np.int64))
-        self.wgstart = _i64(np.array(wgstart, dtype=np.int64))
-        self.blist = _lib.to_dev(np.array(blist, dtype=np.int32))
-        c64 = _lib.ctypes.c_int64 * nlev
-        self._lvl = (c64(*self.lvl_first), c64(*self.lvl_ngroups), c64(*self.lvl_nwg))
+        # expansion plan: 128-row output tiles, each tile's blocks cut into
+        # runs of at most FAR_STEPS 16-rank steps (the padded ranks)
+        tptr, tblk = tile_lists(r0, r1, self.n, 128)
+        items, red, self.nslots = far_plan(tptr, tblk, rp)
+        self.nitems, self.nred = len(items), len(red)
+        self.blist = _lib.to_dev(tblk if tblk.size else np.zeros(1, np.int32))
+        self.items = _i64(items if len(items) else np.zeros((1, 4), np.int64))
+        self.red = _i64(red if len(red) else np.zeros((1, 3), np.int64))
         self._per_k = {}
 
     def _buffers(self, k):
@@ -856,7 +891,9 @@ class FarField16:
             d = self.base.copy()
             d[:, 7] = k * self.rp_off
             T = t.empty(max(self.rp_total * k, 1), dtype=self.dtype, device=self.U16.device)
-            self._per_k[k] = (_i64(d), T)
+            P = t.empty(int(_lib.load().flux_far_work16_elems(self.nslots, k)),
+                        dtype=t.float32, device=self.U16.device)
+            self._per_k[k] = (_i64(d), T, P)
         return self._per_k[k]
 
     @property
@@ -871,12 +908,12 @@ class FarField16:
         n, k = X16.shape
         if n != self.n or tuple(Y.shape) != (n, k):
             raise ValueError("FarField16.mm: shape mismatch")
-        desc, T = self._buffers(k)
-        lf, lg, lw = self._lvl
+        desc, T, P = self._buffers(k)
         _lib.call("flux_far_mm16", self.nfar, desc.data_ptr(), self.U16.data_ptr(),
-                  self.V16.data_ptr(), int(self.half), X16.data_ptr(), k, T.data_ptr(),
-                  self.nlev, lf, lg, lw, self.groups.data_ptr(), self.wgstart.data_ptr(),
-                  self.blist.data_ptr(), _lib.ptr(rscale), Y.data_ptr(), _lib.stream())
+                  self.V16.data_ptr(), int(self.half), X16.data_ptr(), n, k, T.data_ptr(),
+                  self.blist.data_ptr(), self.nitems, self.items.data_ptr(), self.nred,
+                  self.red.data_ptr(), P.data_ptr(), _lib.ptr(rscale), Y.data_ptr(),
+                  _lib.stream())
         return Y
 
 

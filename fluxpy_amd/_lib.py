@@ -63,11 +63,14 @@ SIGNATURES = {
     "flux_scale_basis": (_c_int, [_c_i64, _vp, _vp, _vp, _vp, _vp, _c_dbl, _vp, _vp, _vp]),
     "flux_scale_cols_inv_sqrt": (_c_int, [_c_i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "flux_far_matmat": (_c_int, [_c_i64, _c_i64, _c_i64, _vp, _vp, _c_i64, _vp, _vp, _vp, _c_int,
-                                 _vp, _vp, _vp, _vp, _vp, _c_int, _vp, _c_int, _vp, _vp]),
+                                 _c_i64, _vp, _vp, _c_i64, _vp, _c_i64, _vp, _c_i64, _vp, _vp,
+                                 _c_int, _vp, _c_int, _vp, _vp]),
+    "flux_far_work_elems": (_c_i64, [_c_i64, _c_i64, _c_i64, _c_int]),
     "flux_far_pack16": (_c_int, [_c_i64, _vp, _vp, _vp, _vp, _vp, _c_int, _c_int, _vp, _vp,
                                  _vp]),
-    "flux_far_mm16": (_c_int, [_c_i64, _vp, _vp, _vp, _c_int, _vp, _c_i64, _vp, _c_i64, _vp,
-                               _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "flux_far_mm16": (_c_int, [_c_i64, _vp, _vp, _vp, _c_int, _vp, _c_i64, _c_i64, _vp, _vp,
+                               _c_i64, _vp, _c_i64, _vp, _vp, _vp, _vp, _vp]),
+    "flux_far_work16_elems": (_c_i64, [_c_i64, _c_i64]),
     "flux_permute_rows": (_c_int, [_c_i64, _c_i64, _vp, _c_int, _vp, _vp, _c_int, _vp]),
     "flux_jacobi_update": (_c_int, [_c_i64, _c_i64, _vp, _vp, _vp, _vp, _c_dbl, _c_int, _vp,
                                     _c_int, _vp, _vp]),

@@ -17,11 +17,11 @@
 //  - project: one workgroup per (block, 64 columns of X); X tiles of 16
 //    rows x 64 columns are staged in LDS and shared by the four waves, each
 //    wave owns every fourth 16-row tile of Vt'_b;
-//  - expand: one wave per 16 output rows; the far blocks of one row range
-//    (one cluster) are summed in a fixed order, and row ranges of one
-//    launch are disjoint (the host launches the containment levels of the
-//    cluster tree one after another), so every output element is written
-//    by one wave and the result does not depend on scheduling.
+//  - expand: per 128-row output tile, the far blocks whose row range meets
+//    it, cut by a host plan into short runs (no workgroup walks a long
+//    list); a tile with one run adds into Y, the runs of a longer list
+//    write partial sums that a second launch adds in run order, so every
+//    output element has a fixed summation order.
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
 
@@ -175,61 +175,96 @@ __global__ void __launch_bounds__(256) k_far_project16(
   }
 }
 
-// ---- expansion Y[rows] += s o (U_b T_b) over the blocks of one row range ----
-// groups: int64 rows of 4 (r0, r1, first, end) into blist; wgstart[g] =
-// first workgroup of group g (64 rows per workgroup).
+// ---- expansion Y[rows] += s o (U_b T_b), planned ---------------------------
+// Output rows go in tiles of kTile16; tile t's far blocks (row range
+// meeting the tile) are blist[tptr[t] : tptr[t+1]], and the work items
+// (tile, q0, q1, slot) cut each list into short runs.  An item whose run is
+// its tile's whole list adds into Y (slot < 0); the runs of a longer list
+// write FP32 partial sums to consecutive slots of P (kTile16 x k each),
+// which k_far_reduce16 adds in slot order, so every output element has a
+// fixed summation order.  Workgroup = (64 columns, item); wave w owns rows
+// 32 w .. 32 w + 31 of the tile (two 16-row MFMA tiles); rows of the tile
+// outside a block's range contribute zeros.
+constexpr int64_t kTile16 = 128;
+
 template <bool HALF>
 __global__ void __launch_bounds__(256) k_far_expand16(
-    int64_t ngroups, const int64_t *__restrict__ groups, const int64_t *__restrict__ wgstart,
-    const int32_t *__restrict__ blist, const int64_t *__restrict__ desc,
-    const uint16_t *__restrict__ U16, const uint16_t *__restrict__ T, int64_t k,
-    const float *__restrict__ rscale, float *__restrict__ Y) {
-  const int64_t w = blockIdx.x;
-  int64_t lo = 0, hi = ngroups;
-  while (hi - lo > 1) {
-    const int64_t mid = (lo + hi) >> 1;
-    if (wgstart[mid] <= w)
-      lo = mid;
-    else
-      hi = mid;
-  }
-  const int64_t *g = groups + lo * 4;
-  const int64_t r0 = g[0], r1 = g[1];
+    const int64_t *__restrict__ items, const int32_t *__restrict__ blist,
+    const int64_t *__restrict__ desc, const uint16_t *__restrict__ U16,
+    const uint16_t *__restrict__ T, int64_t n, int64_t k, const float *__restrict__ rscale,
+    float *__restrict__ P, float *__restrict__ Y) {
+  const int64_t ncg = (k + 63) >> 6;
+  const int64_t *it = items + 4 * (int64_t)(blockIdx.x / ncg);
+  const int64_t tile = it[0], slot = it[3];
+  const int64_t h0 = (int64_t)(blockIdx.x % ncg) * 64;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int li = lane & 15, lg = lane >> 4;
-  const int64_t i0 = r0 + (w - wgstart[lo]) * 64 + 16 * wave;
-  if (i0 >= r1) return;  // no barrier in this kernel
-  const int64_t h0 = (int64_t)blockIdx.y * 64;
-  v4f acc[4];
-  for (int c = 0; c < 4; ++c) acc[c] = v4f{0.f, 0.f, 0.f, 0.f};
-  const int64_t row = i0 + li;  // A row of this lane
-  for (int64_t q = g[2]; q < g[3]; ++q) {
+  const int64_t rw = tile * kTile16 + 32 * wave;
+  v4f acc[2][4];
+  for (int a = 0; a < 2; ++a)
+    for (int c = 0; c < 4; ++c) acc[a][c] = v4f{0.f, 0.f, 0.f, 0.f};
+  for (int64_t q = it[1]; q < it[2]; ++q) {
     const int64_t *d = desc + (int64_t)blist[q] * kFarDesc;
-    const int64_t rp = d[4];
-    const uint16_t *urow = U16 + d[5] + (row - r0) * rp;
+    const int64_t r0 = d[0], r1 = d[1], rp = d[4];
     const uint16_t *tb = T + d[7];
     for (int64_t jt = 0; jt < rp / 16; ++jt) {
-      uint2 A = make_uint2(0u, 0u);
-      if (row < r1) A = *reinterpret_cast<const uint2 *>(urow + 16 * jt + 4 * lg);
+      uint2 A[2];
+      for (int a = 0; a < 2; ++a) {
+        const int64_t row = rw + 16 * a + li;
+        A[a] = make_uint2(0u, 0u);
+        if (row >= r0 && row < r1)
+          A[a] = *reinterpret_cast<const uint2 *>(U16 + d[5] + (row - r0) * rp + 16 * jt +
+                                                  4 * lg);
+      }
       for (int c = 0; c < 4; ++c) {
-        const int64_t col = h0 + 16 * c + li;
         if (h0 + 16 * c >= k) break;
+        const int64_t col = h0 + 16 * c + li;
         uint2 B = make_uint2(0u, 0u);
         if (col < k) B = *reinterpret_cast<const uint2 *>(tb + col * rp + 16 * jt + 4 * lg);
-        acc[c] = mfma16<HALF>(A, B, acc[c]);
+        for (int a = 0; a < 2; ++a) acc[a][c] = mfma16<HALF>(A[a], B, acc[a][c]);
       }
     }
   }
-  for (int c = 0; c < 4; ++c) {
-    const int64_t col = h0 + 16 * c + li;
-    if (h0 + 16 * c >= k) break;
-    if (col >= k) continue;
-    for (int q = 0; q < 4; ++q) {
-      const int64_t i = i0 + 4 * lg + q;
-      if (i >= r1) continue;
-      const float s = rscale ? rscale[i] : 1.0f;
-      Y[i * k + col] += s * acc[c][q];
+  // acc[a][c][q] = D[4 lg + q][li] = Y[rw + 16 a + 4 lg + q][h0 + 16 c + li]
+  for (int a = 0; a < 2; ++a)
+    for (int c = 0; c < 4; ++c) {
+      const int64_t col = h0 + 16 * c + li;
+      if (h0 + 16 * c >= k) break;
+      if (col >= k) continue;
+      for (int q = 0; q < 4; ++q) {
+        const int64_t i = rw + 16 * a + 4 * lg + q;
+        if (i >= n) continue;
+        if (slot < 0) {
+          const float s = rscale ? rscale[i] : 1.0f;
+          Y[i * k + col] += s * acc[a][c][q];
+        } else {
+          P[(slot * kTile16 + (i - tile * kTile16)) * k + col] = acc[a][c][q];
+        }
+      }
     }
+}
+
+// Y[tile rows] += s o (sum of the tile's slots, in slot order); red rows of
+// 3 int64: tile, first slot, slot count.  Workgroup = 16 rows of a tile.
+__global__ void __launch_bounds__(256) k_far_reduce16(int64_t n, int64_t k,
+                                                      const int64_t *__restrict__ red,
+                                                      const float *__restrict__ P,
+                                                      const float *__restrict__ rscale,
+                                                      float *__restrict__ Y) {
+  constexpr int64_t kChunks = kTile16 / 16;
+  const int64_t *rd = red + 3 * (int64_t)(blockIdx.x / kChunks);
+  const int64_t tile = rd[0], s0 = rd[1], ns = rd[2];
+  const int64_t r0 = (int64_t)(blockIdx.x % kChunks) * 16;
+  const int64_t i0 = tile * kTile16 + r0;
+  const int64_t m = n - i0 < 16 ? n - i0 : 16;
+  if (m <= 0) return;
+  const float *src = P + (s0 * kTile16 + r0) * k;
+  for (int64_t e = threadIdx.x; e < m * k; e += blockDim.x) {
+    float acc = 0.f;
+    for (int64_t s = 0; s < ns; ++s) acc += src[s * kTile16 * k + e];
+    const int64_t i = i0 + e / k;
+    const float sc = rscale ? rscale[i] : 1.0f;
+    Y[i0 * k + e] += sc * acc;
   }
 }
 
@@ -263,20 +298,23 @@ extern "C" int flux_far_pack16(int64_t nfar, const int64_t *sdesc, const int64_t
   return 0;
 }
 
+extern "C" int64_t flux_far_work16_elems(int64_t nslots, int64_t k) {
+  if (nslots < 0 || k < 0) return -1;
+  return nslots * kTile16 * k > 0 ? nslots * kTile16 * k : 1;
+}
+
 extern "C" int flux_far_mm16(int64_t nfar, const int64_t *desc, const void *U16,
-                             const void *V16, int half, const void *X, int64_t k, void *T,
-                             int64_t nlevels, const int64_t *lvl_first,
-                             const int64_t *lvl_ngroups, const int64_t *lvl_nwg,
-                             const int64_t *groups, const int64_t *wgstart,
-                             const int32_t *blist, const float *rscale, float *Y,
-                             void *stream) {
-  FLUX_CHECK(nfar >= 0 && k >= 0 && nlevels >= 0, "flux_far_mm16: bad sizes");
-  if (nfar == 0 || k == 0) return 0;
+                             const void *V16, int half, const void *X, int64_t n, int64_t k,
+                             void *T, const int32_t *blist, int64_t nitems,
+                             const int64_t *items, int64_t nred, const int64_t *red,
+                             float *P, const float *rscale, float *Y, void *stream) {
+  FLUX_CHECK(nfar >= 0 && k >= 0 && n >= 0 && nitems >= 0 && nred >= 0,
+             "flux_far_mm16: bad sizes");
+  if (nfar == 0 || k == 0 || n == 0) return 0;
   FLUX_CHECK(nfar <= 0x7fffffff, "flux_far_mm16: too many far blocks");
-  FLUX_CHECK(desc && U16 && V16 && X && T && Y && groups && wgstart && blist,
-             "flux_far_mm16: NULL array");
-  FLUX_CHECK(nlevels == 0 || (lvl_first && lvl_ngroups && lvl_nwg),
-             "flux_far_mm16: NULL level table");
+  FLUX_CHECK(desc && U16 && V16 && X && T && Y && blist && nitems > 0 && items &&
+                 (nred == 0 || (red && P)),
+             "flux_far_mm16: NULL array or empty plan");
   hipStream_t st = as_stream(stream);
   const unsigned hg = ceil_div_u(k, 64);
   const bool vec = (k % 4) == 0;
@@ -293,19 +331,21 @@ extern "C" int flux_far_mm16(int64_t nfar, const int64_t *desc, const void *U16,
     PROJ(false, false);
 #undef PROJ
   FLUX_LAUNCH_CHECK();
-  for (int64_t lv = 0; lv < nlevels; ++lv) {
-    const int64_t ng = lvl_ngroups[lv], nwg = lvl_nwg[lv], g0 = lvl_first[lv];
-    if (ng == 0 || nwg == 0) continue;
-    FLUX_CHECK(ng > 0 && nwg > 0 && g0 >= 0 && nwg <= 0x7fffffff,
-               "flux_far_mm16: bad level table entry %lld", (long long)lv);
-    if (half)
-      hipLaunchKernelGGL((k_far_expand16<true>), dim3((unsigned)nwg, hg), dim3(256), 0, st, ng,
-                         groups + 4 * g0, wgstart + g0, blist, desc, (const uint16_t *)U16,
-                         (const uint16_t *)T, k, rscale, Y);
-    else
-      hipLaunchKernelGGL((k_far_expand16<false>), dim3((unsigned)nwg, hg), dim3(256), 0, st,
-                         ng, groups + 4 * g0, wgstart + g0, blist, desc, (const uint16_t *)U16,
-                         (const uint16_t *)T, k, rscale, Y);
+  const int64_t nwg = nitems * (int64_t)hg;
+  FLUX_CHECK(nwg <= 0x7fffffff && nred * (kTile16 / 16) <= 0x7fffffff,
+             "flux_far_mm16: plan too large");
+  if (half)
+    hipLaunchKernelGGL((k_far_expand16<true>), dim3((unsigned)nwg), dim3(256), 0, st, items,
+                       blist, desc, (const uint16_t *)U16, (const uint16_t *)T, n, k, rscale, P,
+                       Y);
+  else
+    hipLaunchKernelGGL((k_far_expand16<false>), dim3((unsigned)nwg), dim3(256), 0, st, items,
+                       blist, desc, (const uint16_t *)U16, (const uint16_t *)T, n, k, rscale, P,
+                       Y);
+  FLUX_LAUNCH_CHECK();
+  if (nred > 0) {
+    hipLaunchKernelGGL(k_far_reduce16, dim3((unsigned)(nred * (kTile16 / 16))), dim3(256), 0, st,
+                       n, k, red, (const float *)P, rscale, Y);
     FLUX_LAUNCH_CHECK();
   }
   return 0;

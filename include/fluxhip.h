@@ -269,18 +269,28 @@ int flux_scale_cols_inv_sqrt(int64_t nb, const int64_t *dims, double *C,
  * (the x @ F of compressed_form_factors.py:905-913, which the reference's
  * compressed blocks cannot serve), the lists then covering columns.
  * desc: nfar rows of 8 int64 (r0, r1, c0, c1, rank, offU, offVt, offS);
- * tstart[b] = sum of rank*nrhs before b, total = the full sum; T: work of
- * `total` elements in the accumulation type (FP64 if X or the factors are
- * FP64, else FP32).  The far blocks covering row i are
- * cblk[cptr[rowcl[i]] : cptr[rowcl[i]+1]].  Factor dtype F64/F32/BF16/F16. */
+ * tstart[b] = sum of rank*nrhs before b, total = the full sum; max_rank >=
+ * every rank.  Output rows go in tiles of 128: the far blocks whose output
+ * range meets tile t are tblk[tptr[t] : tptr[t+1]] (tptr: ceil(n/128)+1
+ * entries), in block order, which fixes the summation order.
+ * FP64 X (factors F64/F32, FP64 MFMA) also takes a work plan: nitems rows
+ * of 4 int64 (tile, q0, q1, slot) cutting each tile's list tblk[q0:q1]
+ * into short items; an item with slot < 0 adds into Y, the items of one
+ * tile with slots s0..s0+ns-1 (consecutive) write partial sums that are
+ * added in slot order, listed in red: nred rows of 3 int64 (tile, s0, ns);
+ * nslots = number of slots.  FP32 X (factors F32/BF16/F16) ignores the plan.
+ * T: work of flux_far_work_elems(total, nrhs, nslots, xdtype) elements of
+ * the accumulation type (FP64 for FP64 X, FP32 for FP32 X). */
 int flux_far_matmat(int64_t n, int64_t nrhs, int64_t nfar, const int64_t *desc,
                     const int64_t *tstart, int64_t total, const void *U,
-                    const void *S, const void *Vt, int fdtype,
-                    const int32_t *rowcl, const int64_t *cptr,
-                    const int32_t *cblk, const void *X, void *Y, int xdtype,
-                    void *T, int trans, const double *rscale, void *stream);
+                    const void *S, const void *Vt, int fdtype, int64_t max_rank,
+                    const int64_t *tptr, const int32_t *tblk, int64_t nitems,
+                    const int64_t *items, int64_t nred, const int64_t *red,
+                    int64_t nslots, const void *X, void *Y, int xdtype, void *T,
+                    int trans, const double *rscale, void *stream);
 /* (rscale: NULL, or n FP64 row factors applied to the far-field sum before
  * it is added to Y -- the albedo of the 'left' placement.) */
+int64_t flux_far_work_elems(int64_t total, int64_t nrhs, int64_t nslots, int xdtype);
 
 /* The far field on MFMA with 16-bit factors (the inner sweeps of the refined
  * solve; same product as flux_far_matmat, trans = 0).
@@ -290,21 +300,22 @@ int flux_far_matmat(int64_t n, int64_t nrhs, int64_t nfar, const int64_t *desc,
  *   rp = 16 ceil(rank / 16) <= 256, U16 m x rp and V16 rp x n row-major.
  * mm16: Y (n x k FP32) += rscale o (U16 (V16 X)) for X (n x k, the 16-bit
  *   type of the factors).  T: work of sum_b rp_b k 16-bit values (toff).
- *   The expansion runs one launch per level: level lv covers groups
- *   lvl_first[lv] .. + lvl_ngroups[lv] (host arrays) of `groups` (device,
- *   int64 rows r0, r1, first, end into blist), with lvl_nwg[lv] workgroups
- *   of 64 rows (wgstart: device, first workgroup of each group within its
- *   level).  The row ranges of one level must be disjoint. */
+ *   The expansion follows a plan over 128-row output tiles: blist holds
+ *   each tile's far blocks (row range meeting the tile) in block order, and
+ *   nitems rows of 4 int64 (tile, q0, q1, slot) cut them into runs
+ *   blist[q0:q1]; a run with slot < 0 adds into Y, the runs of one tile with
+ *   slots s0..s0+ns-1 write FP32 partial sums to P (flux_far_work16_elems
+ *   (nslots, k) floats) that are added in slot order, listed in red (nred
+ *   rows of 3 int64: tile, s0, ns). */
 int flux_far_pack16(int64_t nfar, const int64_t *sdesc, const int64_t *ddesc,
                     const void *U, const void *S, const void *Vt, int fdtype,
                     int half, void *U16, void *V16, void *stream);
 int flux_far_mm16(int64_t nfar, const int64_t *desc, const void *U16,
-                  const void *V16, int half, const void *X, int64_t k, void *T,
-                  int64_t nlevels, const int64_t *lvl_first,
-                  const int64_t *lvl_ngroups, const int64_t *lvl_nwg,
-                  const int64_t *groups, const int64_t *wgstart,
-                  const int32_t *blist, const float *rscale, float *Y,
-                  void *stream);
+                  const void *V16, int half, const void *X, int64_t n, int64_t k,
+                  void *T, const int32_t *blist, int64_t nitems,
+                  const int64_t *items, int64_t nred, const int64_t *red, float *P,
+                  const float *rscale, float *Y, void *stream);
+int64_t flux_far_work16_elems(int64_t nslots, int64_t k);
 
 /* Gather (scatter == 0: out[k] = X[perm[k]]) or scatter (out[perm[k]] =
  * X[k]) of the rows of an n x nrhs block: the cluster ordering. */

@@ -297,3 +297,85 @@ def test_refined_solve_with_far_field(torch_dev, oracle_mod, crater32, compresse
     # and within the compression error of the exact (dense) solution
     Bd = oracle_mod.solve_radiosity_dense(FF, E, rho, place)
     assert abs(B - Bd).max() <= 0.05 * abs(Bd).max()
+
+
+def _far_problem(rng, n, ranks, fdt):
+    """Far blocks on an n x n matrix with nested and overlapping ranges:
+    (desc rows r0, r1, c0, c1, rank, offU, offVt, offS; U, S, Vt flat)."""
+    desc, Us, Ss, Vts = [], [], [], []
+    oU = oV = oS = 0
+    for r in ranks:
+        r0 = int(rng.integers(0, n - 40))
+        r1 = int(min(n, r0 + rng.integers(20, 400)))
+        c0 = int(rng.integers(0, n - 40))
+        c1 = int(min(n, c0 + rng.integers(20, 400)))
+        m, nc = r1 - r0, c1 - c0
+        desc.append([r0, r1, c0, c1, r, oU, oV, oS])
+        Us.append(rng.standard_normal((m, r)).astype(fdt))
+        Ss.append(rng.uniform(0.1, 1.0, r).astype(fdt))
+        Vts.append(rng.standard_normal((r, nc)).astype(fdt))
+        oU += m * r
+        oV += r * nc
+        oS += r
+    return np.array(desc, dtype=np.int64), Us, Ss, Vts
+
+
+@pytest.mark.parametrize("fdt,xdt", [(np.float64, np.float64), (np.float32, np.float64),
+                                     (np.float32, np.float32)])
+@pytest.mark.parametrize("trans", [0, 1])
+@pytest.mark.parametrize("k", [1, 5, 64, 70])
+def test_far_matmat_against_dense(torch_dev, fdt, xdt, trans, k):
+    """flux_far_matmat (FP64 vectors on FP64 MFMA, FP32 vectors per element)
+    against the dense sum of U_b diag(S_b) Vt_b (or its transpose) in FP64:
+    ranks 1 to 300 (above 256 the projection runs in rank chunks), ranges
+    that nest and overlap, row scaling, Y accumulated into."""
+    t = torch_dev
+    from fluxpy_amd import _hbuild, _lib
+    rng = np.random.default_rng(11 + trans + 3 * k)
+    n = 700
+    ranks = [1, 3, 16, 17, 40, 64, 100, 300, 7, 33]
+    desc, Us, Ss, Vts = _far_problem(rng, n, ranks, fdt)
+    Fd = np.zeros((n, n))
+    for (r0, r1, c0, c1, *_), u, s, vt in zip(desc, Us, Ss, Vts):
+        Fd[r0:r1, c0:c1] += (u.astype(np.float64) * s.astype(np.float64)) @ vt.astype(np.float64)
+    if trans:
+        Fd = Fd.T
+    lo, hi = (desc[:, 2], desc[:, 3]) if trans else (desc[:, 0], desc[:, 1])
+    tptr, tblk = _hbuild.tile_lists(lo, hi, n)
+    items, red, nslots = _hbuild.far_plan(tptr, tblk, ranks, max_steps=5)
+    assert len(red) > 0           # some tiles are split into partial sums
+    tdt = t.float64 if fdt == np.float64 else t.float32
+    xt = t.float64 if xdt == np.float64 else t.float32
+    dev = "cuda"
+    U = t.tensor(np.concatenate([u.ravel() for u in Us]), dtype=tdt, device=dev)
+    S = t.tensor(np.concatenate(Ss), dtype=tdt, device=dev)
+    Vt = t.tensor(np.concatenate([v.ravel() for v in Vts]), dtype=tdt, device=dev)
+    X = rng.standard_normal((n, k))
+    Y0 = rng.standard_normal((n, k))
+    rs = rng.uniform(0.5, 1.5, n)
+    ts = np.concatenate([[0], np.cumsum(np.array(ranks) * k)])
+    d_desc = t.tensor(desc, device=dev)
+    d_ts = t.tensor(ts[:-1], device=dev)
+    d_tptr = t.tensor(tptr, device=dev)
+    d_tblk = t.tensor(tblk, device=dev)
+    d_items = t.tensor(items, device=dev)
+    d_red = t.tensor(red, device=dev)
+    d_rs = t.tensor(rs, device=dev)
+    for scaled in (False, True):
+        Xd = t.tensor(X, dtype=xt, device=dev)
+        Yd = t.tensor(Y0, dtype=xt, device=dev)
+        T = t.empty(int(_lib.load().flux_far_work_elems(int(ts[-1]), k, nslots,
+                                                         _lib.dtype_code(xt))),
+                    dtype=xt, device=dev)
+        _lib.call("flux_far_matmat", n, k, len(ranks), d_desc.data_ptr(), d_ts.data_ptr(),
+                  int(ts[-1]), U.data_ptr(), S.data_ptr(), Vt.data_ptr(),
+                  _lib.dtype_code(tdt), max(ranks), d_tptr.data_ptr(), d_tblk.data_ptr(),
+                  len(items), d_items.data_ptr(), len(red), d_red.data_ptr(), nslots,
+                  Xd.data_ptr(), Yd.data_ptr(), _lib.dtype_code(xt), T.data_ptr(), trans,
+                  d_rs.data_ptr() if scaled else None, _lib.stream())
+        got = Yd.cpu().numpy().astype(np.float64)
+        Xr = X.astype(np.float32).astype(np.float64) if xdt == np.float32 else X
+        want = Y0 + (rs[:, None] if scaled else 1.0) * (Fd @ Xr)
+        scale = np.abs(Fd).sum(1).max() * np.abs(X).max()
+        tol = 1e-13 if xdt == np.float64 else 3e-6
+        assert np.abs(got - want).max() <= tol * scale, (scaled, np.abs(got - want).max() / scale)

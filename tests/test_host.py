@@ -244,3 +244,60 @@ def test_current_svds_zero_pattern_is_documented():
             assert nv == 6 * nzc and 6 * nzr <= nu <= 6 * m
         else:
             assert nu == 6 * nzr and 6 * nzc <= nv <= 6 * n
+
+
+def test_far_tile_lists_match_brute_force():
+    """The far-field expansion visits, per output tile of rows, every block
+    whose output range meets the tile, in block order (the summation order
+    of each output element); ranges outside [0, n) are refused."""
+    from fluxpy_amd import _hbuild, _lib
+    rng = np.random.default_rng(5)
+    for n, tl in ((1, 16), (15, 16), (17, 16), (1000, 16), (127, 128), (129, 128),
+                  (3000, 128)):
+        nb = 40
+        lo = rng.integers(0, n, nb)
+        hi = np.minimum(n, lo + rng.integers(1, 300, nb))
+        tptr, tblk = _hbuild.tile_lists(lo, hi, n, tl)
+        assert len(tptr) == (n + tl - 1) // tl + 1 and tptr[-1] == len(tblk)
+        for t in range(len(tptr) - 1):
+            want = [b for b in range(nb) if lo[b] < tl * t + tl and hi[b] > tl * t]
+            assert tblk[tptr[t]:tptr[t + 1]].tolist() == want
+    with pytest.raises(_lib.FluxError):
+        _hbuild.tile_lists([0], [17], 16)
+    with pytest.raises(_lib.FluxError):
+        _hbuild.tile_lists([3], [3], 16)
+
+
+def test_far_plan_covers_every_list_once():
+    """The FP64 far-field work plan: per tile, the runs tile its block list
+    in order, none exceeds the step budget unless it is a single block, a
+    tile with one run adds into Y and the slots of split tiles are
+    consecutive and listed for the ordered reduction."""
+    from fluxpy_amd import _hbuild
+    rng = np.random.default_rng(8)
+    n = 3000
+    lo = rng.integers(0, n - 1, 200)
+    hi = np.minimum(n, lo + rng.integers(1, 700, 200))
+    ranks = rng.integers(1, 200, 200)
+    tptr, tblk = _hbuild.tile_lists(lo, hi, n)
+    for ms in (1, 4, 8, 1000):
+        items, red, nslots = _hbuild.far_plan(tptr, tblk, ranks, max_steps=ms)
+        steps = (ranks + 15) // 16
+        redmap = {int(r[0]): (int(r[1]), int(r[2])) for r in red}
+        slots = []
+        for t in range(len(tptr) - 1):
+            mine = items[items[:, 0] == t]
+            if tptr[t] == tptr[t + 1]:
+                assert len(mine) == 0
+                continue
+            assert mine[0, 1] == tptr[t] and mine[-1, 2] == tptr[t + 1]
+            assert (mine[1:, 1] == mine[:-1, 2]).all()
+            for _, q0, q1, _ in mine:
+                assert q1 > q0 and (q1 - q0 == 1 or steps[tblk[q0:q1]].sum() <= ms)
+            if len(mine) == 1:
+                assert mine[0, 3] == -1 and t not in redmap
+            else:
+                s0, ns = redmap[t]
+                assert ns == len(mine) and mine[:, 3].tolist() == list(range(s0, s0 + ns))
+                slots += mine[:, 3].tolist()
+        assert sorted(slots) == list(range(nslots))
