@@ -101,6 +101,11 @@ def main():
                     help="skip the adjoint gradient measurement")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched solve of 32 uneven meshes (one HIP graph)")
+    ap.add_argument("--c4", action="store_true",
+                    help="run only the precision configuration: the 641^2 fractal grid "
+                         "(819,200 facets), F assembled in row chunks, 8 suns, BF16 sweeps "
+                         "with FP64 residuals against an FP64 solve (several minutes)")
+    ap.add_argument("--c4-grid", type=int, default=641)
     ap.add_argument("--calib", action="store_true",
                     help="end with a 512 MiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -110,6 +115,11 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.c4:
+        torch.cuda.set_device(0)
+        res = c4_line(torch, args.c4_grid)
+        print(json.dumps(res), flush=True)
+        return
     from fluxpy_amd import _hbuild, _kernels, _lib, meshes
     from fluxpy_amd.compressed_form_factors import CompressedFormFactorMatrix, _ClusterTree
     from fluxpy_amd.shape import CgalTrimeshShapeModel
@@ -546,6 +556,81 @@ def batch_line(torch, args, count=32, suns=16, reps=10):
         f"err vs FP64 {max(errs):.1e}")
     del br
     torch.cuda.empty_cache()
+    return res
+
+
+def c4_line(torch, grid=641, suns=8, reps=3):
+    """The precision configuration: fractal height field on a grid x grid
+    vertex mesh (641^2: 819,200 facets; diamond-square on 1025^2, relief
+    0.1), rho = 0.12 +- 0.02, 8 suns at 1-15 degrees.  F is assembled in
+    row chunks in Morton order (StreamedFormFactorMatrix: one copy of the
+    sparse F plus a BF16 copy of its values) and the 8 suns are solved
+    (a) in FP64 (Jacobi, FP64 vectors, reference stopping rule) and (b) by
+    BF16 sweeps with FP64 residuals to max|r| <= 1e-7 max|E| per sun.
+    Reports build time and ray rate, nnz and bytes, both solve times, and
+    per sun the relative error of B and of T = ((1 - rho)(E + F B) /
+    (emiss sigma))^(1/4) (emiss 0.95) of (b) against (a)."""
+    from scipy.constants import Stefan_Boltzmann
+    from fluxpy_amd import _kernels, _lib, meshes
+    from fluxpy_amd.shape import CgalTrimeshShapeModel
+    from fluxpy_amd.solve import solve_radiosity, solve_radiosity_refined
+    from fluxpy_amd.streamed import StreamedFormFactorMatrix
+    t0 = time.perf_counter()
+    levels = max(4, int(np.ceil(np.log2(grid - 1))))
+    V, F = meshes.fractal_terrain(grid=grid, levels=levels, n_craters=30, seed=1004)
+    sm = CgalTrimeshShapeModel(V.astype(np.float32), F)
+    nf = sm.num_faces
+    log(f"[bench] c4: {nf} facets, mesh + scene {time.perf_counter() - t0:.1f} s")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    S = StreamedFormFactorMatrix(sm, sweep_values="bf16", log=log, count_rays=True)
+    t_build = time.perf_counter() - t0
+    st = S.stats
+    log(f"[bench] c4: F assembled in {t_build:.1f} s: nnz {st['nnz']}, rays {st['rays']}, "
+        f"{S.nbytes / 1e9:.1f} GB")
+    rng = np.random.default_rng(1004)
+    rho = (0.12 + rng.uniform(-0.02, 0.02, nf)).astype(np.float64)
+    D = meshes.sun_directions(suns, seed=1004)
+    E32 = sm.get_direct_irradiance_multi(1365.0, D, device=True)
+    E64 = _kernels.convert(E32, torch.float64)
+    rho_d = _lib.to_dev(rho)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    B64, it64 = solve_radiosity(S, E64, rho_d, 'right')
+    torch.cuda.synchronize()
+    t64 = time.perf_counter() - t0
+    log(f"[bench] c4: FP64 Jacobi {t64 * 1e3:.0f} ms ({it64} iterations)")
+    lo = S.sweep_operator()
+    B, info = solve_radiosity_refined(lo, S, E64, rho_d, rtol=1e-7, return_info=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        B, info = solve_radiosity_refined(lo, S, E64, rho_d, rtol=1e-7, return_info=True)
+    torch.cuda.synchronize()
+    tref = (time.perf_counter() - t0) / reps
+    errB = ((B - B64).abs().amax(0) / B64.abs().amax(0)).cpu().numpy()
+    emiss = 0.95
+
+    def temp(Bx):
+        Phi = E64 + S.from_internal(S.apply(S.to_internal(Bx)))
+        return ((1.0 - rho_d)[:, None] * Phi / (emiss * Stefan_Boltzmann)).clamp_min(0) ** 0.25
+
+    T64, Tr = temp(B64), temp(B)
+    errT = ((Tr - T64).abs().amax(0) / T64.abs().amax(0)).cpu().numpy()
+    res = {"metric": "precision configuration: 8-sun radiosity solves, BF16 sweeps + FP64 "
+                     "residual vs FP64", "facets": nf, "grid": grid, "suns": suns,
+           "build_s": round(t_build, 1), "rays": int(st["rays"]),
+           "mrays_per_s_build": round(st["rays"] / t_build / 1e6, 1), "nnz": int(st["nnz"]),
+           "nbytes": int(S.nbytes), "chunks": st["chunks"],
+           "fp64_ms": round(t64 * 1e3, 1), "fp64_iterations": int(it64),
+           "refined_ms": round(tref * 1e3, 1), "refine_rounds": info["rounds"],
+           "solves_per_s_refined": round(suns / tref, 2),
+           "solves_per_s_fp64": round(suns / t64, 2),
+           "max_rel_err_B_per_sun": [float(x) for x in errB],
+           "max_rel_err_T_per_sun": [float(x) for x in errT],
+           "data": "synthetic (seeded fractal terrain, suns and albedo)"}
+    log(f"[bench] c4: refined {tref * 1e3:.0f} ms ({info['rounds']} rounds) vs FP64 "
+        f"{t64 * 1e3:.0f} ms; max err B {errB.max():.1e}, T {errT.max():.1e}")
     return res
 
 

@@ -46,6 +46,8 @@ SIGNATURES = {
     "flux_scan_i64": (_c_int, [_vp, _c_i64, _vp, ctypes.POINTER(_c_i64), _vp]),
     "flux_csr_spmm": (_c_int, [_c_i64, _c_i64, _vp, _vp, _vp, _c_int, _vp, _c_i64, _vp,
                                _c_int, _vp]),
+    "flux_csr_rows_chunk": (_c_int, [_c_i64, _c_i64, _vp, _vp, _vp, _c_int, _vp, _c_i64, _vp,
+                                     _c_int, _vp, _vp, _c_dbl, _c_int, _vp, _vp]),
     "flux_csr_jacobi_step": (_c_int, [_c_i64, _vp, _vp, _vp, _c_int, _vp, _vp, _c_i64, _vp,
                                       _c_dbl, _c_int, _vp, _c_int, _vp, _vp]),
     "flux_csr_transpose": (_c_int, [_c_i64, _c_i64, _vp, _vp, _vp, _c_int, _vp, _vp, _vp,

@@ -11,6 +11,8 @@
 //   visibility  scene ray test of the backend mode      (:57)
 //   s           = pi * (s2 * s2), s2 = (d0^2 + d1^2) + d2^2, d = P_i - P_j
 //   F_ij        = (c * A_j) / s, 0 if s == 0            (:62-64)
+#include <algorithm>
+
 #include "ray.h"
 
 namespace flux {
@@ -44,18 +46,14 @@ __device__ __forceinline__ T ff_value(T c, const T *Pi, const T *Pj, T Aj) {
   return (c * Aj) / s;
 }
 
-// One wave per (row p, 64-column word w).  With CAST_RAYS == false the bit
-// is the prefilter alone (the set of pairs that get a ray).
-template <typename T, bool CAST_RAYS = true>
-__global__ void __launch_bounds__(256) k_ff_mask(flux_scene S, const T *__restrict__ P,
-                                                 const T *__restrict__ N,
-                                                 const int64_t *__restrict__ I, int64_t m,
-                                                 const int64_t *__restrict__ J, int64_t n,
-                                                 T eps, int mode, uint64_t *__restrict__ mask,
-                                                 int64_t W) {
-  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  if (wave >= m * W) return;
+// The bit word w of row p of the candidate / visibility mask.
+template <typename T, bool CAST_RAYS>
+__device__ __forceinline__ void ff_mask_word(const flux_scene &S, const T *__restrict__ P,
+                                             const T *__restrict__ N,
+                                             const int64_t *__restrict__ I,
+                                             const int64_t *__restrict__ J, int64_t n, T eps,
+                                             int mode, uint64_t *__restrict__ mask, int64_t W,
+                                             int64_t wave, int lane) {
   const int64_t p = wave / W, w = wave % W;
   const int64_t q = w * 64 + lane;
   const int64_t i64 = I[p];
@@ -81,6 +79,24 @@ __global__ void __launch_bounds__(256) k_ff_mask(flux_scene S, const T *__restri
   }
   const uint64_t b = __ballot(bit);
   if (lane == 0) mask[p * W + w] = b;
+}
+
+// One wave per (row p, 64-column word w), the waves of the grid striding
+// over all m W words.  With CAST_RAYS == false the bit is the prefilter
+// alone (the set of pairs that get a ray).
+template <typename T, bool CAST_RAYS = true>
+__global__ void __launch_bounds__(256) k_ff_mask(flux_scene S, const T *__restrict__ P,
+                                                 const T *__restrict__ N,
+                                                 const int64_t *__restrict__ I, int64_t m,
+                                                 const int64_t *__restrict__ J, int64_t n,
+                                                 T eps, int mode, uint64_t *__restrict__ mask,
+                                                 int64_t W) {
+  const int lane = threadIdx.x & 63;
+  const int64_t nwaves = (int64_t)gridDim.x * (blockDim.x >> 6);
+  for (int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; wave < m * W;
+       wave += nwaves) {
+    ff_mask_word<T, CAST_RAYS>(S, P, N, I, J, n, eps, mode, mask, W, wave, lane);
+  }
 }
 
 // Same result as k_ff_mask<T, true>, with stream compaction: one wave per
@@ -319,7 +335,7 @@ extern "C" int flux_ff_candidates(const flux_scene *scene, const void *P, const 
   hipStream_t st = as_stream(stream);
   const int64_t W = (n + 63) / 64;
   if (W > 0) {
-    const unsigned blocks = ceil_div_u(m * W, 4);
+    const unsigned blocks = (unsigned)std::min<int64_t>((m * W + 3) / 4, 1 << 20);
     if (dtype == FLUX_F64)
       hipLaunchKernelGGL((k_ff_mask<double, false>), dim3(blocks), dim3(256), 0, st, *scene,
                          (const double *)P, (const double *)N, I, m, J, n, (double)eps, 0,

@@ -53,7 +53,7 @@ __global__ void __launch_bounds__(256) k_csr_rows(
     int64_t m, const int64_t *__restrict__ indptr, const int32_t *__restrict__ indices,
     const TA *__restrict__ data, const TX *__restrict__ X, int64_t nrhs, int R,
     TX *__restrict__ Y, const TX *__restrict__ E, const TX *__restrict__ rho, double rho_s,
-    double *__restrict__ colmax) {
+    double *__restrict__ colmax, int64_t row0) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int G = 64 / R;
   const int g = lane / R, c = lane % R;
@@ -101,12 +101,12 @@ __global__ void __launch_bounds__(256) k_csr_rows(
     }
     for (int off = R; off < 64; off <<= 1) acc += __shfl_xor(acc, off, 64);
     if (g == 0 && active) {
-      const int64_t o = row * nrhs + col;
+      const int64_t o = (row0 + row) * nrhs + col;  // row0: first row of a row chunk
       if (MODE == SPMM_PLAIN) {
         Y[o] = (TX)acc;
       } else {
         TAcc y = acc;
-        if (MODE == SPMM_JACOBI_LEFT) y = (rho ? (TAcc)rho[row] : (TAcc)rho_s) * y;
+        if (MODE == SPMM_JACOBI_LEFT) y = (rho ? (TAcc)rho[row0 + row] : (TAcc)rho_s) * y;
         const TX b1 = (TX)((TAcc)E[o] + y);
         Y[o] = b1;
         const double diff = fabs((double)b1 - (double)X[o]);
@@ -132,7 +132,8 @@ __global__ void __launch_bounds__(256) k_csr_rows(
 template <typename TA, typename TX, int MODE>
 static int launch_rows(int64_t m, const int64_t *indptr, const int32_t *indices,
                        const void *data, const void *X, int64_t nrhs, void *Y, const void *E,
-                       const void *rho, double rho_s, double *colmax, hipStream_t st) {
+                       const void *rho, double rho_s, double *colmax, hipStream_t st,
+                       int64_t row0) {
   using TAcc = typename AccOf<decltype(typename WideOf<TA>::type() + TX())>::type;
   int R = 1;
   while (R < nrhs && R < 64) R <<= 1;
@@ -142,7 +143,7 @@ static int launch_rows(int64_t m, const int64_t *indptr, const int32_t *indices,
   if (blocks == 0) return 0;
   hipLaunchKernelGGL((k_csr_rows<TA, TX, TAcc, MODE>), dim3(blocks), dim3(256), 0, st, m,
                      indptr, indices, (const TA *)data, (const TX *)X, nrhs, R, (TX *)Y,
-                     (const TX *)E, (const TX *)rho, rho_s, colmax);
+                     (const TX *)E, (const TX *)rho, rho_s, colmax, row0);
   FLUX_LAUNCH_CHECK();
   return 0;
 }
@@ -151,25 +152,25 @@ template <int MODE>
 static int dispatch_rows(int64_t m, const int64_t *indptr, const int32_t *indices,
                          const void *data, int dtype, const void *X, int64_t nrhs, void *Y,
                          int xdtype, const void *E, const void *rho, double rho_s,
-                         double *colmax, hipStream_t st) {
+                         double *colmax, hipStream_t st, int64_t row0 = 0) {
   if (dtype == FLUX_F64 && xdtype == FLUX_F64)
     return launch_rows<double, double, MODE>(m, indptr, indices, data, X, nrhs, Y, E, rho,
-                                             rho_s, colmax, st);
+                                             rho_s, colmax, st, row0);
   if (dtype == FLUX_F32 && xdtype == FLUX_F64)
     return launch_rows<float, double, MODE>(m, indptr, indices, data, X, nrhs, Y, E, rho,
-                                            rho_s, colmax, st);
+                                            rho_s, colmax, st, row0);
   if (dtype == FLUX_F64 && xdtype == FLUX_F32)
     return launch_rows<double, float, MODE>(m, indptr, indices, data, X, nrhs, Y, E, rho,
-                                            rho_s, colmax, st);
+                                            rho_s, colmax, st, row0);
   if (dtype == FLUX_BF16 && xdtype == FLUX_F32)
     return launch_rows<__hip_bfloat16, float, MODE>(m, indptr, indices, data, X, nrhs, Y, E,
-                                                    rho, rho_s, colmax, st);
+                                                    rho, rho_s, colmax, st, row0);
   if (dtype == FLUX_F16 && xdtype == FLUX_F32)
     return launch_rows<__half, float, MODE>(m, indptr, indices, data, X, nrhs, Y, E, rho,
-                                            rho_s, colmax, st);
+                                            rho_s, colmax, st, row0);
   if (dtype == FLUX_F32 && xdtype == FLUX_F32)
     return launch_rows<float, float, MODE>(m, indptr, indices, data, X, nrhs, Y, E, rho,
-                                           rho_s, colmax, st);
+                                           rho_s, colmax, st, row0);
   set_error("csr product: unsupported dtypes (%d, %d)", dtype, xdtype);
   return -2;
 }
@@ -302,6 +303,28 @@ extern "C" int flux_csr_transpose(int64_t m, int64_t n, const int64_t *indptr,
   } while (0);
   (void)hipFree(work);
   return rc;
+}
+
+extern "C" int flux_csr_rows_chunk(int64_t m, int64_t row0, const int64_t *indptr,
+                                   const int32_t *indices, const void *data, int dtype,
+                                   const void *X, int64_t nrhs, void *Y, int xdtype,
+                                   const void *E, const void *rho, double rho_scalar, int mode,
+                                   double *colmax, void *stream) {
+  FLUX_CHECK(m >= 0 && row0 >= 0 && nrhs >= 0, "flux_csr_rows_chunk: negative sizes");
+  if (m == 0 || nrhs == 0) return 0;
+  FLUX_CHECK(indptr && X && Y, "flux_csr_rows_chunk: NULL array");
+  FLUX_CHECK(mode >= 0 && mode <= 2, "flux_csr_rows_chunk: mode must be 0, 1 or 2");
+  FLUX_CHECK(mode == 0 || (E && colmax && X != Y),
+             "flux_csr_rows_chunk: a Jacobi step needs E, colmax and Y != X");
+  hipStream_t st = as_stream(stream);
+  if (mode == 0)
+    return dispatch_rows<SPMM_PLAIN>(m, indptr, indices, data, dtype, X, nrhs, Y, xdtype,
+                                     nullptr, nullptr, 0.0, nullptr, st, row0);
+  if (mode == 2)
+    return dispatch_rows<SPMM_JACOBI_LEFT>(m, indptr, indices, data, dtype, X, nrhs, Y, xdtype,
+                                           E, rho, rho_scalar, colmax, st, row0);
+  return dispatch_rows<SPMM_JACOBI_RIGHT>(m, indptr, indices, data, dtype, X, nrhs, Y, xdtype,
+                                          E, rho, rho_scalar, colmax, st, row0);
 }
 
 extern "C" int flux_csr_jacobi_step(int64_t m, const int64_t *indptr, const int32_t *indices,

@@ -186,6 +186,20 @@ int flux_csr_transpose(int64_t m, int64_t n, const int64_t *indptr,
                        int64_t *t_indptr, int32_t *t_indices, void *t_data,
                        void *stream);
 
+/* Rows row0 .. row0+m-1 of a row-chunked CSR (indptr local to the chunk,
+ * column indices global): mode 0 writes Y[row0+r] = (A X)[r]; mode 1
+ * (albedo on the right) / 2 (left) is the Jacobi step of
+ * flux_csr_jacobi_step restricted to the chunk's rows, with X = B (all
+ * rows), E and Y = B1 indexed by global row, and colmax accumulated with
+ * max (the caller zeroes it once before the first chunk).  No reference
+ * counterpart: the chunks let an F too large for one allocation (819,200
+ * facets) be assembled and applied chunk by chunk. */
+int flux_csr_rows_chunk(int64_t m, int64_t row0, const int64_t *indptr,
+                        const int32_t *indices, const void *data, int dtype,
+                        const void *X, int64_t nrhs, void *Y, int xdtype,
+                        const void *E, const void *rho, double rho_scalar, int mode,
+                        double *colmax, void *stream);
+
 /* One Jacobi radiosity iteration on a square CSR F, fused with its update
  * and convergence measure (solve.py:25-45):
  *   right (left == 0): B1 = E + F (rho o B)      (solve.py:41)

@@ -301,3 +301,29 @@ def test_far_plan_covers_every_list_once():
                 assert ns == len(mine) and mine[:, 3].tolist() == list(range(s0, s0 + ns))
                 slots += mine[:, 3].tolist()
         assert sorted(slots) == list(range(nslots))
+
+
+def test_morton_order_is_z_order():
+    """morton_order sorts points by the bit interleaving of their quantised
+    coordinates (x in the even bits): checked against a bit-by-bit key on
+    random points, on a 4 x 4 grid (the Z pattern) and with ties."""
+    from fluxpy_amd.streamed import morton_order
+    rng = np.random.default_rng(3)
+    P = rng.uniform(-2, 5, (500, 2))
+    perm = morton_order(P)
+    assert sorted(perm.tolist()) == list(range(500))
+    lo, span = P.min(0), P.max(0) - P.min(0)
+    q = np.minimum(((P - lo) / span * (1 << 21)).astype(np.int64), (1 << 21) - 1)
+
+    def key(x, y):
+        k = 0
+        for b in range(21):
+            k |= ((int(x) >> b) & 1) << (2 * b) | ((int(y) >> b) & 1) << (2 * b + 1)
+        return k
+
+    keys = np.array([key(x, y) for x, y in q])
+    assert (np.diff(keys[perm]) >= 0).all()
+    g = np.array([[x, y] for y in range(4) for x in range(4)], dtype=float)
+    order = morton_order(g)
+    assert [tuple(g[i].astype(int)) for i in order[:4]] == [(0, 0), (1, 0), (0, 1), (1, 1)]
+    assert morton_order(np.zeros((5, 2))).tolist() == [0, 1, 2, 3, 4]
