@@ -566,7 +566,8 @@ def c4_line(torch, grid=641, suns=8, reps=3):
     row chunks in Morton order (StreamedFormFactorMatrix: one copy of the
     sparse F plus a BF16 copy of its values) and the 8 suns are solved
     (a) in FP64 (Jacobi, FP64 vectors, reference stopping rule) and (b) by
-    BF16 sweeps with FP64 residuals to max|r| <= 1e-7 max|E| per sun.
+    one BF16 sweep per round with FP64 residuals to max|r| <= 1e-7 max|E|
+    per sun.
     Reports build time and ray rate, nnz and bytes, both solve times, and
     per sun the relative error of B and of T = ((1 - rho)(E + F B) /
     (emiss sigma))^(1/4) (emiss 0.95) of (b) against (a)."""
@@ -600,12 +601,25 @@ def c4_line(torch, grid=641, suns=8, reps=3):
     torch.cuda.synchronize()
     t64 = time.perf_counter() - t0
     log(f"[bench] c4: FP64 Jacobi {t64 * 1e3:.0f} ms ({it64} iterations)")
+    # one FP64-vector product over the FP32 values: bytes of indices, values
+    # and row pointers (the gathers of the 8 right-hand sides come on top)
+    X = torch.ones((nf, suns), dtype=torch.float64, device=E64.device)
+    S.apply(X)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        S.apply(X)
+    torch.cuda.synchronize()
+    t_mv = (time.perf_counter() - t0) / 3
+    mv_bytes = st["nnz"] * 8 + (nf + st["chunks"]) * 8 + 2 * nf * suns * 8
     lo = S.sweep_operator()
-    B, info = solve_radiosity_refined(lo, S, E64, rho_d, rtol=1e-7, return_info=True)
+    B, info = solve_radiosity_refined(lo, S, E64, rho_d, rtol=1e-7, inner_iters=1,
+                                      return_info=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(reps):
-        B, info = solve_radiosity_refined(lo, S, E64, rho_d, rtol=1e-7, return_info=True)
+        B, info = solve_radiosity_refined(lo, S, E64, rho_d, rtol=1e-7, inner_iters=1,
+                                          return_info=True)
     torch.cuda.synchronize()
     tref = (time.perf_counter() - t0) / reps
     errB = ((B - B64).abs().amax(0) / B64.abs().amax(0)).cpu().numpy()
@@ -623,6 +637,7 @@ def c4_line(torch, grid=641, suns=8, reps=3):
            "mrays_per_s_build": round(st["rays"] / t_build / 1e6, 1), "nnz": int(st["nnz"]),
            "nbytes": int(S.nbytes), "chunks": st["chunks"],
            "fp64_ms": round(t64 * 1e3, 1), "fp64_iterations": int(it64),
+           "matvec_ms": round(t_mv * 1e3, 2), "matvec_GBps": round(mv_bytes / t_mv / 1e9, 1),
            "refined_ms": round(tref * 1e3, 1), "refine_rounds": info["rounds"],
            "solves_per_s_refined": round(suns / tref, 2),
            "solves_per_s_fp64": round(suns / t64, 2),
