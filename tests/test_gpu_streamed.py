@@ -96,3 +96,56 @@ def test_solves_match_plain_matrix(torch_dev, terrain, streamed):
         B, info = solve_radiosity_refined(lo, S, E, rho, rtol=1e-7, return_info=True)
         err = (np.abs(B - B64).max(0) / np.abs(B64).max(0)).max()
         assert err <= 1e-6, (err, info)
+
+
+def test_admissible_blocks_do_not_compress(torch_dev):
+    """The premise of assembling large meshes as sparse F rather than by
+    low-rank blocks: on the fractal terrain (10,368 facets) the blocks a
+    64-leaf quadtree makes admissible (min diam <= dist between bounding
+    boxes) hold fewer nonzeros than their rank-revealing factors at tol 1e-5
+    (singular values above 1e-5 s_0) would take, by more than 3x."""
+    from fluxpy_amd.form_factors import FormFactorMatrix
+    from fluxpy_amd.quadtree import get_quadrant_order
+    from fluxpy_amd.shape import CgalTrimeshShapeModel
+    V, F = meshes.fractal_terrain(grid=73)
+    sm = CgalTrimeshShapeModel(V.astype(np.float32), F)
+    A = FormFactorMatrix.from_shape_model(sm).tocsr().astype(np.float64)
+    P = sm.P.astype(np.float64)
+    nodes, perm = [], []
+
+    def build(idx):
+        me = len(nodes)
+        nodes.append(None)
+        parts = [idx[p] for p in get_quadrant_order(P[idx, :2])] if idx.size > 64 else []
+        if not parts or max(p.size for p in parts) == idx.size:
+            s = len(perm)
+            perm.extend(idx.tolist())
+            nodes[me] = (s, s + idx.size, P[idx].min(0), P[idx].max(0), None)
+            return me
+        ch = [build(p) for p in parts if p.size]
+        nodes[me] = (nodes[ch[0]][0], nodes[ch[-1]][1], P[idx].min(0), P[idx].max(0), ch)
+        return me
+
+    build(np.arange(sm.num_faces))
+    Ap = A[perm][:, perm].tocsr()
+    diam = lambda c: np.linalg.norm(c[3] - c[2])
+    dist = lambda a, b: np.linalg.norm(np.maximum(0, np.maximum(a[2] - b[3], b[2] - a[3])))
+    nnz = entries = 0
+    stack = [(0, 0)]
+    while stack:
+        a, b = stack.pop()
+        ca, cb = nodes[a], nodes[b]
+        if min(diam(ca), diam(cb)) <= dist(ca, cb):
+            blk = Ap[ca[0]:ca[1], cb[0]:cb[1]]
+            if blk.nnz:
+                s = np.linalg.svd(blk.toarray(), compute_uv=False)
+                nnz += blk.nnz
+                entries += int((s > 1e-5 * s[0]).sum()) * sum(blk.shape)
+            continue
+        if ca[4] is None and cb[4] is None:
+            continue
+        kids_a = [a] if ca[4] is None else ca[4]
+        kids_b = [b] if cb[4] is None else cb[4]
+        stack.extend((x, y) for x in kids_a for y in kids_b)
+    assert nnz > 0.5 * A.nnz
+    assert entries > 3 * nnz, (entries, nnz)
