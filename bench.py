@@ -106,6 +106,8 @@ def main():
                          "(819,200 facets), F assembled in row chunks, 8 suns, BF16 sweeps "
                          "with FP64 residuals against an FP64 solve (several minutes)")
     ap.add_argument("--c4-grid", type=int, default=641)
+    ap.add_argument("--vis-only", action="store_true",
+                    help="run only the visibility measurement (for counter runs)")
     ap.add_argument("--calib", action="store_true",
                     help="end with a 512 MiB device copy (counter calibration)")
     args = ap.parse_args()
@@ -139,6 +141,9 @@ def main():
     log(f"[bench] facets={nf} suns={args.suns}")
 
     vis = visibility(torch, _lib, sm, _ClusterTree, tm)
+    if args.vis_only:
+        print(json.dumps({"visibility": vis, "facets": nf}), flush=True)
+        return
 
     # ---------------- compressed matrix (exact F: low-rank leaves off) ----------------
     torch.cuda.synchronize()
