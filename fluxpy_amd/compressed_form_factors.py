@@ -566,8 +566,11 @@ class CompressedFormFactorMatrix(scipy.sparse.linalg.LinearOperator):
 
     @staticmethod
     def from_file(path):
-        with open(path, 'rb') as f:
-            return pickle.load(f)
+        """Load a matrix saved by ``save``, or a fluxpy ``FF.bin``
+        (compressed_form_factors.py:748-751), through the whitelisting loader
+        of ``fluxpy_amd.persist``."""
+        from . import persist
+        return persist.load(path)
 
     def save(self, path):
         with open(path, 'wb') as f:

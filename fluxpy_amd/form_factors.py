@@ -63,8 +63,10 @@ class FormFactorMatrix(scipy.sparse.linalg.LinearOperator):
 
     @staticmethod
     def from_file(path):
-        with open(path, 'rb') as f:
-            return pickle.load(f)
+        """Load a matrix saved by ``save`` (whitelisting loader of
+        ``fluxpy_amd.persist``)."""
+        from . import persist
+        return persist.load(path)
 
     def save(self, path):
         with open(path, 'wb') as f:
