@@ -106,6 +106,11 @@ SIGNATURES = {
     "flux_vec_rowscale": (_c_int, [_c_i64, _c_i64, _vp, _c_dbl, _c_int, _c_dbl, _vp, _vp,
                                    _c_int, _vp]),
     "flux_vec_clamp0": (_c_int, [_c_i64, _vp, _vp, _c_int, _vp]),
+    "flux_gs_work_elems": (_c_i64, [_c_i64, _c_i64, _c_int]),
+    "flux_gs_project": (_c_int, [_c_i64, _c_i64, _c_int, _vp, _c_i64, _vp, _vp, _c_i64, _c_int,
+                                 _vp, _vp]),
+    "flux_gs_combine": (_c_int, [_c_i64, _c_i64, _c_int, _vp, _c_i64, _vp, _c_i64, _vp, _vp]),
+    "flux_gs_normalize": (_c_int, [_c_i64, _c_i64, _vp, _vp, _vp, _vp, _vp]),
     "flux_vec_convert": (_c_int, [_c_i64, _vp, _c_int, _vp, _c_int, _vp]),
     "flux_temperature": (_c_int, [_c_i64, _c_i64, _vp, _vp, _vp, _c_dbl, _vp, _c_dbl, _c_dbl,
                                   _c_int, _c_dbl, _vp, _c_int, _vp]),

@@ -219,75 +219,108 @@ def _cg(op, E, rho, left, tol, maxiter):
             return X, niter
 
 
+GMRES_CHECK = 4     # Arnoldi steps between the host's convergence checks
+
+
 def _gmres(op, E, rho, left, rtol, maxiter, restart, x0):
-    """Restarted GMRES(m), modified Gram-Schmidt, one Krylov space per column;
-    the small Hessenberg least-squares problems are solved with Givens
-    rotations on the host."""
+    """Restarted GMRES(m), one Krylov space per column, resident on the GPU:
+    the basis V (m + 1 blocks of n x k), the Hessenberg matrix H and every
+    Arnoldi step (the product, classical Gram-Schmidt with one
+    re-orthogonalisation, the normalisation) stay on the device
+    (``flux_gs_*``).  The host reads the new columns of H every GMRES_CHECK
+    steps, applies the Givens rotations and tests |g_{j+1}| <= rtol |E_c|
+    per column, and reads |r| once per restart: a few synchronisations per
+    cycle instead of one per inner product.  Up to GMRES_CHECK - 1 steps
+    may run past convergence within a check window.  FP64 throughout; the
+    result has E's dtype."""
     t = _lib.torch()
     n, k = E.shape
     dev = E.device
+    out_dtype = E.dtype
+    E = E.to(t.float64).contiguous()
+    f64 = t.float64
     vo = _kernels.VecOps(k)
-    X = t.zeros_like(E) if x0 is None else _lib.to_dev(x0).reshape(n, k).to(E.dtype).clone()
-    bnorm = np.sqrt(vo.dot(E, E).cpu().numpy())
+    X = (t.zeros_like(E) if x0 is None
+         else _lib.to_dev(x0).reshape(n, k).to(f64).contiguous().clone())
+    bnorm = np.sqrt(np.maximum(vo.dot(E, E).cpu().numpy(), 0.0))
     bnorm[bnorm == 0] = 1.0
-    niter = 0
     m = max(1, int(restart))
+    V = t.empty((m + 1, n, k), dtype=f64, device=dev)
+    H = t.zeros((m + 1, m, k), dtype=f64, device=dev)
+    work = t.empty(int(_lib.load().flux_gs_work_elems(n, k, min(m, 32))), dtype=f64, device=dev)
+    st = _lib.stream()
+    nk = n * k
+
+    def vptr(i):
+        return V.data_ptr() + i * nk * 8
+
+    def hptr(i, j):
+        return H.data_ptr() + (i * m + j) * k * 8
+
+    niter = 0
     while True:
         R = E.clone()
         if x0 is not None or niter > 0:
             KX = _apply_K(op, X, rho, left)
-            minus = t.full((k,), -1.0, dtype=t.float64, device=dev)
+            minus = t.full((k,), -1.0, dtype=f64, device=dev)
             _kernels.axpby(minus, KX, None, R)
-        beta = np.sqrt(np.maximum(vo.dot(R, R).cpu().numpy(), 0.0))
+        nrm2 = vo.dot(R, R)
+        beta = np.sqrt(np.maximum(nrm2.cpu().numpy(), 0.0))
         if np.all(beta <= rtol * bnorm) or niter >= maxiter:
             if niter >= maxiter and not np.all(beta <= rtol * bnorm):
                 warnings.warn(f"GMRES did not converge in {maxiter} iterations", RuntimeWarning)
-            return X, niter
-        inv = np.where(beta > 0, 1.0 / np.where(beta > 0, beta, 1.0), 0.0)
-        Vs = [_scale_cols(R, inv)]
-        H = np.zeros((m + 1, m, k))
+            return X.to(out_dtype), niter
+        _lib.call("flux_gs_normalize", n, k, R.data_ptr(), nrm2.data_ptr(), None, vptr(0), st)
+        Hh = np.zeros((m + 1, m, k))
         cs = np.zeros((m, k))
         sn = np.zeros((m, k))
         g = np.zeros((m + 1, k))
         g[0] = beta
         done = beta <= rtol * bnorm
-        j_used = 0
-        for j in range(m):
-            w = _apply_K(op, Vs[j], rho, left)
-            for i in range(j + 1):
-                h = vo.dot(w, Vs[i]).cpu().numpy().copy()
-                H[i, j] = h
-                _kernels.axpby(t.tensor(-h, dtype=t.float64, device=dev), Vs[i], None, w)
-            hn = np.sqrt(np.maximum(vo.dot(w, w).cpu().numpy(), 0.0))
-            H[j + 1, j] = hn
-            Vs.append(_scale_cols(w, np.where(hn > 0, 1.0 / np.where(hn > 0, hn, 1.0), 0.0)))
-            # Givens rotations per column
-            for i in range(j):
-                a, b = H[i, j].copy(), H[i + 1, j].copy()
-                H[i, j] = cs[i] * a + sn[i] * b
-                H[i + 1, j] = -sn[i] * a + cs[i] * b
-            a, b = H[j, j].copy(), H[j + 1, j].copy()
-            r = np.hypot(a, b)
-            safe = r > 0
-            cs[j] = np.where(safe, a / np.where(safe, r, 1.0), 1.0)
-            sn[j] = np.where(safe, b / np.where(safe, r, 1.0), 0.0)
-            H[j, j] = np.where(safe, r, a)
-            H[j + 1, j] = 0.0
-            g[j + 1] = -sn[j] * g[j]
-            g[j] = cs[j] * g[j]
+        j = j0 = 0
+        while j < m:
+            w = V[j + 1]
+            _apply_K(op, V[j], rho, left, out=w)
+            for accumulate in (0, 1):
+                for s0 in range(0, j + 1, 32):
+                    nv = min(32, j + 1 - s0)
+                    _lib.call("flux_gs_project", n, k, nv, vptr(s0), nk, w.data_ptr(),
+                              hptr(s0, j), m * k, accumulate, work.data_ptr(), st)
+            nrm2 = vo.dot(w, w)
+            _lib.call("flux_gs_normalize", n, k, w.data_ptr(), nrm2.data_ptr(),
+                      hptr(j + 1, j), w.data_ptr(), st)
+            j += 1
             niter += 1
-            j_used = j + 1
-            done = done | (np.abs(g[j + 1]) <= rtol * bnorm)
+            if j % GMRES_CHECK and j < m and niter < maxiter:
+                continue
+            # new Hessenberg columns j0 .. j-1 to the host; Givens rotations
+            Hh[:, j0:j] = H[:, j0:j].cpu().numpy()
+            for jj in range(j0, j):
+                for i in range(jj):
+                    a, b_ = Hh[i, jj].copy(), Hh[i + 1, jj].copy()
+                    Hh[i, jj] = cs[i] * a + sn[i] * b_
+                    Hh[i + 1, jj] = -sn[i] * a + cs[i] * b_
+                a, b_ = Hh[jj, jj].copy(), Hh[jj + 1, jj].copy()
+                r = np.hypot(a, b_)
+                safe = r > 0
+                cs[jj] = np.where(safe, a / np.where(safe, r, 1.0), 1.0)
+                sn[jj] = np.where(safe, b_ / np.where(safe, r, 1.0), 0.0)
+                Hh[jj, jj] = np.where(safe, r, a)
+                Hh[jj + 1, jj] = 0.0
+                g[jj + 1] = -sn[jj] * g[jj]
+                g[jj] = cs[jj] * g[jj]
+                done = done | (np.abs(g[jj + 1]) <= rtol * bnorm)
+            j0 = j
             if np.all(done) or niter >= maxiter:
                 break
-        # back substitution per column and update X
-        y = np.zeros((j_used, k))
-        for i in range(j_used - 1, -1, -1):
-            s = g[i] - (H[i, i + 1:j_used] * y[i + 1:j_used]).sum(0)
-            d = H[i, i]
-            y[i] = np.where(d != 0, s / np.where(d != 0, d, 1.0), 0.0)
-        for i in range(j_used):
-            _kernels.axpby(t.tensor(y[i], dtype=t.float64, device=dev), Vs[i], None, X)
+        # back substitution per column, then X += V y on the device
+        y = np.zeros((j, k))
+        for i in range(j - 1, -1, -1):
+            s_ = g[i] - (Hh[i, i + 1:j] * y[i + 1:j]).sum(0)
+            d = Hh[i, i]
+            y[i] = np.where(d != 0, s_ / np.where(d != 0, d, 1.0), 0.0)
+        yd = _lib.to_dev(np.ascontiguousarray(y))
+        _lib.call("flux_gs_combine", n, k, j, vptr(0), nk, yd.data_ptr(), k, X.data_ptr(), st)
 
 
 def _panel_ready(FF):
@@ -440,12 +473,4 @@ def solve_radiosity_refined(FF_lo, FF_hi, E, rho=1, albedo_placement='right', rt
     out = out.cpu().numpy() if host else out
     if return_info:
         return out, {"rounds": rounds, "inner_iters": inner_total, "residual_history": hist}
-    return out
-
-
-def _scale_cols(X, s):
-    """Return X[:, c] * s[c] (s: host FP64 per column) as a new tensor."""
-    t = _lib.torch()
-    out = t.zeros_like(X)
-    _kernels.axpby(t.tensor(np.asarray(s, dtype=np.float64), device=X.device), X, None, out)
     return out

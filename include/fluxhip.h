@@ -543,6 +543,27 @@ int flux_vec_rowscale(int64_t n, int64_t nrhs, const void *rho, double rho_s,
 int flux_vec_clamp0(int64_t total, const void *x, void *out, int dtype,
                     void *stream);
 
+/* Arnoldi steps of the GPU-resident GMRES (the 'gmres' method of
+ * solve.py:4-109, scipy.sparse.linalg.gmres there): one Krylov space per
+ * column of an n x k FP64 block.  The basis V holds vectors of n x k at
+ * V + i * vstride.
+ * project: one classical Gram-Schmidt pass of w against V_0 .. V_{nv-1}
+ *   (nv <= 32): p[i, c] = V_i[:, c] . w[:, c] is stored into (accumulate
+ *   == 0) or added to (accumulate != 0) h[i * hstride + c], then
+ *   w -= sum_i p_i V_i.  work: flux_gs_work_elems(n, k, nv) doubles.  The
+ *   dot products reduce in a fixed order.
+ * combine: x += sum_i y[i * ystride + c] V_i (the GMRES update).
+ * normalize: hn = sqrt(max(nrm2, 0)) per column, v = w / hn (0 where
+ *   hn == 0), hout[c] = hn (hout may be NULL). */
+int64_t flux_gs_work_elems(int64_t n, int64_t k, int nv);
+int flux_gs_project(int64_t n, int64_t k, int nv, const double *V, int64_t vstride,
+                    double *w, double *h, int64_t hstride, int accumulate,
+                    double *work, void *stream);
+int flux_gs_combine(int64_t n, int64_t k, int nv, const double *V, int64_t vstride,
+                    const double *y, int64_t ystride, double *x, void *stream);
+int flux_gs_normalize(int64_t n, int64_t k, const double *w, const double *nrm2,
+                      double *hout, double *v, void *stream);
+
 /* Element-wise dtype conversion between FP32 and FP64 blocks. */
 int flux_vec_convert(int64_t total, const void *x, int xdtype, void *y,
                      int ydtype, void *stream);

@@ -361,6 +361,33 @@ def test_cg_gmres_against_dense(torch_dev, oracle_mod, crater):
         assert it < 60
 
 
+@pytest.mark.parametrize("restart", [3, 7, 40])
+def test_gmres_restarts_against_dense(torch_dev, oracle_mod, crater, restart):
+    """The device-resident GMRES with restarts shorter than convergence
+    (several cycles), longer than one projection call (40 > 32 vectors),
+    three columns converging at different rates, a starting guess and FP32
+    input, against the oracle's dense LU."""
+    from fluxpy_amd.form_factors import FormFactorMatrix
+    from fluxpy_amd.solve import solve_radiosity
+    V, F = crater
+    sm = _model(V, F)
+    Fm = FormFactorMatrix.from_shape_model(sm)
+    sp = Fm.tocsr()
+    E = sm.get_direct_irradiance_multi(1365.0, meshes.sun_directions(3, seed=8))
+    E[:, 2] *= 1e-3
+    Bd = oracle_mod.solve_radiosity_dense(sp, E, 0.6, "right")
+    B, it = solve_radiosity(Fm, E, 0.6, "right", method="gmres", tol=1e-12, restart=restart)
+    assert abs(B - Bd).max(0).max() <= 1e-10 * abs(Bd).max()
+    assert (abs(B - Bd).max(0) <= 1e-10 * abs(Bd).max(0)).all()
+    x0 = Bd * (1 + 1e-3)
+    B2, it2 = solve_radiosity(Fm, E, 0.6, "right", method="gmres", tol=1e-12, restart=restart,
+                              x0=x0)
+    assert abs(B2 - Bd).max() <= 1e-10 * abs(Bd).max() and it2 <= it
+    B3, _ = solve_radiosity(Fm, E.astype(np.float32), 0.6, "right", method="gmres", tol=1e-6,
+                            restart=restart)
+    assert B3.dtype == np.float32 and abs(B3 - Bd).max() <= 1e-5 * abs(Bd).max()
+
+
 def test_steady_state_temperature(torch_dev, oracle_mod, crater):
     from fluxpy_amd.form_factors import FormFactorMatrix
     from fluxpy_amd.model import compute_steady_state_temp
