@@ -69,9 +69,7 @@ __device__ __forceinline__ void ff_mask_word(const flux_scene &S, const T *__res
     T c = ff_prefilter(Pi, Ni, Pj, Nj);
     if (i != j && c > eps) {
       if (CAST_RAYS) {
-        double pi64[3] = {(double)Pi[0], (double)Pi[1], (double)Pi[2]};
-        double pj64[3] = {(double)Pj[0], (double)Pj[1], (double)Pj[2]};
-        bit = face_to_face_vis(S, i, j, mode, pi64, pj64);
+        bit = face_to_face_vis(S, i, j, mode, Pi, Pj);
       } else {
         bit = true;
       }
@@ -128,7 +126,6 @@ __global__ void __launch_bounds__(256) k_ff_mask_rows(flux_scene S, const T *__r
   const int i = row_ok ? (int)i64 : 0;
   T Pi[3] = {P[i * 3], P[i * 3 + 1], P[i * 3 + 2]};
   T Ni[3] = {N[i * 3], N[i * 3 + 1], N[i * 3 + 2]};
-  const double pi64[3] = {(double)Pi[0], (double)Pi[1], (double)Pi[2]};
   const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   int count = 0;
 
@@ -136,9 +133,10 @@ __global__ void __launch_bounds__(256) k_ff_mask_rows(flux_scene S, const T *__r
     if (lane < active) {
       const int32_t q = queue[wv][lane];
       const int j = (int)J[q];
-      double pj64[3] = {0, 0, 0};
-      if (mode == FLUX_MODE_EMBREE) load3(P, j, pj64);
-      if (face_to_face_vis(S, i, j, mode, pi64, pj64))
+      T Pj[3] = {0, 0, 0};
+      if (mode == FLUX_MODE_EMBREE)
+        for (int c = 0; c < 3; ++c) Pj[c] = P[(int64_t)j * 3 + c];
+      if (face_to_face_vis(S, i, j, mode, Pi, Pj))
         atomicOr(&bits[wv][(q >> 6) & 63], 1ull << (q & 63));
     }
     __builtin_amdgcn_wave_barrier();

@@ -184,19 +184,54 @@ __device__ inline int closest_hit(const flux_scene &S, const Ray64 &r, const Ray
   return best_k;
 }
 
-// 1e3 * float32 resolution: the fixed ray offset of the Embree backend
-// (shape.py:351, 405).
-constexpr double kEmbreeEps = 1e-3;
+// eps of the Embree backend (shape.py:351, 405): 1e3 times
+// np.finfo(np.float32).resolution, the float32 1e-6 times a Python float,
+// an FP64 number (NumPy 1.x).
+constexpr double kEmbreeEps = 1e3 * (double)1e-6f;
+
+// Embree's ray for the pair (i, j) as fluxpy forms it (shape.py:355-385):
+// d = P_j - P_i, |d| = sqrt((d0^2 + d1^2) + d2^2), the unit direction and
+// the origin P_i + eps d/|d| in the model dtype T (NumPy applies the FP64
+// eps to a T array in T), then stored into Embree's FP32 ray arrays.
+// Returns false when |d| <= eps (the pair is visible by default).
+template <typename T>
+__device__ inline bool embree_pair_ray(const T *Pi, const T *Pj, double o[3], double d[3]) {
+  const T e = (T)kEmbreeEps;
+  const T dd[3] = {Pj[0] - Pi[0], Pj[1] - Pi[1], Pj[2] - Pi[2]};
+  const T nrm = sqrt((dd[0] * dd[0] + dd[1] * dd[1]) + dd[2] * dd[2]);
+  if (!(nrm > e)) return false;
+  for (int c = 0; c < 3; ++c) {
+    const T dh = dd[c] / nrm;
+    const T oc = Pi[c] + e * dh;
+    o[c] = (double)(float)oc;
+    d[c] = (double)(float)dh;
+  }
+  return true;
+}
+
+// Embree's occlusion ray (shape.py:410-411): origin P + eps N in the model
+// dtype, direction D (FP64), both stored as FP32.
+template <typename T>
+__device__ inline void embree_sun_ray(const T *Pi, const T *Ni, const double *D, double o[3],
+                                      double d[3]) {
+  const T e = (T)kEmbreeEps;
+  for (int c = 0; c < 3; ++c) {
+    const T oc = Pi[c] + e * Ni[c];
+    o[c] = (double)(float)oc;
+    d[c] = (double)(float)D[c];
+  }
+}
 
 // Facet-to-facet visibility of facet i -> facet j.
 //   CGAL mode (aabb_wrapper.cpp:64-93): origin = FP64 centroid of i,
 //   direction = centroid_j - centroid_i, i skipped, visible iff j is the
 //   closest hit; i == j -> false.
-//   EMBREE mode (shape.py:349-398): d = P_j - P_i, |d| <= 1e-3 -> visible;
-//   origin = P_i + 1e-3 d/|d|, unit direction, nothing skipped, visible iff j
-//   is the closest hit.  Pm: model centroids in FP64.
+//   EMBREE mode (shape.py:349-398): the ray of embree_pair_ray from the
+//   model centroids Pi, Pj (read in this mode only), nothing skipped,
+//   visible iff j is the closest hit.
+template <typename T>
 __device__ inline bool face_to_face_vis(const flux_scene &S, int i, int j, int mode,
-                                        const double *pi_model, const double *pj_model) {
+                                        const T *Pi, const T *Pj) {
   double o[3], d[3];
   int skip = -1;
   if (mode == FLUX_MODE_CGAL) {
@@ -209,13 +244,7 @@ __device__ inline bool face_to_face_vis(const flux_scene &S, int i, int j, int m
     }
     skip = i;
   } else {
-    for (int c = 0; c < 3; ++c) d[c] = pj_model[c] - pi_model[c];
-    double nrm = sqrt((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
-    if (!(nrm > kEmbreeEps)) return true;
-    for (int c = 0; c < 3; ++c) {
-      d[c] = d[c] / nrm;
-      o[c] = pi_model[c] + kEmbreeEps * d[c];
-    }
+    if (!embree_pair_ray(Pi, Pj, o, d)) return true;
   }
   Ray64 r;
   ray_setup(r, o, d);

@@ -17,10 +17,12 @@ __global__ void k_vis_mn(flux_scene S, const int64_t *__restrict__ I, int64_t m,
     return;
   }
   int i = (int)I[p], j = (int)J[q];
-  double pi[3] = {0, 0, 0}, pj[3] = {0, 0, 0};
+  T pi[3] = {0, 0, 0}, pj[3] = {0, 0, 0};
   if (mode == FLUX_MODE_EMBREE) {
-    load3(Pm, i, pi);
-    load3(Pm, j, pj);
+    for (int c = 0; c < 3; ++c) {
+      pi[c] = Pm[(int64_t)i * 3 + c];
+      pj[c] = Pm[(int64_t)j * 3 + c];
+    }
   }
   out[p * n + q] = face_to_face_vis(S, i, j, mode, pi, pj) ? 1 : 0;
 }
@@ -52,10 +54,7 @@ __global__ void k_occluded(flux_scene S, const int64_t *__restrict__ I, int64_t 
     o[2] = ci[2];
     skip = i;
   } else {
-    double pi[3], ni[3];
-    load3(Pm, i, pi);
-    load3(Nrm, i, ni);
-    for (int c = 0; c < 3; ++c) o[c] = pi[c] + kEmbreeEps * ni[c];
+    embree_sun_ray(Pm + (int64_t)i * 3, Nrm + (int64_t)i * 3, dq, o, d);
   }
   Ray64 r;
   ray_setup(r, o, d);
@@ -100,15 +99,13 @@ __global__ void k_direct_irradiance(flux_scene S, const T *__restrict__ Nrm,
   if (mode == FLUX_MODE_CGAL) {
     for (int c = 0; c < 3; ++c) o[c] = S.cen[f * 3 + c];
     skip = (int)f;
-  } else {
-    double pf[3];
-    load3(Pm, f, pf);
-    for (int c = 0; c < 3; ++c) o[c] = pf[c] + kEmbreeEps * nf3[c];
   }
+  double dr[3] = {d[0], d[1], d[2]};  // the ray's direction (FP32 in EMBREE mode)
+  if (mode != FLUX_MODE_CGAL) embree_sun_ray(Pm + f * 3, Nrm + f * 3, d, o, dr);
   Ray64 r;
-  ray_setup(r, o, d);
+  ray_setup(r, o, dr);
   RayBox rb;
-  raybox_setup(rb, o, d);
+  raybox_setup(rb, o, dr);
   const bool occ = any_blocker(S, r, rb, INFINITY, 0x7fffffff, skip, -1);
   const double cosi = (nf3[0] * d[0] + nf3[1] * d[1]) + nf3[2] * d[2];
   E[idx] = occ ? (T)0 : (T)(F0 * (cosi > 0.0 ? cosi : 0.0));

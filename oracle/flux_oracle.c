@@ -309,9 +309,60 @@ static int cls_cb(void *p, int k) {
   return 0;
 }
 
-static const double kEmb = 1e-3; /* 1e3 * float32 resolution, shape.py:351 */
+/* eps of shape.py:351, 405: 1e3 * np.finfo(np.float32).resolution, the
+ * float32 1e-6 times a Python float, an FP64 number (NumPy 1.x). */
+static const double kEmb = 1e3 * (double)1e-6f;
 
-static int ovis(const oscene *s, int i, int j, int mode, const double *Pm) {
+/* Embree's ray for the pair (i, j) as fluxpy forms it (shape.py:355-385):
+ * d = P_j - P_i, |d| = sqrt((d0^2 + d1^2) + d2^2), the unit direction and
+ * the origin P_i + eps d/|d| in the model dtype (f32 != 0: FP32; NumPy
+ * applies the FP64 eps to an FP32 array in FP32), then stored into
+ * Embree's FP32 ray arrays.  Returns 0 when |d| <= eps (visible by
+ * default).  pi, pj: model centroids widened to FP64 (exact). */
+static int oemb_pair(const double *pi, const double *pj, int f32, double o[3], double d[3]) {
+  int c;
+  if (f32) {
+    float e = (float)kEmb, dd[3], nrm;
+    for (c = 0; c < 3; ++c) dd[c] = (float)pj[c] - (float)pi[c];
+    nrm = sqrtf((dd[0] * dd[0] + dd[1] * dd[1]) + dd[2] * dd[2]);
+    if (!(nrm > e)) return 0;
+    for (c = 0; c < 3; ++c) {
+      float dh = dd[c] / nrm;
+      float oc = (float)pi[c] + e * dh;
+      o[c] = (double)oc;
+      d[c] = (double)dh;
+    }
+  } else {
+    double dd[3], nrm;
+    for (c = 0; c < 3; ++c) dd[c] = pj[c] - pi[c];
+    nrm = sqrt((dd[0] * dd[0] + dd[1] * dd[1]) + dd[2] * dd[2]);
+    if (!(nrm > kEmb)) return 0;
+    for (c = 0; c < 3; ++c) {
+      double dh = dd[c] / nrm;
+      o[c] = (double)(float)(pi[c] + kEmb * dh);
+      d[c] = (double)(float)dh;
+    }
+  }
+  return 1;
+}
+
+/* Embree's occlusion ray (shape.py:410-411): origin P + eps N in the model
+ * dtype, direction D, both stored as FP32. */
+static void oemb_sun(const double *pi, const double *ni, const double *D, int f32, double o[3],
+                     double d[3]) {
+  int c;
+  for (c = 0; c < 3; ++c) {
+    if (f32) {
+      float oc = (float)pi[c] + (float)kEmb * (float)ni[c];
+      o[c] = (double)oc;
+    } else {
+      o[c] = (double)(float)(pi[c] + kEmb * ni[c]);
+    }
+    d[c] = (double)(float)D[c];
+  }
+}
+
+static int ovis(const oscene *s, int i, int j, int mode, const double *Pm, int f32) {
   double o[3], d[3], tj;
   int skip = -1, c;
   oray r;
@@ -323,14 +374,7 @@ static int ovis(const oscene *s, int i, int j, int mode, const double *Pm) {
     }
     skip = i;
   } else {
-    double nrm;
-    for (c = 0; c < 3; ++c) d[c] = Pm[(int64_t)j * 3 + c] - Pm[(int64_t)i * 3 + c];
-    nrm = sqrt((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
-    if (!(nrm > kEmb)) return 1;
-    for (c = 0; c < 3; ++c) {
-      d[c] = d[c] / nrm;
-      o[c] = Pm[(int64_t)i * 3 + c] + kEmb * d[c];
-    }
+    if (!oemb_pair(Pm + (int64_t)i * 3, Pm + (int64_t)j * 3, f32, o, d)) return 1;
   }
   oray_init(&r, o, d);
   tj = otri(&r, s->tri + (int64_t)j * 9);
@@ -338,20 +382,36 @@ static int ovis(const oscene *s, int i, int j, int mode, const double *Pm) {
   return !oany(s, o, d, tj, j, skip, j);
 }
 
-/* out[p*n+q] = vis(I[p], J[q]).  Pm: model centroids (FP64), EMBREE only. */
+/* The Embree rays of m pairs (pi[p], pj[p]) (oemb_pair): o, d (m x 3) and
+ * ok[p] = 0 where |d| <= eps (visible without a ray). */
+void oracle_embree_pair_rays(const double *pi, const double *pj, int64_t m, int f32, double *o,
+                             double *d, uint8_t *ok) {
+  int64_t p;
+  for (p = 0; p < m; ++p) ok[p] = (uint8_t)oemb_pair(pi + p * 3, pj + p * 3, f32, o + p * 3, d + p * 3);
+}
+
+/* The Embree occlusion rays of m facets (oemb_sun) toward one direction D. */
+void oracle_embree_sun_rays(const double *pi, const double *ni, int64_t m, const double *D,
+                            int f32, double *o, double *d) {
+  int64_t p;
+  for (p = 0; p < m; ++p) oemb_sun(pi + p * 3, ni + p * 3, D, f32, o + p * 3, d + p * 3);
+}
+
+/* out[p*n+q] = vis(I[p], J[q]).  Pm: model centroids widened to FP64,
+ * EMBREE only; f32: the model dtype is FP32. */
 void oracle_vis_mn(void *sp, const int64_t *I, int64_t m, const int64_t *J, int64_t n,
-                   int mode, const double *Pm, uint8_t *out) {
+                   int mode, const double *Pm, int f32, uint8_t *out) {
   const oscene *s = (const oscene *)sp;
   int64_t p;
 #pragma omp parallel for schedule(dynamic, 1)
   for (p = 0; p < m; ++p) {
     int64_t q;
-    for (q = 0; q < n; ++q) out[p * n + q] = (uint8_t)ovis(s, (int)I[p], (int)J[q], mode, Pm);
+    for (q = 0; q < n; ++q) out[p * n + q] = (uint8_t)ovis(s, (int)I[p], (int)J[q], mode, Pm, f32);
   }
 }
 
 void oracle_occluded(void *sp, const int64_t *I, int64_t m, const double *D, int64_t M,
-                     int per_face, int mode, const double *Pm, const double *Nm,
+                     int per_face, int mode, const double *Pm, const double *Nm, int f32,
                      uint8_t *out) {
   const oscene *s = (const oscene *)sp;
   int64_t cols = per_face ? 1 : M, p;
@@ -367,7 +427,7 @@ void oracle_occluded(void *sp, const int64_t *I, int64_t m, const double *D, int
         for (c = 0; c < 3; ++c) o[c] = s->cen[(int64_t)i * 3 + c];
         skip = i;
       } else {
-        for (c = 0; c < 3; ++c) o[c] = Pm[(int64_t)i * 3 + c] + kEmb * Nm[(int64_t)i * 3 + c];
+        oemb_sun(Pm + (int64_t)i * 3, Nm + (int64_t)i * 3, dq, f32, o, d);
       }
       out[p * cols + q] = (uint8_t)oany(s, o, d, INFINITY, 0x7fffffff, skip, -1);
     }
@@ -415,14 +475,14 @@ void oracle_intersect1(void *sp, const double *x, const double *d, int64_t nr, i
       int vis;                                                                            \
       if (i == j || !(c > eps)) continue;                                                 \
       if (mode == 0) {                                                                    \
-        vis = ovis(s, i, j, 0, NULL);                                                     \
+        vis = ovis(s, i, j, 0, NULL, 0);                                                  \
       } else {                                                                            \
         int k;                                                                            \
         for (k = 0; k < 3; ++k) {                                                         \
           pm_i[k] = (double)Pi[k];                                                        \
           pm_j[k] = (double)Pj[k];                                                        \
         }                                                                                 \
-        vis = ovis_embree_ids(s, j, pm_i, pm_j);                                          \
+        vis = ovis_embree_ids(s, j, pm_i, pm_j, sizeof(T) == 4);                          \
       }                                                                                   \
       if (!vis) continue;                                                                 \
       {                                                                                   \
@@ -438,17 +498,10 @@ void oracle_intersect1(void *sp, const double *x, const double *d, int64_t nr, i
   }
 
 /* Embree-mode visibility of facet j from model centroid pi (shape.py:349-398). */
-static int ovis_embree_ids(const oscene *s, int j, const double *pi, const double *pj) {
-  double o[3], d[3], nrm, tj;
-  int c;
+static int ovis_embree_ids(const oscene *s, int j, const double *pi, const double *pj, int f32) {
+  double o[3], d[3], tj;
   oray r;
-  for (c = 0; c < 3; ++c) d[c] = pj[c] - pi[c];
-  nrm = sqrt((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
-  if (!(nrm > kEmb)) return 1;
-  for (c = 0; c < 3; ++c) {
-    d[c] = d[c] / nrm;
-    o[c] = pi[c] + kEmb * d[c];
-  }
+  if (!oemb_pair(pi, pj, f32, o, d)) return 1;
   oray_init(&r, o, d);
   tj = otri(&r, s->tri + (int64_t)j * 9);
   if (tj < 0.0) return 0;

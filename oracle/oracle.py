@@ -51,8 +51,10 @@ def lib():
         L.oracle_scene_destroy.argtypes = [vp]
         L.oracle_scene_centroids.restype = ctypes.POINTER(ctypes.c_double)
         L.oracle_scene_centroids.argtypes = [vp]
-        L.oracle_vis_mn.argtypes = [vp, vp, i64, vp, i64, i32, vp, vp]
-        L.oracle_occluded.argtypes = [vp, vp, i64, vp, i64, i32, i32, vp, vp, vp]
+        L.oracle_vis_mn.argtypes = [vp, vp, i64, vp, i64, i32, vp, i32, vp]
+        L.oracle_embree_pair_rays.argtypes = [vp, vp, i64, i32, vp, vp, vp]
+        L.oracle_embree_sun_rays.argtypes = [vp, vp, i64, vp, i32, vp, vp]
+        L.oracle_occluded.argtypes = [vp, vp, i64, vp, i64, i32, i32, vp, vp, i32, vp]
         L.oracle_intersect1.argtypes = [vp, vp, vp, i64, vp, vp]
         L.oracle_ff_csr.argtypes = [vp, vp, vp, vp, i32, vp, i64, vp, i64, ctypes.c_double,
                                     i32, vp, vp, vp, vp]
@@ -115,7 +117,8 @@ class OracleScene:
         J = np.ascontiguousarray(J, dtype=np.int64)
         out = np.zeros((I.size, J.size), dtype=np.uint8)
         Pm = None if P is None else np.ascontiguousarray(P, dtype=np.float64)
-        lib().oracle_vis_mn(self.h, _p(I), I.size, _p(J), J.size, mode, _p(Pm), _p(out))
+        f32 = int(P is not None and np.asarray(P).dtype == np.float32)
+        lib().oracle_vis_mn(self.h, _p(I), I.size, _p(J), J.size, mode, _p(Pm), f32, _p(out))
         return out.astype(bool)
 
     def occluded(self, I, D, per_face=False, mode=MODE_CGAL, P=None, N=None):
@@ -125,8 +128,9 @@ class OracleScene:
         out = np.zeros((I.size, cols), dtype=np.uint8)
         Pm = None if P is None else np.ascontiguousarray(P, dtype=np.float64)
         Nm = None if N is None else np.ascontiguousarray(N, dtype=np.float64)
+        f32 = int(P is not None and np.asarray(P).dtype == np.float32)
         lib().oracle_occluded(self.h, _p(I), I.size, _p(D), D.shape[0], int(per_face), mode,
-                              _p(Pm), _p(Nm), _p(out))
+                              _p(Pm), _p(Nm), f32, _p(out))
         return out.astype(bool)
 
     def intersect1(self, x, d):
@@ -162,6 +166,37 @@ class OracleScene:
 # --------------------------------------------------------------------------
 # Insolation, solvers, temperature (NumPy restatements).
 # --------------------------------------------------------------------------
+
+def embree_pair_rays(Pi, Pj):
+    """Embree's ray inputs for the pairs (Pi[p], Pj[p]) as shape.py:355-385
+    forms them, in the dtype of Pi (flux_oracle.c oemb_pair): (origins,
+    directions, mask of the pairs that get a ray)."""
+    Pi = np.asarray(Pi)
+    f32 = int(Pi.dtype == np.float32)
+    a = np.ascontiguousarray(Pi, dtype=np.float64)
+    b = np.ascontiguousarray(Pj, dtype=np.float64)
+    m = a.shape[0]
+    o = np.zeros((m, 3))
+    d = np.zeros((m, 3))
+    ok = np.zeros(m, dtype=np.uint8)
+    lib().oracle_embree_pair_rays(_p(a), _p(b), m, f32, _p(o), _p(d), _p(ok))
+    return o, d, ok.astype(bool)
+
+
+def embree_sun_rays(P, N, D):
+    """Embree's occlusion-ray inputs (shape.py:410-411) for the facets of P,
+    N toward the direction D (flux_oracle.c oemb_sun)."""
+    P = np.asarray(P)
+    f32 = int(P.dtype == np.float32)
+    a = np.ascontiguousarray(P, dtype=np.float64)
+    b = np.ascontiguousarray(N, dtype=np.float64)
+    Dd = np.ascontiguousarray(D, dtype=np.float64).reshape(3)
+    m = a.shape[0]
+    o = np.zeros((m, 3))
+    d = np.zeros((m, 3))
+    lib().oracle_embree_sun_rays(_p(a), _p(b), m, _p(Dd), f32, _p(o), _p(d))
+    return o, d
+
 
 def direct_irradiance(N, occluded, F0, d):
     """1-D sun branch of shape.py:224-226 with the dot product in FP64.

@@ -361,3 +361,37 @@ def test_insolation_is_cosine_law(oracle_mod):
     d = np.array([np.cos(e), 0, np.sin(e)])
     E = oracle_mod.direct_irradiance(N, S.occluded(np.arange(len(A)), d)[:, 0], 1000.0, d)
     assert np.allclose(E, 1000 * np.sin(e))
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_embree_ray_inputs_follow_the_reference_expressions(oracle_mod, dt):
+    """The oracle's Embree ray inputs equal those of fluxpy's own NumPy
+    expressions (shape.py:351-385 for pairs, 405-411 for occlusion) on the
+    same arrays, stored as FP32 (Embree's ray arrays).  For FP32 models
+    NumPy 1.x and 2.x agree on them (eps = 1e3 * float32 resolution rounds
+    once to FP32 either way); for FP64 models eps is the FP64 product, as in
+    the NumPy 1.x the reference requires.  Pairs closer than eps get no
+    ray."""
+    rng = np.random.default_rng(17)
+    m = 400
+    P = rng.uniform(-1, 1, (m, 3)).astype(dt)
+    Pj = (P + rng.normal(0, 1, (m, 3)) * np.where(rng.random((m, 1)) < 0.1, 1e-4, 1.0)).astype(dt)
+    eps = 1e3 * np.finfo(np.float32).resolution if dt == np.float32 \
+        else 1e3 * float(np.finfo(np.float32).resolution)
+    # shape.py:357-364, 377-378 for one row at a time
+    D = (Pj - P).astype(dt)
+    D_norm = np.sqrt(np.sum(D ** 2, axis=1))
+    mask = D_norm > eps
+    Dn = D[mask] / D_norm[mask].reshape(-1, 1)
+    org = P[mask] + eps * Dn
+    o, d, ok = oracle_mod.embree_pair_rays(P, Pj)
+    assert np.array_equal(ok, mask) and (~mask).sum() > 10
+    assert np.array_equal(o[ok], org.astype(np.float32).astype(np.float64))
+    assert np.array_equal(d[ok], Dn.astype(np.float32).astype(np.float64))
+    N = rng.normal(0, 1, (m, 3))
+    N = (N / np.linalg.norm(N, axis=1, keepdims=True)).astype(dt)
+    Dsun = np.array([0.3, -0.2, 0.93])
+    Dsun /= np.linalg.norm(Dsun)
+    o, d = oracle_mod.embree_sun_rays(P, N, Dsun)
+    assert np.array_equal(o, (P + eps * N).astype(np.float32).astype(np.float64))
+    assert np.array_equal(d, np.tile(Dsun.astype(np.float32).astype(np.float64), (m, 1)))
