@@ -433,10 +433,12 @@ __global__ void __launch_bounds__(256) k_batched_syevj(
         sn[k] = s;
       }
       __syncthreads();
-      // columns: A <- A J, V <- V J
+      // columns: A <- A J, V <- V J.  Consecutive threads take consecutive
+      // pairs of one row, so a wave's loads stay within that row (a
+      // thread per row of one column would touch a cache line per lane).
       for (int64_t w = tid; w < (int64_t)half * n; w += nt) {
-        const int k = (int)(w / n);
-        const int64_t i = w % n;
+        const int k = (int)(w % half);
+        const int64_t i = w / half;
         const int p = pp[k], q = qq[k];
         if (q >= n || sn[k] == 0.0) continue;
         const double c = cs[k], s = sn[k];
