@@ -324,16 +324,22 @@ class _SvdBatch:
                   d_oG.data_ptr(), self.lam.data_ptr(), d_oE.data_ptr(), st)
         _gemm(np.stack([rs, n, l, oG, l, zeros + 1, self.offZ[bs], l, zeros + 1, offV[:-1], n,
                         zeros], 1), self.Xe, self.Z, Vt, st)
-        lam = self.lam
+        # S = sqrt(max(lambda, 0)) for every pick in one launch; each factor
+        # buffer converted once; the leaves' factors are views into them
+        offS = np.concatenate([[0], np.cumsum(rs)])
+        S = t.empty(max(int(offS[-1]), 1), dtype=t.float64, device=dev)
+        _check_extent("singular values", oE, rs, self.lam.numel())
+        d_oL, d_rs, d_oS = _i64(oE), _i64(rs), _i64(offS[:-1])
+        _lib.call("flux_sigma_from_lambda", np_, d_oL.data_ptr(), d_rs.data_ptr(),
+                  d_oS.data_ptr(), self.lam.data_ptr(), S.data_ptr(), st)
+        if out_dtype != t.float64:
+            U, S, Vt = (_kernels.convert(x, out_dtype) for x in (U, S, Vt))
         for q in range(np_):
-            b, r = int(bs[q]), int(rs[q])
-            u = U[offU[q]:offU[q + 1]].reshape(int(m[q]), r)
-            vt = Vt[offV[q]:offV[q + 1]].reshape(r, int(n[q]))
-            s = lam[self.offE[b]:self.offE[b] + r].clamp_min(0).sqrt()
-            res.append((_kernels.convert(u.contiguous(), out_dtype),
-                        _kernels.convert(s.contiguous(), out_dtype),
-                        _kernels.convert(vt.contiguous(), out_dtype)))
-        del U, Vt, d_dims, d_oG, d_oE
+            r = int(rs[q])
+            res.append((U[offU[q]:offU[q + 1]].reshape(int(m[q]), r),
+                        S[offS[q]:offS[q + 1]],
+                        Vt[offV[q]:offV[q + 1]].reshape(r, int(n[q]))))
+        del d_dims, d_oG, d_oE, d_oL, d_rs, d_oS
         return res
 
 

@@ -63,6 +63,7 @@ SIGNATURES = {
                                     _vp]),
     "flux_batched_ns_factor": (_c_int, [_c_i64, _vp, _vp, _vp, _vp]),
     "flux_scale_basis": (_c_int, [_c_i64, _vp, _vp, _vp, _vp, _vp, _c_dbl, _vp, _vp, _vp]),
+    "flux_sigma_from_lambda": (_c_int, [_c_i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "flux_scale_cols_inv_sqrt": (_c_int, [_c_i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "flux_far_matmat": (_c_int, [_c_i64, _c_i64, _c_i64, _vp, _vp, _c_i64, _vp, _vp, _vp, _c_int,
                                  _c_i64, _vp, _vp, _c_i64, _vp, _c_i64, _vp, _c_i64, _vp, _vp,

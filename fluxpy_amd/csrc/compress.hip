@@ -630,6 +630,31 @@ extern "C" int flux_scale_basis(int64_t nb, const int64_t *ns, const double *V,
   return 0;
 }
 
+// S[offS[q] + i] = sqrt(max(lam[offL[q] + i], 0)), i < r[q]: the singular
+// values of the accepted blocks (the eigenvalues of Z^T Z) in one buffer.
+__global__ void k_sigma_from_lambda(int64_t np_, const int64_t *__restrict__ offL,
+                                    const int64_t *__restrict__ r,
+                                    const int64_t *__restrict__ offS,
+                                    const double *__restrict__ lam, double *__restrict__ S) {
+  const int64_t q = blockIdx.y;
+  if (q >= np_) return;
+  for (int64_t i = threadIdx.x + (int64_t)blockIdx.x * blockDim.x; i < r[q];
+       i += (int64_t)gridDim.x * blockDim.x)
+    S[offS[q] + i] = sqrt(fmax(lam[offL[q] + i], 0.0));
+}
+
+extern "C" int flux_sigma_from_lambda(int64_t np_, const int64_t *offL, const int64_t *r,
+                                      const int64_t *offS, const double *lam, double *S,
+                                      void *stream) {
+  FLUX_CHECK(np_ >= 0 && np_ <= 65535, "flux_sigma_from_lambda: bad sizes");
+  if (np_ == 0) return 0;
+  FLUX_CHECK(offL && r && offS && lam && S, "flux_sigma_from_lambda: NULL array");
+  hipLaunchKernelGGL(k_sigma_from_lambda, dim3(1, (unsigned)np_), dim3(256), 0,
+                     as_stream(stream), np_, offL, r, offS, lam, S);
+  FLUX_LAUNCH_CHECK();
+  return 0;
+}
+
 extern "C" int flux_scale_cols_inv_sqrt(int64_t nb, const int64_t *dims, double *C,
                                         const int64_t *offC, const double *lam,
                                         const int64_t *offL, void *stream) {

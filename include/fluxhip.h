@@ -273,6 +273,11 @@ int flux_scale_basis(int64_t nb, const int64_t *ns, const double *V,
                      double cut, double *T, int64_t *kept, void *stream);
 
 /* Columns j of C_b (dims[b] = rows, cols) divided by sqrt(lam_j). */
+/* S[offS[q] + i] = sqrt(max(lam[offL[q] + i], 0)) for i < r[q], q < np_:
+ * the singular values of the accepted sketch blocks in one buffer. */
+int flux_sigma_from_lambda(int64_t np_, const int64_t *offL, const int64_t *r,
+                           const int64_t *offS, const double *lam, double *S,
+                           void *stream);
 int flux_scale_cols_inv_sqrt(int64_t nb, const int64_t *dims, double *C,
                              const int64_t *offC, const double *lam,
                              const int64_t *offL, void *stream);
