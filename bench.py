@@ -434,16 +434,21 @@ def lowrank_line(torch, sm, args, E64, rho, rho_d, B64_all, rel_err, timed, make
     """The same batched solve on the tree WITH low-rank leaves: near-field
     panels plus the far field (BF16 MFMA in the sweeps, FP32 master factors
     with FP64 vectors in the residual)."""
+    from fluxpy_amd import _hbuild
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     H = CompressedFormFactorMatrix(sm, tol=args.tol, min_size=args.min_size)
     torch.cuda.synchronize()
     t_build = time.perf_counter() - t0
+    # the sketch's MFMA launches timed in a second build of the same tree
+    _hbuild.TIME_MFMA = True
+    mfma = CompressedFormFactorMatrix(sm, tol=args.tol, min_size=args.min_size) \
+        .build_stats["sketch_mfma"]
+    _hbuild.TIME_MFMA = False
     bs = H.build_stats
     op64 = H.device_operator("fp64")
     Bc, _ = solve_radiosity(op64, E64, rho, 'right')                # FP64, same operator
     t_step, B, info = timed(make_step(H, args.residual), max(5, args.steps // 5), 2)
-    from fluxpy_amd import _hbuild
     far = _hbuild.far16(H, "bf16")
     res = {"solves_per_s": round(args.suns / t_step, 2), "ms_per_step": round(t_step * 1e3, 3),
            "rounds": info["rounds"], "build_s": round(t_build, 2),
@@ -451,11 +456,12 @@ def lowrank_line(torch, sm, args, E64, rho, rho_d, B64_all, rel_err, timed, make
            "svd_leaves": bs["svd_leaves"], "leaves": bs["leaf_counts"],
            "near_nnz": H._near.nnz, "far_factor_bytes_bf16": far.nbytes,
            "max_rel_err_vs_fp64_same_operator": rel_err(B, Bc),
-           "max_rel_err_vs_fp64_exact_F": rel_err(B, B64_all)}
+           "max_rel_err_vs_fp64_exact_F": rel_err(B, B64_all),
+           "sketch_mfma": {"type": "fp64", **mfma, "peak": 78.6}}
     log(f"[bench] low-rank tree: build {t_build:.1f} s, {bs['svd_leaves']} SVD leaves, "
         f"nbytes {H.nbytes / 1e6:.0f} MB, {res['solves_per_s']:.0f} solves/s, err vs FP64 "
         f"same operator {res['max_rel_err_vs_fp64_same_operator']:.1e}, vs exact F "
-        f"{res['max_rel_err_vs_fp64_exact_F']:.1e}")
+        f"{res['max_rel_err_vs_fp64_exact_F']:.1e}; sketch MFMA {mfma}")
     del H, op64, Bc, B
     torch.cuda.empty_cache()
     return res

@@ -90,6 +90,46 @@ def _check_gemm(desc, sizeA, sizeB, sizeC):
     _check_extent("batched GEMM result C", d[:, 9], _gemm_extent(M, N, d[:, 10], 0), sizeC)
 
 
+# FP64 MFMA work of the sketch (batched Gram matrices and GEMMs): flop count
+# per build, and with TIME_MFMA the launches' durations (HIP events on the
+# current stream), for the compression's MFMA utilisation
+TIME_MFMA = False
+_MFMA = {"flops": 0, "events": []}
+
+
+def _mfma_launch(flops, name, *args):
+    _MFMA["flops"] += int(flops)
+    if not TIME_MFMA:
+        _lib.call(name, *args)
+        return
+    t = _lib.torch()
+    e0, e1 = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
+    e0.record()
+    _lib.call(name, *args)
+    e1.record()
+    _MFMA["events"].append((e0, e1))
+
+
+def _mfma_stats():
+    """{'flops', 'ms', 'TFLOP/s', 'frac'} of the sketch's MFMA launches since
+    the last call (timings only with TIME_MFMA)."""
+    out = {"flops": _MFMA["flops"]}
+    if _MFMA["events"]:
+        _lib.torch().cuda.synchronize()
+        ms = sum(a.elapsed_time(b) for a, b in _MFMA["events"])
+        out.update({"ms": round(ms, 3), "TFLOP/s": round(_MFMA["flops"] / ms / 1e9, 2),
+                    "frac": round(_MFMA["flops"] / ms / 1e9 / 78.6, 4)})
+    _MFMA["flops"], _MFMA["events"] = 0, []
+    return out
+
+
+def _gram(nb, d_rows, rows, d_ls, ls, lmax, X, d_offX, G, d_offG, st):
+    """flux_batched_gram: G_b = X_b^T X_b (2 rows_b l_b^2 flops each)."""
+    flops = int((2 * np.asarray(rows, dtype=np.int64) * np.asarray(ls, dtype=np.int64) ** 2).sum())
+    _mfma_launch(flops, "flux_batched_gram", nb, d_rows.data_ptr(), d_ls.data_ptr(), lmax,
+                 X.data_ptr(), d_offX.data_ptr(), G.data_ptr(), d_offG.data_ptr(), st)
+
+
 def _gemm(desc, A, B, C, st):
     """flux_batched_gemm after the host-side bounds check."""
     desc = np.ascontiguousarray(desc, dtype=np.int64).reshape(-1, 12)
@@ -98,8 +138,9 @@ def _gemm(desc, A, B, C, st):
     _check_gemm(desc, A.numel(), B.numel(), C.numel())
     tiles = int((((desc[:, 0] + 15) // 16) * ((desc[:, 1] + 15) // 16)).max())
     d_desc = _i64(desc)
-    _lib.call("flux_batched_gemm", desc.shape[0], d_desc.data_ptr(), tiles, A.data_ptr(),
-              B.data_ptr(), C.data_ptr(), st)
+    _mfma_launch(int((2 * desc[:, 0] * desc[:, 1] * desc[:, 2]).sum()), "flux_batched_gemm",
+                 desc.shape[0], d_desc.data_ptr(), tiles, A.data_ptr(), B.data_ptr(),
+                 C.data_ptr(), st)
     del d_desc
 
 
@@ -240,8 +281,7 @@ class _SvdBatch:
                   status.data_ptr(), st)
         basis = np.stack([m, l, l, offY[:-1], l, zeros, og, l, zeros, offY[:-1], l, zeros], 1)
         # rank-revealing pass: Q = Y V D^-1/2 from the eigenpairs of Y^T Y
-        _lib.call("flux_batched_gram", nb, d_m.data_ptr(), d_ls.data_ptr(), lmax,
-                  Q1.data_ptr(), d_offY.data_ptr(), G.data_ptr(), d_offG.data_ptr(), st)
+        _gram(nb, d_m, m, d_ls, l, lmax, Q1, d_offY, G, d_offG, st)
         _lib.call("flux_batched_syevj", nb, d_ls.data_ptr(), lmax, G.data_ptr(),
                   d_offG.data_ptr(), Vw.data_ptr(), self.Xe.data_ptr(),
                   self.lam.data_ptr(), d_offE.data_ptr(), 30, st)
@@ -255,8 +295,7 @@ class _SvdBatch:
         # (zero) columns zero
         cur, other = self.Q, Q1
         for _ in range(NS_STEPS):
-            _lib.call("flux_batched_gram", nb, d_m.data_ptr(), d_ls.data_ptr(), lmax,
-                      cur.data_ptr(), d_offY.data_ptr(), G.data_ptr(), d_offG.data_ptr(), st)
+            _gram(nb, d_m, m, d_ls, l, lmax, cur, d_offY, G, d_offG, st)
             _lib.call("flux_batched_ns_factor", nb, d_ls.data_ptr(), G.data_ptr(),
                       d_offG.data_ptr(), st)
             _gemm(basis, cur, G, other, st)
@@ -269,8 +308,7 @@ class _SvdBatch:
                   d_rsT.data_ptr(), int(n.sum()), self.Q.data_ptr(),
                   d_offY.data_ptr(), self.Z.data_ptr(), d_offZ.data_ptr(), seed,
                   status.data_ptr(), st)
-        _lib.call("flux_batched_gram", nb, d_n.data_ptr(), d_ls.data_ptr(), lmax,
-                  self.Z.data_ptr(), d_offZ.data_ptr(), G.data_ptr(), d_offG.data_ptr(), st)
+        _gram(nb, d_n, n, d_ls, l, lmax, self.Z, d_offZ, G, d_offG, st)
         _lib.call("flux_batched_syevj", nb, d_ls.data_ptr(), lmax, G.data_ptr(),
                   d_offG.data_ptr(), Vw.data_ptr(), self.Xe.data_ptr(), self.lam.data_ptr(),
                   d_offE.data_ptr(), 30, st)
@@ -500,6 +538,7 @@ def _collect(nd, out_far, out_ranges):
 def build(H, dims):
     t = _lib.require_gpu()
     t0 = time.perf_counter()
+    _MFMA["flops"], _MFMA["events"] = 0, []
     sm = H.shape_model
     dt = np.dtype(sm.dtype)
     itemsize = dt.itemsize
@@ -573,7 +612,7 @@ def build(H, dims):
     H._build_stats.update({
         "facets": sm.num_faces, "nnz": F.nnz, "levels": nlev,
         "cluster_s": t1 - t0, "assembly_s": t2 - t1, "tree_s": t3 - t2, "finish_s": t4 - t3,
-        "total_s": t4 - t0, "svd_leaves": len(H._far_used),
+        "total_s": t4 - t0, "svd_leaves": len(H._far_used), "sketch_mfma": _mfma_stats(),
         "leaf_counts": _leaf_counts(root),
     })
 

@@ -122,35 +122,68 @@ __global__ void __launch_bounds__(256) k_block_spmm(
   }
 }
 
-// G_b = X_b^T X_b (l_b x l_b) for X_b (rows_b x l_b) row-major.  One 16x16
-// tile of G per workgroup (grid.x over tiles, grid.y over blocks).
+// Batched FP64 products on v_mfma_f64_16x16x4_f64.  A workgroup computes a
+// 64 x 64 tile of C = op(A) op(B): K is consumed in slices of 16, staged in
+// LDS (A: 64 rows x 16, B: 16 x 64 columns, both zero outside the matrix),
+// and wave w owns rows 16 w .. 16 w + 15 of the tile (four 16 x 16 MFMA
+// accumulators).  Lane l = 16 g + i holds A[i][g], B[g][i] and D[4 q + g][i].
+typedef double v4d __attribute__((ext_vector_type(4)));
+constexpr int kGemmTile = 64;
+constexpr int kGemmKs = 16;
+
+// a(i, k) and b(k, j) read the operands (bounds already checked).
+template <class LA, class LB>
+__device__ __forceinline__ void mfma_tile64(LA a, LB b, int64_t M, int64_t N, int64_t K,
+                                            int64_t i0, int64_t j0, v4d acc[4]) {
+  __shared__ double sa[kGemmTile][kGemmKs + 1];
+  __shared__ double sb[kGemmKs][kGemmTile + 1];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int li = lane & 15, lg = lane >> 4;
+  for (int c = 0; c < 4; ++c) acc[c] = v4d{0.0, 0.0, 0.0, 0.0};
+  for (int64_t k0 = 0; k0 < K; k0 += kGemmKs) {
+    // 1024 values of each operand, 4 per thread
+    for (int e = tid; e < kGemmTile * kGemmKs; e += 256) {
+      const int r = e / kGemmKs, kk = e % kGemmKs;       // A: row r, k kk
+      const int64_t gi = i0 + r, gk = k0 + kk;
+      sa[r][kk] = (gi < M && gk < K) ? a(gi, gk) : 0.0;
+      const int kr = e / kGemmTile, cc = e % kGemmTile;   // B: k kr, column cc
+      const int64_t gk2 = k0 + kr, gj = j0 + cc;
+      sb[kr][cc] = (gk2 < K && gj < N) ? b(gk2, gj) : 0.0;
+    }
+    __syncthreads();
+    for (int s = 0; s < kGemmKs; s += 4) {
+      const double av = sa[16 * wave + li][s + lg];
+      for (int c = 0; c < 4; ++c)
+        acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, sb[s + lg][16 * c + li], acc[c], 0, 0,
+                                                      0);
+    }
+    __syncthreads();
+  }
+}
+
+// G_b = X_b^T X_b (l_b x l_b) for X_b (rows_b x l_b) row-major.  grid.x over
+// 64 x 64 tiles (surplus workgroups exit), grid.y over blocks.
 __global__ void __launch_bounds__(256) k_batched_gram(
     const int64_t *__restrict__ rows, const int64_t *__restrict__ ls,
     const double *__restrict__ X, const int64_t *__restrict__ offX, double *__restrict__ G,
     const int64_t *__restrict__ offG) {
   const int64_t b = blockIdx.y;
-  const int64_t l = ls[b];
-  const int64_t nt = (l + 15) / 16;
+  const int64_t l = ls[b], m = rows[b];
+  const int64_t nt = (l + kGemmTile - 1) / kGemmTile;
   const int64_t tile = blockIdx.x;
-  if (tile >= nt * nt) return;
-  const int64_t ti = tile / nt, tj = tile % nt;
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  const int64_t gi = ti * 16 + ty, gj = tj * 16 + tx;
-  __shared__ double sa[16][17], sb[16][17];
-  double acc = 0.0;
+  if (tile >= nt * nt) return;  // uniform over the workgroup
+  const int64_t i0 = (tile / nt) * kGemmTile, j0 = (tile % nt) * kGemmTile;
   const double *Xb = X + offX[b];
-  const int64_t m = rows[b];
-  for (int64_t r0 = 0; r0 < m; r0 += 16) {
-    // load X[r0 + ty, ti*16 + tx] and X[r0 + ty, tj*16 + tx]
-    const int64_t rr = r0 + ty;
-    const int64_t ca = ti * 16 + tx, cb = tj * 16 + tx;
-    sa[ty][tx] = (rr < m && ca < l) ? Xb[rr * l + ca] : 0.0;
-    sb[ty][tx] = (rr < m && cb < l) ? Xb[rr * l + cb] : 0.0;
-    __syncthreads();
-    for (int q = 0; q < 16; ++q) acc += sa[q][ty] * sb[q][tx];
-    __syncthreads();
-  }
-  if (gi < l && gj < l) G[offG[b] + gi * l + gj] = acc;
+  v4d acc[4];
+  mfma_tile64([&](int64_t i, int64_t k) { return Xb[k * l + i]; },
+              [&](int64_t k, int64_t j) { return Xb[k * l + j]; }, l, l, m, i0, j0, acc);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int li = lane & 15, lg = lane >> 4;
+  for (int c = 0; c < 4; ++c)
+    for (int q = 0; q < 4; ++q) {
+      const int64_t gi = i0 + 16 * wave + 4 * q + lg, gj = j0 + 16 * c + li;
+      if (gi < l && gj < l) G[offG[b] + gi * l + gj] = acc[c][q];
+    }
 }
 
 // C_b = op(A_b) op(B_b), FP64, row-major, per-block shapes.  desc rows of 12
@@ -161,33 +194,28 @@ __global__ void __launch_bounds__(256) k_batched_gemm(const int64_t *__restrict_
                                                       double *__restrict__ C) {
   const int64_t *d = desc + (int64_t)blockIdx.y * 12;
   const int64_t M = d[0], N = d[1], K = d[2];
-  const int64_t ntn = (N + 15) / 16;
+  const int64_t ntn = (N + kGemmTile - 1) / kGemmTile;
   const int64_t tile = blockIdx.x;
-  if (tile >= ((M + 15) / 16) * ntn) return;
-  const int64_t ti = tile / ntn, tj = tile % ntn;
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  if (tile >= ((M + kGemmTile - 1) / kGemmTile) * ntn) return;  // uniform
+  const int64_t i0 = (tile / ntn) * kGemmTile, j0 = (tile % ntn) * kGemmTile;
   const double *Ab = A + d[3];
   const double *Bb = B + d[6];
   const int64_t lda = d[4], ldb = d[7];
   const bool ta = d[5] != 0, tb = d[8] != 0;
-  __shared__ double sa[16][17], sb[16][17];
-  double acc = 0.0;
-  const int64_t gi = ti * 16 + ty, gj = tj * 16 + tx;
-  for (int64_t k0 = 0; k0 < K; k0 += 16) {
-    {  // A tile: rows ti*16.., cols k0..
-      const int64_t i = ti * 16 + ty, k = k0 + tx;
-      sa[ty][tx] = (i < M && k < K) ? (ta ? Ab[k * lda + i] : Ab[i * lda + k]) : 0.0;
-      const int64_t kk = k0 + ty, j = tj * 16 + tx;
-      sb[ty][tx] = (kk < K && j < N) ? (tb ? Bb[j * ldb + kk] : Bb[kk * ldb + j]) : 0.0;
+  v4d acc[4];
+  mfma_tile64([&](int64_t i, int64_t k) { return ta ? Ab[k * lda + i] : Ab[i * lda + k]; },
+              [&](int64_t k, int64_t j) { return tb ? Bb[j * ldb + k] : Bb[k * ldb + j]; }, M,
+              N, K, i0, j0, acc);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int li = lane & 15, lg = lane >> 4;
+  for (int c = 0; c < 4; ++c)
+    for (int q = 0; q < 4; ++q) {
+      const int64_t gi = i0 + 16 * wave + 4 * q + lg, gj = j0 + 16 * c + li;
+      if (gi < M && gj < N) {
+        double *cp = C + d[9] + gi * d[10] + gj;
+        *cp = d[11] ? *cp + acc[c][q] : acc[c][q];
+      }
     }
-    __syncthreads();
-    for (int q = 0; q < 16; ++q) acc += sa[ty][q] * sb[q][tx];
-    __syncthreads();
-  }
-  if (gi < M && gj < N) {
-    double *c = C + d[9] + gi * d[10] + gj;
-    *c = d[11] ? *c + acc : acc;
-  }
 }
 
 // Tournament pairing of step `step` of a cyclic Jacobi sweep over npad
@@ -573,7 +601,7 @@ extern "C" int flux_batched_gram(int64_t nb, const int64_t *rows, const int64_t 
                                  const int64_t *offG, void *stream) {
   FLUX_CHECK(nb >= 0 && nb <= 65535 && lmax >= 0, "flux_batched_gram: bad sizes");
   if (nb == 0 || lmax == 0) return 0;
-  const int64_t nt = (lmax + 15) / 16;
+  const int64_t nt = (lmax + kGemmTile - 1) / kGemmTile;
   hipLaunchKernelGGL(k_batched_gram, dim3((unsigned)(nt * nt), (unsigned)nb), dim3(256), 0,
                      as_stream(stream), rows, ls, X, offX, G, offG);
   FLUX_LAUNCH_CHECK();
