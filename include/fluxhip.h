@@ -31,14 +31,14 @@ enum { FLUX_F32 = 0, FLUX_F64 = 1, FLUX_BF16 = 2, FLUX_F16 = 3 };
 /* CGAL semantics (cgal/aabb_wrapper.cpp:64-108): ray origin = centroid of
  * facet i, the source facet is skipped, visible iff the closest hit is j,
  * no hit => not visible.
- * EMBREE semantics (shape.py:349-421): origin offset by 1e-3 along the unit
- * ray (visibility) or the normal (occlusion), nothing skipped, pairs closer
- * than 1e-3 are visible by default.  Both are evaluated with the same
- * FP64 watertight ray/triangle test. */
+ * EMBREE semantics (shape.py:349-421): origin offset by eps = 1e3 x float32
+ * resolution along the unit ray (visibility) or the normal (occlusion),
+ * formed in the model dtype and rounded to FP32 as the reference hands them
+ * to Embree, nothing skipped, pairs closer than eps are visible by default.
+ * Both are evaluated with the same FP64 watertight ray/triangle test. */
 enum { FLUX_MODE_CGAL = 0, FLUX_MODE_EMBREE = 1 };
 
 typedef struct flux_scene flux_scene;
-typedef struct flux_hmat flux_hmat;
 
 /* Thread-local description of the last error. Never NULL. */
 const char *flux_last_error(void);
