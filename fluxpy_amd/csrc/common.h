@@ -46,12 +46,15 @@ constexpr int kWave = 64;
 }  // namespace flux
 
 // Scene layout shared by the ray kernels (scene.hip builds it).
+// BVH node layouts per scene: one per octant of ray directions (scene.hip)
+constexpr int kOctants = 8;
+
 struct flux_scene {
   int device;
   int64_t nv, nf;
   int32_t nnodes;
   double *tri;    // nf * 9: v0, v1, v2 in FP64, original facet order
   double *cen;    // nf * 3: ((v0+v1)+v2)/3 in FP64
-  float4 *nodes;  // 2 * nnodes: {lo.xyz, escape bits}, {hi.xyz, tri bits}
+  float4 *nodes;  // kOctants layouts of 2 * nnodes: {lo.xyz, escape bits}, {hi.xyz, tri bits}
   float pad;      // absolute box padding used at build
 };

@@ -8,8 +8,8 @@
 // host.  Hits count for t >= 0 (the ray source included, as a CGAL Ray_3
 // does); edges are shared (a ray through a common edge hits both facets).
 //
-// Traversal: stackless walk over the preorder node array of scene.hip with
-// escape links.  Node boxes are FP32, rounded outward and padded at build,
+// Traversal: stackless walk over a preorder node array of scene.hip with
+// escape links (the layout of the ray's direction octant).  Node boxes are FP32, rounded outward and padded at build,
 // so the FP32 slab test is conservative: the set of triangles tested always
 // contains every triangle the FP64 test hits on the queried segment.  The
 // result of every query is therefore independent of the tree shape.
@@ -116,13 +116,21 @@ __device__ __forceinline__ bool slab(const RayBox &rb, const float4 &lo, const f
   return tn <= tf;
 }
 
+// The node layout of the ray's direction octant (scene.hip): children in
+// roughly front-to-back order for this ray.
+__device__ __forceinline__ const float4 *octant_nodes(const flux_scene &S, const RayBox &rb) {
+  const int oct = (rb.inv[0] < 0.0f ? 1 : 0) | (rb.inv[1] < 0.0f ? 2 : 0) |
+                  (rb.inv[2] < 0.0f ? 4 : 0);
+  return S.nodes + (int64_t)oct * 2 * S.nnodes;
+}
+
 // Any-hit query: is there a facet k (k != skip1, k != skip2) hit at t with
 // (t, k) lexicographically before (tlim, klim)?  tlim = +inf / klim = INT_MAX
 // gives "anything at t >= 0".
 __device__ inline bool any_blocker(const flux_scene &S, const Ray64 &r, const RayBox &rb,
                                    double tlim, int klim, int skip1, int skip2) {
   const float tmax = isinf(tlim) ? 3.0e38f : (float)(tlim * (1.0 + 1e-5)) + 1e-30f;
-  const float4 *__restrict__ nodes = S.nodes;
+  const float4 *__restrict__ nodes = octant_nodes(S, rb);
   const int nn = S.nnodes;
   int n = 0;
   while (n < nn) {
@@ -153,7 +161,7 @@ __device__ inline int closest_hit(const flux_scene &S, const Ray64 &r, const Ray
                                   int skip, double *t_out) {
   double best_t = INFINITY;
   int best_k = -1;
-  const float4 *__restrict__ nodes = S.nodes;
+  const float4 *__restrict__ nodes = octant_nodes(S, rb);
   const int nn = S.nnodes;
   int n = 0;
   while (n < nn) {

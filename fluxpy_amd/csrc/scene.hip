@@ -12,8 +12,14 @@
 //   5. bottom-up refit with per-node arrival counters; leaf boxes are the
 //      FP64 triangle boxes rounded outward to FP32 and padded, so the FP32
 //      slab test never culls a triangle the FP64 triangle test would hit;
-//   6. preorder layout with escape links (stackless traversal): a node's
-//      escape is its preorder position plus its subtree size.
+//   6. preorder layouts with escape links (stackless traversal): a node's
+//      escape is its preorder position plus its subtree size.  There are
+//      eight layouts, one per octant of ray directions: in the layout of
+//      octant o (bit c set: direction component c negative) the child whose
+//      box centre lies earlier along s_o = (+-1, +-1, +-1) comes first, so a
+//      ray walks its octant's layout roughly front to back and an any-hit
+//      query meets a blocker near its origin sooner.  Every layout holds
+//      the same boxes, so results do not depend on which one a ray walks.
 #include "common.h"
 
 namespace flux {
@@ -220,34 +226,56 @@ __device__ inline int subtree_size(int u, int n, const int *rlo, const int *rhi)
   return 2 * (rhi[u] - rlo[u] + 1) - 1;
 }
 
-__global__ void k_preorder(const int *__restrict__ parent, const int *__restrict__ lch,
-                           const int *__restrict__ rlo, const int *__restrict__ rhi,
-                           int n, int *__restrict__ pos) {
-  int uid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (uid >= 2 * n - 1) return;
-  int p = 0, u = uid;
-  // A Karras tree over unique 64-bit keys is at most 64 levels deep; the
-  // step bound only guarantees termination.
-  for (int steps = 0; parent[u] >= 0 && steps < 256; ++steps) {
-    int par = parent[u];
-    int l = lch[par];
-    p += (l == u) ? 1 : 1 + subtree_size(l, n, rlo, rhi);
-    u = par;
+// true if child l precedes child r in the layout of octant oct: its box
+// centre has the smaller coordinate along s_oct (ties: l first)
+__device__ inline bool child_first(const float *__restrict__ boxes, int l, int r, int oct) {
+  float kl = 0.0f, kr = 0.0f;
+  for (int c = 0; c < 3; ++c) {
+    const float sg = ((oct >> c) & 1) ? -1.0f : 1.0f;
+    kl += sg * (boxes[(int64_t)l * 6 + c] + boxes[(int64_t)l * 6 + 3 + c]);
+    kr += sg * (boxes[(int64_t)r * 6 + c] + boxes[(int64_t)r * 6 + 3 + c]);
   }
-  pos[uid] = p;
+  return kl <= kr;
+}
+
+// pos[oct * n_all + u] = preorder position of node u in the layout of octant oct
+__global__ void k_preorder(const int *__restrict__ parent, const int *__restrict__ lch,
+                           const int *__restrict__ rch, const int *__restrict__ rlo,
+                           const int *__restrict__ rhi, const float *__restrict__ boxes, int n,
+                           int *__restrict__ pos) {
+  const int uid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n_all = 2 * n - 1;
+  if (uid >= n_all) return;
+  for (int oct = 0; oct < kOctants; ++oct) {
+    int p = 0, u = uid;
+    // A Karras tree over unique 64-bit keys is at most 64 levels deep; the
+    // step bound only guarantees termination.
+    for (int steps = 0; parent[u] >= 0 && steps < 256; ++steps) {
+      const int par = parent[u];
+      const int l = lch[par], r = rch[par];
+      const int first = child_first(boxes, l, r, oct) ? l : r;
+      p += (first == u) ? 1 : 1 + subtree_size(first, n, rlo, rhi);
+      u = par;
+    }
+    pos[(int64_t)oct * n_all + uid] = p;
+  }
 }
 
 __global__ void k_emit(const uint64_t *__restrict__ keys, const float *__restrict__ boxes,
                        const int *__restrict__ pos, const int *__restrict__ rlo,
                        const int *__restrict__ rhi, int n, float4 *__restrict__ nodes) {
-  int uid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (uid >= 2 * n - 1) return;
-  int p = pos[uid];
-  int esc = p + subtree_size(uid, n, rlo, rhi);
-  int t = (uid >= n - 1) ? (int)(keys[uid - (n - 1)] & 0xffffffffull) : -1;
+  const int uid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n_all = 2 * n - 1;
+  if (uid >= n_all) return;
+  const int sz = subtree_size(uid, n, rlo, rhi);
+  const int t = (uid >= n - 1) ? (int)(keys[uid - (n - 1)] & 0xffffffffull) : -1;
   const float *b = boxes + (int64_t)uid * 6;
-  nodes[2 * p] = make_float4(b[0], b[1], b[2], __int_as_float(esc));
-  nodes[2 * p + 1] = make_float4(b[3], b[4], b[5], __int_as_float(t));
+  for (int oct = 0; oct < kOctants; ++oct) {
+    const int p = pos[(int64_t)oct * n_all + uid];
+    float4 *nd = nodes + (int64_t)oct * 2 * n_all;
+    nd[2 * p] = make_float4(b[0], b[1], b[2], __int_as_float(p + sz));
+    nd[2 * p + 1] = make_float4(b[3], b[4], b[5], __int_as_float(t));
+  }
 }
 
 static void scene_free(flux_scene *s) {
@@ -295,7 +323,7 @@ extern "C" int flux_scene_create(const void *V, int64_t nv, const int64_t *F,
   do {
     if (hipMalloc(&s->tri, nf * 9 * sizeof(double)) != hipSuccess ||
         hipMalloc(&s->cen, nf * 3 * sizeof(double)) != hipSuccess ||
-        hipMalloc(&s->nodes, (size_t)s->nnodes * 2 * sizeof(float4)) != hipSuccess) {
+        hipMalloc(&s->nodes, (size_t)kOctants * s->nnodes * 2 * sizeof(float4)) != hipSuccess) {
       set_error("flux_scene_create: out of device memory (nf=%lld)", (long long)nf);
       rc = -1;
       break;
@@ -305,7 +333,8 @@ extern "C" int flux_scene_create(const void *V, int64_t nv, const int64_t *F,
     int64_t n_int = nf - 1;
     int64_t n_all = 2 * nf - 1;
     if (hipMalloc(&tb_keys.p, P * sizeof(uint64_t)) != hipSuccess ||
-        hipMalloc(&tb_int.p, (size_t)(6 * n_all + nf + 16) * sizeof(int)) != hipSuccess ||
+        hipMalloc(&tb_int.p, (size_t)((5 + kOctants) * n_all + nf + 16) * sizeof(int)) !=
+            hipSuccess ||
         hipMalloc(&tb_box.p, (size_t)n_all * 6 * sizeof(float)) != hipSuccess ||
         hipMalloc(&tb_bounds.p, 16 * sizeof(double)) != hipSuccess) {
       set_error("flux_scene_create: out of device memory for temporaries");
@@ -319,8 +348,8 @@ extern "C" int flux_scene_create(const void *V, int64_t nv, const int64_t *F,
     int *rhi = rlo + n_all;
     int *parent = rhi + n_all;
     int *flags = parent + n_all;
-    int *pos = flags + nf;
-    int *bad = pos + n_all;
+    int *pos = flags + nf;               // kOctants * n_all
+    int *bad = pos + kOctants * n_all;
     (void)n_int;
     if (hipMemsetAsync(bad, 0, sizeof(int), st) != hipSuccess ||
         hipMemsetAsync(flags, 0, nf * sizeof(int), st) != hipSuccess) {
@@ -401,7 +430,7 @@ extern "C" int flux_scene_create(const void *V, int64_t nv, const int64_t *F,
     hipLaunchKernelGGL(k_refit, dim3(ceil_div_u(n, TB)), dim3(TB), 0, st, s->tri, keys, n,
                        s->pad, parent, lch, rch, boxes, flags);
     hipLaunchKernelGGL(k_preorder, dim3(ceil_div_u(n_all, TB)), dim3(TB), 0, st, parent,
-                       lch, rlo, rhi, n, pos);
+                       lch, rch, rlo, rhi, boxes, n, pos);
     hipLaunchKernelGGL(k_emit, dim3(ceil_div_u(n_all, TB)), dim3(TB), 0, st, keys, boxes,
                        pos, rlo, rhi, n, s->nodes);
     hipError_t e = hipGetLastError();
@@ -449,7 +478,7 @@ extern "C" int flux_scene_copy_nodes(const flux_scene *scene, float *nodes_out,
              "flux_scene_copy_nodes: NULL argument");
   if (scene->nnodes == 0) return 0;
   FLUX_TRY_HIP(hipMemcpyAsync(nodes_out, scene->nodes,
-                              (size_t)scene->nnodes * 2 * sizeof(float4),
+                              (size_t)kOctants * scene->nnodes * 2 * sizeof(float4),
                               hipMemcpyDeviceToDevice, as_stream(stream)));
   return 0;
 }

@@ -82,9 +82,12 @@ int flux_scene_num_nodes(const flux_scene *scene, int64_t *nnodes);
 /* Device pointer to the scene's FP64 centroids (nf*3, ((v0+v1)+v2)/3). */
 int flux_scene_centroids(const flux_scene *scene, const double **cen);
 
-/* Copy the preorder node array to `nodes_out` (device, 8 floats per node:
- * lo.xyz, escape, hi.xyz, triangle-or--1; the last float of each half is an
- * int32 bit pattern).  Used by the structural tests. */
+/* Copy the 8 preorder node arrays to `nodes_out` (device, 8 * nnodes nodes
+ * of 8 floats: lo.xyz, escape, hi.xyz, triangle-or--1; the last float of
+ * each half is an int32 bit pattern).  Array o is the layout for ray
+ * directions in octant o (bit c set: component c negative), whose children
+ * come in order of their box centres along (+-1, +-1, +-1).  Used by the
+ * structural tests. */
 int flux_scene_copy_nodes(const flux_scene *scene, float *nodes_out,
                           void *stream);
 

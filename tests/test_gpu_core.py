@@ -57,25 +57,41 @@ def test_scene_structure(torch_dev, crater):
     _lib.call("flux_scene_num_nodes", sm._scene.handle, _lib.ctypes.byref(nn))
     nn = nn.value
     assert nn == 2 * sm.num_faces - 1
-    buf = torch_dev.empty(nn * 8, dtype=torch_dev.float32, device=_lib_device())
+    buf = torch_dev.empty(8 * nn * 8, dtype=torch_dev.float32, device=_lib_device())
     _lib.call("flux_scene_copy_nodes", sm._scene.handle, buf.data_ptr(), _lib.stream())
-    nodes = buf.cpu().numpy().reshape(nn, 8)
-    esc = nodes[:, 3].copy().view(np.int32)
-    tri = nodes[:, 7].copy().view(np.int32)
-    leaves = tri[tri >= 0]
-    assert sorted(leaves.tolist()) == list(range(sm.num_faces))
-    assert (esc[tri >= 0] == np.arange(nn)[tri >= 0] + 1).all()
-    assert (esc > np.arange(nn)).all() and (esc <= nn).all() and esc[0] == nn
+    layouts = buf.cpu().numpy().reshape(8, nn, 8)
     V = sm.V[sm.F]
-    lo, hi = nodes[:, :3], nodes[:, 4:7]
-    for k in np.where(tri >= 0)[0]:
-        t = tri[k]
-        assert (lo[k] <= V[t].min(0)).all() and (hi[k] >= V[t].max(0)).all()
-    # internal boxes contain their two children (preorder: left child = k+1)
-    for k in np.where(tri < 0)[0]:
-        l, r = k + 1, esc[k + 1]
-        for c in (l, r):
-            assert (lo[k] <= lo[c]).all() and (hi[k] >= hi[c]).all()
+    boxes0 = None
+    for oct in range(8):
+        nodes = layouts[oct]
+        esc = nodes[:, 3].copy().view(np.int32)
+        tri = nodes[:, 7].copy().view(np.int32)
+        leaves = tri[tri >= 0]
+        assert sorted(leaves.tolist()) == list(range(sm.num_faces))
+        assert (esc[tri >= 0] == np.arange(nn)[tri >= 0] + 1).all()
+        assert (esc > np.arange(nn)).all() and (esc <= nn).all() and esc[0] == nn
+        lo, hi = nodes[:, :3], nodes[:, 4:7]
+        for k in np.where(tri >= 0)[0]:
+            t = tri[k]
+            assert (lo[k] <= V[t].min(0)).all() and (hi[k] >= V[t].max(0)).all()
+        # internal boxes contain their two children (preorder: first child =
+        # k+1, second = its escape), and the first child's box centre is not
+        # later along this octant's diagonal
+        sg = np.array([-1.0 if (oct >> c) & 1 else 1.0 for c in range(3)], dtype=np.float32)
+        for k in np.where(tri < 0)[0]:
+            l, r = k + 1, esc[k + 1]
+            for c in (l, r):
+                assert (lo[k] <= lo[c]).all() and (hi[k] >= hi[c]).all()
+            kl = np.float32(0)
+            kr = np.float32(0)
+            for c in range(3):
+                kl = np.float32(kl + sg[c] * (lo[l, c] + hi[l, c]))
+                kr = np.float32(kr + sg[c] * (lo[r, c] + hi[r, c]))
+            assert kl <= kr
+        # every layout holds the same set of boxes
+        key = sorted(zip(map(tuple, lo.tolist()), map(tuple, hi.tolist()), tri.tolist()))
+        boxes0 = key if boxes0 is None else boxes0
+        assert key == boxes0
 
 
 def test_scene_single_facet_and_empty_queries(torch_dev):
