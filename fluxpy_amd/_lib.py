@@ -95,12 +95,14 @@ SIGNATURES = {
                                     _vp]),
     "flux_panel8_layout": (_c_int, [_c_i64, _vp, _vp, _vp, _vp, _vp]),
     "flux_panel8_fill": (_c_int, [_c_i64, _c_i64, _vp, _vp, _vp, _c_int, _vp, _c_i64, _vp, _vp,
-                                  _vp, _vp, _vp, _vp]),
+                                  _vp, _vp, _vp]),
+    "flux_panel8_lanemask": (_c_int, [_c_i64, _vp, _vp, _vp, _vp, _vp]),
+    "flux_panel8_pack": (_c_int, [_c_i64, _vp, _vp, _vp, _vp, _vp]),
     "flux_slice_x_bytes": (_c_i64, [_c_i64, _c_i64]),
     "flux_slice_x": (_c_int, [_c_i64, _c_i64, _vp, _vp, _c_dbl, _c_int, _vp, _vp]),
     "flux_panel8_mm": (_c_int, [_c_i64, _vp, _vp, _vp, _vp, _vp, _c_i64, _vp, _vp, _vp, _vp,
                                 _c_dbl, _c_int, _vp, _vp, _vp, _c_i64, _vp, _vp, _vp, _c_i64,
-                                _vp, _c_int, _vp, _vp]),
+                                _vp, _c_int, _vp, _vp, _vp]),
     "flux_vec_reduce":(_c_int, [_c_i64, _c_i64, _vp, _vp, _c_int, _c_int, _vp, _vp, _vp]),
     "flux_vec_reduce_noalloc": (_c_int, [_c_i64, _c_i64, _vp, _vp, _c_int, _c_int, _vp, _vp]),
     "flux_vec_axpby": (_c_int, [_c_i64, _c_i64, _vp, _vp, _vp, _vp, _c_int, _vp]),

@@ -139,15 +139,19 @@ class PanelMatrix:
         """out = base + s (P Xb) with the Jacobi epilogue (see panel.hip)."""
         k = Xb.shape[1]
         plan = self.plan.args("bf16", k, 4) if self.plan else _NO_PLAN
-        _lib.call("flux_panel_mm_bf16", self.shape[0], self.colptr.data_ptr(),
-                  self.cols.data_ptr(), self.panel.data_ptr(), Xb.data_ptr(), k, _lib.ptr(base),
-                  _lib.ptr(rho), float(rho_s), int(left), out.data_ptr(), _lib.ptr(xnext),
-                  _lib.ptr(prev), _lib.ptr(colmax), *plan, int(self.stage), _lib.stream())
+        with _launch_timer(self.timing, "bf16"):
+            _lib.call("flux_panel_mm_bf16", self.shape[0], self.colptr.data_ptr(),
+                      self.cols.data_ptr(), self.panel.data_ptr(), Xb.data_ptr(), k,
+                      _lib.ptr(base), _lib.ptr(rho), float(rho_s), int(left), out.data_ptr(),
+                      _lib.ptr(xnext), _lib.ptr(prev), _lib.ptr(colmax), *plan,
+                      int(self.stage), _lib.stream())
         return out
 
     # 16-column chunks per pipeline stage of the BF16 product (0: library
     # default); a per-matrix tuning choice that leaves the result unchanged
     stage = 0
+    # a list to receive (kind, start event, end event) per product, or None
+    timing = None
 
     def replan(self, cap):
         """A new work plan with pieces of at most `cap` quads (shared with
@@ -165,16 +169,44 @@ class PanelMatrix:
         return out
 
 
+class _launch_timer:
+    """Records HIP events on the current stream around a launch when `rec`
+    is a list (appending (kind, start, end)); does nothing for None."""
+
+    def __init__(self, rec, kind):
+        self.rec, self.kind = rec, kind
+
+    def __enter__(self):
+        if self.rec is not None:
+            t = _lib.torch()
+            self.e0 = t.cuda.Event(enable_timing=True)
+            self.e0.record()
+        return self
+
+    def __exit__(self, *exc):
+        if self.rec is not None and exc[0] is None:
+            t = _lib.torch()
+            e1 = t.cuda.Event(enable_timing=True)
+            e1.record()
+            self.rec.append((self.kind, self.e0, e1))
+        return False
+
+
 class SlicedPanelMatrix:
     """Int8-sliced panels of a square CSR: every value split into 5 signed
     7-bit slices under a power-of-two scale per row, for FP64-accurate
     products on int8 MFMA (see csrc/panel8.hip for the error bound).
-    Shares the column lists of ``like``, a PanelMatrix of the same CSR."""
+    Shares the column lists of ``like``, a PanelMatrix of the same CSR.
+    The panels are kept packed: of every 16 x 64 subtile only the 16-byte
+    lane chunks (one row, 16 columns) that hold a nonzero, with a 64-bit
+    lane mask and a base offset per subtile.  ``keep_dense`` also keeps the
+    dense 64 x 64 quads they were packed from (for tests)."""
 
     ROWS = 64
     QUAD_BYTES = 5 * 64 * 64
+    CHUNK_BYTES = 5 * 16        # one lane chunk: 16 bytes of each of 5 slices
 
-    def __init__(self, csr, like):
+    def __init__(self, csr, like, keep_dense=False):
         t = _lib.require_gpu()
         m, n = csr.shape
         self.shape = (m, n)
@@ -188,22 +220,40 @@ class SlicedPanelMatrix:
         _lib.call("flux_panel8_layout", nblk, self.colptr.data_ptr(), self.qptr.data_ptr(),
                   ctypes.byref(total), ctypes.byref(widest), st)
         self.quads = total.value
-        self.panel = t.empty(max(self.quads * self.QUAD_BYTES, 16), dtype=t.int8, device=dev)
+        dense = t.empty(max(self.quads * self.QUAD_BYTES, 16), dtype=t.int8, device=dev)
         self.rowexp = t.empty(max(m, 1), dtype=t.int32, device=dev)
-        # zero-subtile flags, one byte per (quad, 16-row group)
-        self.qmask = t.empty(max(self.quads, 1) * 4, dtype=t.uint8, device=dev)
         _lib.call("flux_panel8_fill", m, n, csr.indptr.data_ptr(), csr.indices.data_ptr(),
                   csr.data.data_ptr(), _lib.dtype_code(csr.data.dtype), self.qptr.data_ptr(),
-                  self.quads, self.rowexp.data_ptr(), self.panel.data_ptr(),
-                  self.qmask.data_ptr(), self.colptr.data_ptr(), _lib.ptr(self.order), st)
+                  self.quads, self.rowexp.data_ptr(), dense.data_ptr(),
+                  self.colptr.data_ptr(), _lib.ptr(self.order), st)
+        # packed form: only the nonzero 16-byte lane chunks of every 16 x 64
+        # subtile (flux_panel8_pack); the dense quads are a build intermediate
+        nsub = 4 * self.quads
+        self.lmask = t.empty(max(nsub, 1), dtype=t.int64, device=dev)   # uint64 bits
+        self.sbase = t.zeros(nsub + 1, dtype=t.int64, device=dev)
+        chunks = ctypes.c_int64()
+        _lib.call("flux_panel8_lanemask", self.quads, dense.data_ptr(), self.lmask.data_ptr(),
+                  self.sbase.data_ptr(), ctypes.byref(chunks), st)
+        self.chunks = chunks.value
+        self.panel = t.empty(max(self.chunks * self.CHUNK_BYTES, 16), dtype=t.int8, device=dev)
+        _lib.call("flux_panel8_pack", self.quads, dense.data_ptr(), self.lmask.data_ptr(),
+                  self.sbase.data_ptr(), self.panel.data_ptr(), st)
+        self.dense = dense if keep_dense else None
         self.nnz = csr.nnz
         self.plan = like.plan
         self._work = {}
-        self.skip_zero = True       # products skip the zero subtiles' reads
+        self.packed = True          # products read the packed panels (False: self.dense)
+
+    @property
+    def nonzero_subtiles(self):
+        """Number of 16 x 64 subtiles holding a nonzero (the products skip
+        the MFMAs of the others)."""
+        return int((self.lmask[:4 * self.quads] != 0).sum().item())
 
     @property
     def nbytes(self):
-        return (self.panel.numel() + self.rowexp.numel() * 4 + self.qptr.numel() * 8
+        return (self.panel.numel() + self.lmask.numel() * 8 + self.sbase.numel() * 8
+                + self.rowexp.numel() * 4 + self.qptr.numel() * 8
                 + self.cols.numel() * 4 + self.colptr.numel() * 8)
 
     def workspace(self, k, slot=0):
@@ -251,12 +301,22 @@ class SlicedPanelMatrix:
         work = self.workspace(k, slot)
         st = _lib.stream()
         plan = self.plan.args("int8", k, 8) if self.plan else _NO_PLAN
-        _lib.call("flux_panel8_mm", self.shape[0], self.colptr.data_ptr(), self.cols.data_ptr(),
-                  self.qptr.data_ptr(), self.panel.data_ptr(), self.rowexp.data_ptr(), k,
-                  work.data_ptr(), _lib.ptr(E), _lib.ptr(B), _lib.ptr(rho), float(rho_s),
-                  int(left), out.data_ptr(), _lib.ptr(colmax), *plan, int(nslices),
-                  self.qmask.data_ptr() if self.skip_zero else None, st)
+        if self.packed:
+            panel, lmask, sbase = self.panel, self.lmask.data_ptr(), self.sbase.data_ptr()
+        else:
+            if self.dense is None:
+                raise ValueError("dense panels were not kept (keep_dense=False)")
+            panel, lmask, sbase = self.dense, None, None
+        with _launch_timer(self.timing, f"int8x{nslices}"):
+            _lib.call("flux_panel8_mm", self.shape[0], self.colptr.data_ptr(),
+                      self.cols.data_ptr(), self.qptr.data_ptr(), panel.data_ptr(),
+                      self.rowexp.data_ptr(), k, work.data_ptr(), _lib.ptr(E), _lib.ptr(B),
+                      _lib.ptr(rho), float(rho_s), int(left), out.data_ptr(), _lib.ptr(colmax),
+                      *plan, int(nslices), lmask, sbase, st)
         return out
+
+    # a list to receive (kind, start event, end event) per product, or None
+    timing = None
 
 
 def to_bf16_scaled(x32, rho=None, rho_s=1.0, scale=True, out=None):

@@ -63,6 +63,14 @@ __device__ __forceinline__ v2i lds_read_tr8(const uint8_t *p) {
 }
 constexpr int64_t kMaxBlockCols = 104832;             // int32 bound of the sums S_d
 
+// A value the whole wave holds (read from LDS at a wave-uniform address),
+// moved to scalar registers so that branches on it are scalar.
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
 // a_0..a_4 of v with |v| < 2^e (see the header).
 __device__ __forceinline__ void slice5(double v, int e, int8_t a[kSlices]) {
   double t = ldexp(v, 6 - e);
@@ -171,30 +179,60 @@ __global__ void __launch_bounds__(256) k_panel8_fill(int64_t nrows, int64_t ncol
   }
 }
 
-// Zero subtiles: byte g of qmask[q] is 1 if rows 16 g .. 16 g + 15 of quad q
-// hold a nonzero slice byte, else 0.  One wave per (quad, row group).
-__global__ void __launch_bounds__(256) k_panel8_qmask(int64_t total_quads,
-                                                      const int8_t *__restrict__ panel8,
-                                                      uint8_t *__restrict__ qmask) {
-  const int64_t wv = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+// Lane chunks.  Lane l of 16-row group g of quad q reads rows 16 g + (l & 15),
+// columns 16 (l >> 4) .. + 15 of the quad: the 16-byte B operand of one int8
+// MFMA in k_panel8_mm.  Subtile s = 4 q + g.
+__device__ __forceinline__ const v4i *lane_chunk(const int8_t *panel8, int64_t s, int lane) {
+  return reinterpret_cast<const v4i *>(panel8 + (s >> 2) * kQuadBytes +
+                                       (16 * (s & 3) + (lane & 15)) * kQuadCols +
+                                       16 * (lane >> 4));
+}
+
+// lmask[s] has bit l set when lane chunk l of subtile s holds a nonzero slice
+// byte; count[s] is its number of set bits.  One wave per subtile.
+__global__ void __launch_bounds__(256) k_panel8_lanemask(int64_t nsub,
+                                                         const int8_t *__restrict__ panel8,
+                                                         uint64_t *__restrict__ lmask,
+                                                         int64_t *__restrict__ count) {
+  const int64_t s = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
-  if (wv >= total_quads * 4) return;
-  const int64_t q = wv >> 2;
-  const int g = (int)(wv & 3);
-  const v4i *p = reinterpret_cast<const v4i *>(panel8 + q * kQuadBytes +
-                                               (16 * g + (lane >> 2)) * kQuadCols +
-                                               16 * (lane & 3));
+  if (s >= nsub) return;  // uniform per wave
+  const v4i *p = lane_chunk(panel8, s, lane);
   int any = 0;
   for (int i = 0; i < kSlices; ++i) {
     const v4i v = p[i * (kSliceTile / 16)];
     any |= v[0] | v[1] | v[2] | v[3];
   }
-  const bool nz = __any(any != 0);
-  if (lane == 0) qmask[4 * q + g] = nz ? 1 : 0;
+  const uint64_t m = __ballot(any != 0);
+  if (lane == 0) {
+    lmask[s] = m;
+    count[s] = __popcll(m);
+  }
 }
 
-// A quad of zero slices: the panel loads of a zero subtile read it instead
-// (an L2-resident line) and the subtile's bytes are not fetched from HBM.
+// Packed panels: subtile s keeps only its nonzero lane chunks, slice-major:
+// slice i of the chunk of lane l at 16-byte unit
+//   5 sbase[s] + i popcount(lmask[s]) + popcount(lmask[s] & ((1 << l) - 1)),
+// sbase = exclusive prefix sum of the popcounts.  One wave per subtile.
+__global__ void __launch_bounds__(256) k_panel8_pack(int64_t nsub,
+                                                     const int8_t *__restrict__ panel8,
+                                                     const uint64_t *__restrict__ lmask,
+                                                     const int64_t *__restrict__ sbase,
+                                                     int8_t *__restrict__ packed) {
+  const int64_t s = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (s >= nsub) return;  // uniform per wave
+  const uint64_t m = lmask[s];
+  if (!((m >> lane) & 1ull)) return;
+  const int cnt = __popcll(m);
+  const int off = __popcll(m & ((1ull << lane) - 1ull));
+  const v4i *p = lane_chunk(panel8, s, lane);
+  v4i *d = reinterpret_cast<v4i *>(packed) + sbase[s] * kSlices + off;
+  for (int i = 0; i < kSlices; ++i) d[i * cnt] = p[i * (kSliceTile / 16)];
+}
+
+// A quad of zero slices: the panel loads of a zero lane chunk read it instead
+// (an L2-resident line), so packed panels need not store zero chunks.
 __device__ __align__(256) int8_t g_zero_quad[kQuadBytes];
 
 __global__ void k_zero_u32(uint32_t *p, int64_t n) {
@@ -386,7 +424,11 @@ __global__ void __launch_bounds__(256) k_colmax_blocks(const double *__restrict_
 // column (the truncation error grows to about 2^(e_r + e_k - 7 S) per term),
 // which the refinement uses for a residual update whose X is small.  Each
 // thread stages S 16-byte units of X per stage.
-template <int S>
+// PACKED: panel8 holds the packed panels of k_panel8_pack (lmask, sbase
+// describe them); a lane whose chunk is zero reads the zero quad, and a wave
+// whose whole subtile is zero skips its MFMAs.  Otherwise panel8 holds the
+// dense quads and every subtile is read.
+template <int S, bool PACKED>
 __global__ void __launch_bounds__(256, 2) k_panel8_mm(
     int64_t nrows, int64_t nblk, int nh, const int64_t *__restrict__ colptr,
     const int32_t *__restrict__ cols, const int64_t *__restrict__ qptr,
@@ -396,10 +438,12 @@ __global__ void __launch_bounds__(256, 2) k_panel8_mm(
     double rho_s, int left, double *__restrict__ out, double *__restrict__ bcm,
     const int32_t *__restrict__ items, const int32_t *__restrict__ nsplit,
     const int32_t *__restrict__ slot0, double *__restrict__ ws, int64_t nitems,
-    const uint32_t *__restrict__ qmask) {
+    const uint64_t *__restrict__ lmask, const int64_t *__restrict__ sbase) {
   __shared__ __align__(16) uint8_t xs[2][S * kXsSlice];
   __shared__ __align__(16) int32_t ids[4][kQuadCols];
-  __shared__ uint32_t qms[4];
+  // per stage and wave: the subtile's lane mask and packed base (PACKED)
+  __shared__ uint64_t lms[4][4];
+  __shared__ int64_t sbs[4][4];
   const int64_t pair = xcd_block(blockIdx.x, nitems * nh);
   const int64_t it = pair / nh;
   const int h = (int)(pair - it * nh);
@@ -415,6 +459,8 @@ __global__ void __launch_bounds__(256, 2) k_panel8_mm(
   const v4i *pb = reinterpret_cast<const v4i *>(panel8 + qbase * kQuadBytes +
                                                 (16 * w + li) * kQuadCols + 16 * lg);
   const v4i *pz = reinterpret_cast<const v4i *>(g_zero_quad + (16 * w + li) * kQuadCols + 16 * lg);
+  const v4i *pk = reinterpret_cast<const v4i *>(panel8);
+  const uint64_t below = (1ull << lane) - 1ull;
   const int8_t *xg = x8 + h * (int64_t)kXRow;
   const int64_t xstride = (int64_t)nh * kXRow;
   v4i acc[S][4];
@@ -425,9 +471,11 @@ __global__ void __launch_bounds__(256, 2) k_panel8_mm(
     auto load_id = [&](int64_t s) -> int32_t {
       return cols[c0 + min(s * kQuadCols + lane, ncl - 1)];
     };
-    // the quad's zero-subtile flags (all ones without qmask)
-    auto load_qm = [&](int64_t s) -> uint32_t {
-      return qmask ? qmask[qbase + min(s, nq - 1)] : 0xffffffffu;
+    // this wave's subtile of stage s: lane mask and packed base
+    auto load_sub = [&](int64_t s, uint64_t &m, int64_t &bs) {
+      const int64_t sub = (qbase + min(s, nq - 1)) * 4 + w;
+      m = lmask[sub];
+      bs = sbase[sub];
     };
     auto load_x = [&](int64_t s, v4i (&xr)[S]) {
       const int32_t *id = ids[s & 3];
@@ -446,11 +494,25 @@ __global__ void __launch_bounds__(256, 2) k_panel8_mm(
       }
     };
     auto load_p = [&](int64_t s, v4i (&pf)[S]) {
-      // a zero subtile of this wave reads the zero quad (no branch)
-      const bool nz = (qms[s & 3] >> (8 * w)) & 0xffu;
-      const v4i *p = nz ? pb + min(s, nq - 1) * (kQuadBytes / 16) : pz;
+      const v4i *p;
+      int64_t stride;
+      if (PACKED) {
+        // a lane whose chunk is zero reads the next nonzero lane's chunk (a
+        // line the wave fetches anyway; the value is zeroed before its
+        // MFMAs); a zero subtile reads the zero quad
+        const uint64_t m = uniform_u64(lms[s & 3][w]);
+        const int64_t bs = (int64_t)uniform_u64((uint64_t)sbs[s & 3][w]);
+        const int cnt = (int)__popcll(m);
+        const int rank = (int)__popcll(m & below);
+        const int off = rank < cnt - 1 ? rank : cnt - 1;
+        p = cnt > 0 ? pk + bs * kSlices + off : pz;
+        stride = cnt > 0 ? (int64_t)cnt : kSliceTile / 16;
+      } else {
+        p = pb + min(s, nq - 1) * (kQuadBytes / 16);
+        stride = kSliceTile / 16;
+      }
       // streaming loads: the panels are read once, the X slices many times
-      for (int i = 0; i < S; ++i) pf[i] = __builtin_nontemporal_load(p + i * (kSliceTile / 16));
+      for (int i = 0; i < S; ++i) pf[i] = __builtin_nontemporal_load(p + i * stride);
     };
     v4i xr[S];
     v4i p0[S], p1[S];
@@ -459,7 +521,16 @@ __global__ void __launch_bounds__(256, 2) k_panel8_mm(
     ids[0][lane] = load_id(0);
     ids[1][lane] = load_id(1);
     ids[2][lane] = load_id(2);
-    if (tid < 3) qms[tid] = load_qm(tid);
+    if (PACKED)
+      for (int j = 0; j < 3; ++j) {
+        uint64_t m;
+        int64_t bs;
+        load_sub(j, m, bs);
+        if (lane == 0) {
+          lms[j][w] = m;
+          sbs[j][w] = bs;
+        }
+      }
     __syncthreads();
     load_x(0, xr);
     load_p(0, p0);
@@ -469,7 +540,9 @@ __global__ void __launch_bounds__(256, 2) k_panel8_mm(
       const int buf = (int)(s & 1);
       load_x(s + 1, xr);
       const int32_t idn = load_id(s + 3);
-      const uint32_t qmn = load_qm(s + 3);
+      uint64_t mn = 0;
+      int64_t bn = 0;
+      if (PACKED) load_sub(s + 3, mn, bn);
       load_p(s + 1, pl);
       // keep every load of the stage ahead of its MFMAs
       __builtin_amdgcn_sched_barrier(0);
@@ -478,20 +551,39 @@ __global__ void __launch_bounds__(256, 2) k_panel8_mm(
       const uint8_t *xq = xs[buf] + (16 * lg + (li >> 1)) * kXsRow + 8 * (li & 1);
       // a zero subtile of this wave contributes nothing: its MFMAs are skipped
       // (a wave-uniform branch with no load inside)
-      const bool nzs = (qms[s & 3] >> (8 * w)) & 0xffu;
-      if (nzs)
-      for (int t = 0; t < 4; ++t) {
+      const uint64_t msk = PACKED ? uniform_u64(lms[s & 3][w]) : ~0ull;
+      const bool nzs = msk != 0ull;
+      // a zero lane chunk holds another lane's bytes: zero them
+      v4i pz4[S];
+      for (int i = 0; i < S; ++i)
+        pz4[i] = (!PACKED || ((msk >> lane) & 1ull)) ? pc[i] : v4i{0, 0, 0, 0};
+      if (nzs) {
+        // X slice j of all four tiles is read one slice ahead of its
+        // 4 (S - j) MFMAs (independent accumulators), so the transposing
+        // LDS reads overlap the matrix work
+        auto read_a = [&](int j, v4i (&dst)[4]) {
+          for (int t = 0; t < 4; ++t) {
+            const uint8_t *pa = xq + j * kXsSlice + 16 * t;
+            const v2i a0 = lds_read_tr8(pa), a1 = lds_read_tr8(pa + 8 * kXsRow);
+            dst[t] = v4i{a0[0], a0[1], a1[0], a1[1]};
+          }
+        };
+        v4i ab[2][4];
+        read_a(0, ab[0]);
         for (int j = 0; j < S; ++j) {
-          const uint8_t *pa = xq + j * kXsSlice + 16 * t;
-          const v2i a0 = lds_read_tr8(pa), a1 = lds_read_tr8(pa + 8 * kXsRow);
-          const v4i a = v4i{a0[0], a0[1], a1[0], a1[1]};
+          if (j + 1 < S) read_a(j + 1, ab[(j + 1) & 1]);
           for (int i = 0; i + j < S; ++i)
-            acc[i + j][t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, pc[i], acc[i + j][t], 0, 0, 0);
+            for (int t = 0; t < 4; ++t)
+              acc[i + j][t] =
+                  __builtin_amdgcn_mfma_i32_16x16x64_i8(ab[j & 1][t], pz4[i], acc[i + j][t], 0, 0, 0);
         }
       }
       store_x(buf ^ 1, xr);
       ids[(s + 3) & 3][lane] = idn;
-      if (tid == 0) qms[(s + 3) & 3] = qmn;
+      if (PACKED && lane == 0) {
+        lms[(s + 3) & 3][w] = mn;
+        sbs[(s + 3) & 3][w] = bn;
+      }
       __syncthreads();
     };
     int64_t s = 0;
@@ -572,6 +664,9 @@ __global__ void __launch_bounds__(256) k_panel8_reduce(
 
 using namespace flux;
 
+extern "C" int flux_scan_i64(const int64_t *in, int64_t n, int64_t *out, int64_t *total_host,
+                             void *stream);
+
 extern "C" int flux_panel8_layout(int64_t nblk, const int64_t *colptr, int64_t *qptr,
                                   int64_t *total_quads, int64_t *max_cols, void *stream) {
   FLUX_CHECK(nblk >= 0, "flux_panel8_layout: negative block count");
@@ -601,8 +696,8 @@ extern "C" int flux_panel8_layout(int64_t nblk, const int64_t *colptr, int64_t *
 extern "C" int flux_panel8_fill(int64_t nrows, int64_t ncols, const int64_t *indptr,
                                 const int32_t *indices, const void *data, int dtype,
                                 const int64_t *qptr, int64_t total_quads, int32_t *rowexp,
-                                int8_t *panel8, uint8_t *qmask, const int64_t *colptr,
-                                const int32_t *order, void *stream) {
+                                int8_t *panel8, const int64_t *colptr, const int32_t *order,
+                                void *stream) {
   FLUX_CHECK(nrows >= 0 && ncols >= 0 && total_quads >= 0, "flux_panel8_fill: bad sizes");
   FLUX_CHECK(2 * panel_lds_words(ncols) * 4 <= 150 * 1024,
              "flux_panel8_fill: %lld columns exceed the LDS bitmap limit", (long long)ncols);
@@ -631,9 +726,40 @@ extern "C" int flux_panel8_fill(int64_t nrows, int64_t ncols, const int64_t *ind
                        ncols, indptr, indices, (const double *)data, qptr, colptr, order, rowexp,
                        panel8);
   }
-  if (qmask && total_quads > 0)
-    hipLaunchKernelGGL(k_panel8_qmask, dim3(ceil_div_u(total_quads * 4, 4)), dim3(256), 0, st,
-                       total_quads, panel8, qmask);
+  FLUX_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int flux_panel8_lanemask(int64_t total_quads, const int8_t *panel8, uint64_t *lmask,
+                                    int64_t *sbase, int64_t *total_chunks, void *stream) {
+  FLUX_CHECK(total_quads >= 0 && total_chunks, "flux_panel8_lanemask: bad arguments");
+  FLUX_CHECK(total_quads == 0 || (panel8 && lmask && sbase), "flux_panel8_lanemask: NULL array");
+  *total_chunks = 0;
+  if (total_quads == 0) return 0;
+  hipStream_t st = as_stream(stream);
+  const int64_t nsub = total_quads * 4;
+  int64_t *count = nullptr;
+  FLUX_TRY_HIP(hipMallocAsync((void **)&count, nsub * sizeof(int64_t), st));
+  hipLaunchKernelGGL(k_panel8_lanemask, dim3(ceil_div_u(nsub, 4)), dim3(256), 0, st, nsub, panel8,
+                     lmask, count);
+  hipError_t e1 = hipGetLastError();
+  const int r = e1 == hipSuccess ? flux_scan_i64(count, nsub, sbase, total_chunks, st) : -1;
+  hipError_t e2 = hipFreeAsync(count, st);
+  FLUX_TRY_HIP(e1);
+  if (r != 0) return r;
+  FLUX_TRY_HIP(e2);
+  return 0;
+}
+
+extern "C" int flux_panel8_pack(int64_t total_quads, const int8_t *panel8, const uint64_t *lmask,
+                                const int64_t *sbase, int8_t *packed, void *stream) {
+  FLUX_CHECK(total_quads >= 0, "flux_panel8_pack: bad sizes");
+  if (total_quads == 0) return 0;
+  FLUX_CHECK(panel8 && lmask && sbase && packed, "flux_panel8_pack: NULL array");
+  FLUX_CHECK(((uintptr_t)packed & 15) == 0, "flux_panel8_pack: packed must be 16-byte aligned");
+  const int64_t nsub = total_quads * 4;
+  hipLaunchKernelGGL(k_panel8_pack, dim3(ceil_div_u(nsub, 4)), dim3(256), 0, as_stream(stream),
+                     nsub, panel8, lmask, sbase, packed);
   FLUX_LAUNCH_CHECK();
   return 0;
 }
@@ -683,14 +809,17 @@ extern "C" int flux_panel8_mm(int64_t nrows, const int64_t *colptr, const int32_
                               double *colmax, const int32_t *items, int64_t n_items,
                               const int32_t *nsplit, const int32_t *slot0,
                               const int32_t *split_blocks, int64_t n_split, double *ws,
-                              int nslices, const uint8_t *qmask, void *stream) {
+                              int nslices, const uint64_t *lmask, const int64_t *sbase,
+                              void *stream) {
   FLUX_CHECK(nrows >= 0 && nrhs >= 0, "flux_panel8_mm: bad sizes");
   FLUX_CHECK(nslices >= 3 && nslices <= kSlices, "flux_panel8_mm: nslices must be 3, 4 or 5");
   const int64_t nblk = (nrows + kPanelRows - 1) / kPanelRows;
   if (nblk == 0 || nrhs == 0) return 0;
   FLUX_CHECK(colptr && cols && qptr && panel8 && rowexp && work && out,
              "flux_panel8_mm: NULL array");
-  FLUX_CHECK(((uintptr_t)qmask & 3) == 0, "flux_panel8_mm: qmask must be 4-byte aligned");
+  FLUX_CHECK((lmask == nullptr) == (sbase == nullptr),
+             "flux_panel8_mm: lmask and sbase come together");
+  FLUX_CHECK(((uintptr_t)panel8 & 15) == 0, "flux_panel8_mm: panel8 must be 16-byte aligned");
   FLUX_CHECK(E == nullptr || B != nullptr, "flux_panel8_mm: E needs B");
   FLUX_CHECK(items == nullptr ||
                  (n_items >= nblk && nsplit && slot0 && (n_split == 0 || (split_blocks && ws))),
@@ -705,11 +834,15 @@ extern "C" int flux_panel8_mm(int64_t nrows, const int64_t *colptr, const int32_
                               : nullptr;
   FLUX_CHECK(nit * nh < (1ll << 31) && n_split * nh * 4 < (1ll << 31),
              "flux_panel8_mm: too many workgroups");
-  auto kernel = nslices == 5 ? k_panel8_mm<5> : nslices == 4 ? k_panel8_mm<4> : k_panel8_mm<3>;
+  auto kernel = lmask ? (nslices == 5   ? k_panel8_mm<5, true>
+                         : nslices == 4 ? k_panel8_mm<4, true>
+                                        : k_panel8_mm<3, true>)
+                      : (nslices == 5   ? k_panel8_mm<5, false>
+                         : nslices == 4 ? k_panel8_mm<4, false>
+                                        : k_panel8_mm<3, false>);
   hipLaunchKernelGGL(kernel, dim3((unsigned)(nit * nh)), dim3(256), 0, st, nrows, nblk,
                      (int)nh, colptr, cols, qptr, panel8, rowexp, x8, xexp, nrhs, E, B, rho,
-                     rho_s, left, out, bcm, items, nsplit, slot0, ws, nit,
-                     reinterpret_cast<const uint32_t *>(qmask));
+                     rho_s, left, out, bcm, items, nsplit, slot0, ws, nit, lmask, sbase);
   if (items && n_split > 0)
     hipLaunchKernelGGL(k_panel8_reduce, dim3((unsigned)(n_split * nh * 4)), dim3(256), 0, st,
                        split_blocks, nsplit, slot0, ws, (int)nh, nrows, rowexp, xexp, nrhs, E, B,

@@ -469,16 +469,30 @@ int flux_scale_to_bf16(int64_t n, int64_t nrhs, const float *x,
 int flux_panel8_layout(int64_t nblk, const int64_t *colptr, int64_t *qptr,
                        int64_t *total_quads, int64_t *max_cols, void *stream);
 
-/* rowexp[r] (nrows int32) and the sliced panels (total_quads * 20480
- * bytes) from the CSR values (F32 or F64).  qmask (optional, total_quads *
- * 4 bytes, 4-byte aligned): byte 4 q + g is 1 if rows 16 g .. 16 g + 15 of
- * quad q hold a nonzero, else 0; flux_panel8_mm skips the HBM reads of the
- * zero subtiles it marks. */
+/* rowexp[r] (nrows int32) and the dense sliced panels (total_quads * 20480
+ * bytes) from the CSR values (F32 or F64). */
 int flux_panel8_fill(int64_t nrows, int64_t ncols, const int64_t *indptr,
                      const int32_t *indices, const void *data, int dtype,
                      const int64_t *qptr, int64_t total_quads, int32_t *rowexp,
-                     int8_t *panel8, uint8_t *qmask, const int64_t *colptr,
-                     const int32_t *order, void *stream);
+                     int8_t *panel8, const int64_t *colptr, const int32_t *order,
+                     void *stream);
+
+/* Lane chunks of the dense panels: subtile s = 4 q + g is rows 16 g ..
+ * 16 g + 15 of quad q; its lane chunk l is rows 16 g + (l & 15), columns
+ * 16 (l >> 4) .. + 15 (16 bytes per slice, one int8 MFMA operand).
+ * lmask[s] (total_quads * 4 uint64) has bit l set when chunk l holds a
+ * nonzero slice byte; sbase (total_quads * 4 + 1) is the exclusive prefix
+ * sum of the popcounts, and *total_chunks its total on the host
+ * (synchronises). */
+int flux_panel8_lanemask(int64_t total_quads, const int8_t *panel8, uint64_t *lmask,
+                         int64_t *sbase, int64_t *total_chunks, void *stream);
+
+/* Packed panels (total_chunks * 80 bytes, 16-byte aligned): the nonzero
+ * lane chunks of subtile s, slice-major; slice i of chunk l at 16-byte
+ * unit 5 sbase[s] + i popcount(lmask[s]) + popcount(lmask[s] & (2^l - 1)).
+ * The dense panels may be freed afterwards. */
+int flux_panel8_pack(int64_t total_quads, const int8_t *panel8, const uint64_t *lmask,
+                     const int64_t *sbase, int8_t *packed, void *stream);
 
 /* Bytes of the X-slice workspace for n rows and nrhs right-hand sides (it
  * also holds the column maxima of flux_panel8_mm on the square n x n
@@ -500,7 +514,9 @@ int flux_slice_x(int64_t n, int64_t nrhs, const double *X, const double *rho,
  * i + j < nslices; 5 is the full product, fewer slices bound the error by
  * about 2^(e_r + e_k - 7 nslices) per term (for an X much smaller than the
  * quantity the result is compared with). */
-/* qmask: the flags of flux_panel8_fill, or NULL (every subtile read). */
+/* panel8: the packed panels of flux_panel8_pack with their lmask and
+ * sbase (zero lane chunks are not read, zero subtiles skip their MFMAs),
+ * or the dense panels of flux_panel8_fill with lmask = sbase = NULL. */
 int flux_panel8_mm(int64_t nrows, const int64_t *colptr, const int32_t *cols,
                    const int64_t *qptr, const int8_t *panel8,
                    const int32_t *rowexp, int64_t nrhs, const void *work,
@@ -508,8 +524,8 @@ int flux_panel8_mm(int64_t nrows, const int64_t *colptr, const int32_t *cols,
                    double rho_s, int left, double *out, double *colmax,
                    const int32_t *items, int64_t n_items, const int32_t *nsplit,
                    const int32_t *slot0, const int32_t *split_blocks,
-                   int64_t n_split, double *ws, int nslices, const uint8_t *qmask,
-                   void *stream);
+                   int64_t n_split, double *ws, int nslices, const uint64_t *lmask,
+                   const int64_t *sbase, void *stream);
 
 /* ---------------------------------------------------------------------- */
 /* Column-blocked vector kernels (solver building blocks; blocks are      */

@@ -37,7 +37,7 @@ def setup():
     ct = _ClusterTree(np.asarray(sm.P, dtype=np.float64), 2)
     csr = _kernels.ff_csr(sm, ct.perm, ct.perm)
     Pb = PanelMatrix(csr, "bf16")
-    P8 = SlicedPanelMatrix(csr, Pb)
+    P8 = SlicedPanelMatrix(csr, Pb, keep_dense=True)
     Pb._default_plan = Pb.plan
     return torch, sm, csr, Pb, P8, ct
 
@@ -67,7 +67,7 @@ def test_panel8_layout_and_slices(setup):
     rowmax = np.asarray(abs(sp).max(axis=1).todense()).ravel()
     er = P8.rowexp.cpu().numpy()[:n]
     assert np.array_equal(er, _exp(rowmax))
-    panel = P8.panel.cpu().numpy()[:P8.quads * 20480].reshape(P8.quads, SLICES, 64, 64)
+    panel = P8.dense.cpu().numpy()[:P8.quads * 20480].reshape(P8.quads, SLICES, 64, 64)
     assert abs(panel.astype(np.int32)).max() <= 64
     w = np.ldexp(1.0, -6 - 7 * np.arange(SLICES))
     for b in range(len(colptr) - 1):
@@ -83,27 +83,48 @@ def test_panel8_layout_and_slices(setup):
         tol = np.ldexp(1.0, er[r0:r1] - 35)[:, None]
         assert (abs(dec - want) <= tol).all(), b
         assert (dec[:, ncl:] == 0).all() and (dec[want == 0] == 0).all()
-    # zero-subtile flags: byte 4 q + g set iff rows 16 g .. 16 g + 15 of
-    # quad q hold a nonzero slice byte
-    qm = P8.qmask.cpu().numpy()[:4 * P8.quads].reshape(P8.quads, 4)
-    sub = (panel != 0).reshape(P8.quads, SLICES, 4, 16, 64).any(axis=(1, 3, 4))
-    assert np.array_equal(qm.astype(bool), sub)
-    assert 0 < qm.sum() < qm.size
+    # packed form: lane chunk l of subtile s = 4 q + g is rows 16 g + (l & 15),
+    # columns 16 (l >> 4) .. + 15 of quad q; lmask bit l says it holds a
+    # nonzero, and the nonzero chunks are stored slice-major from 80 sbase[s]
+    lm = P8.lmask.cpu().numpy()[:4 * P8.quads].view(np.uint64)
+    sb = P8.sbase.cpu().numpy()
+    packed = P8.panel.cpu().numpy()
+    # (quad, slice, group g, row li, chunk lg, 16 bytes) -> (s, slice, lane, 16)
+    ch = panel.reshape(P8.quads, SLICES, 4, 16, 4, 16).transpose(0, 2, 1, 4, 3, 5)
+    ch = ch.reshape(4 * P8.quads, SLICES, 64, 16)
+    nzc = (ch != 0).any(axis=(1, 3))                          # (s, lane)
+    bits = (lm[:, None] >> np.arange(64, dtype=np.uint64)) & np.uint64(1)
+    assert np.array_equal(bits.astype(bool), nzc)
+    cnt = nzc.sum(axis=1)
+    assert sb[0] == 0 and np.array_equal(np.diff(sb), cnt) and sb[-1] == P8.chunks
+    assert packed.size >= 80 * P8.chunks
+    for s_ in range(4 * P8.quads):
+        c = int(cnt[s_])
+        if c == 0:
+            continue
+        got = packed[80 * sb[s_]:80 * (sb[s_] + c)].reshape(SLICES, c, 16)
+        assert np.array_equal(got, ch[s_][:, nzc[s_]]), s_
+    # some subtiles are zero, some lane chunks inside nonzero subtiles too
+    assert 0 < (lm != 0).sum() < lm.size
+    assert nzc[lm != 0].mean() < 1.0
 
 
-def test_panel8_zero_subtile_skip_is_exact(setup):
-    """Reading the zero quad for flagged subtiles changes nothing."""
+def test_panel8_packed_equals_dense(setup):
+    """The packed panels (zero lane chunks read from the zero quad, zero
+    subtiles' MFMAs skipped) give the dense panels' product bit for bit, for
+    every slice count."""
     torch, sm, csr, Pb, P8, ct = setup
     n = csr.shape[0]
     X = torch.rand((n, 64), dtype=torch.float64, device=_dev())
-    Y1, Y0 = torch.empty_like(X), torch.empty_like(X)
-    P8.mm_f64(X, Y1)
-    P8.skip_zero = False
-    try:
-        P8.mm_f64(X, Y0)
-    finally:
-        P8.skip_zero = True
-    assert torch.equal(Y1, Y0)
+    for ns in (5, 4, 3):
+        Y1, Y0 = torch.empty_like(X), torch.empty_like(X)
+        P8.mm_f64(X, Y1, nslices=ns)
+        P8.packed = False
+        try:
+            P8.mm_f64(X, Y0, nslices=ns)
+        finally:
+            P8.packed = True
+        assert torch.equal(Y1, Y0), ns
 
 
 @pytest.mark.parametrize("k", [64, 16, 5, 70, 33])
