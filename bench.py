@@ -601,10 +601,11 @@ def c4_line(torch, grid=641, suns=8, reps=3):
     vertex mesh (641^2: 819,200 facets; diamond-square on 1025^2, relief
     0.1), rho = 0.12 +- 0.02, 8 suns at 1-15 degrees.  F is assembled in
     row chunks in Morton order (StreamedFormFactorMatrix: one copy of the
-    sparse F plus a BF16 copy of its values) and the 8 suns are solved
+    sparse F plus BF16 and FP16 copies of its values) and the 8 suns are solved
     (a) in FP64 (Jacobi, FP64 vectors, reference stopping rule) and (b) by
     one BF16 sweep per round with FP64 residuals to max|r| <= 1e-7 max|E|
-    per sun.
+    per sun, and (c) as (b) with FP16 sweep values (sigma F, sigma a power
+    of two).
     Reports build time and ray rate, nnz and bytes, both solve times, and
     per sun the relative error of B and of T = ((1 - rho)(E + F B) /
     (emiss sigma))^(1/4) (emiss 0.95) of (b) against (a)."""
@@ -621,7 +622,7 @@ def c4_line(torch, grid=641, suns=8, reps=3):
     log(f"[bench] c4: {nf} facets, mesh + scene {time.perf_counter() - t0:.1f} s")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    S = StreamedFormFactorMatrix(sm, sweep_values="bf16", log=log, count_rays=True)
+    S = StreamedFormFactorMatrix(sm, sweep_values=("bf16", "fp16"), log=log, count_rays=True)
     t_build = time.perf_counter() - t0
     st = S.stats
     log(f"[bench] c4: F assembled in {t_build:.1f} s: nnz {st['nnz']}, rays {st['rays']}, "
@@ -649,25 +650,34 @@ def c4_line(torch, grid=641, suns=8, reps=3):
     torch.cuda.synchronize()
     t_mv = (time.perf_counter() - t0) / 3
     mv_bytes = st["nnz"] * 8 + (nf + st["chunks"]) * 8 + 2 * nf * suns * 8
-    lo = S.sweep_operator()
-    B, info = solve_radiosity_refined(lo, S, E64, rho_d, rtol=1e-7, inner_iters=1,
-                                      return_info=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        B, info = solve_radiosity_refined(lo, S, E64, rho_d, rtol=1e-7, inner_iters=1,
-                                          return_info=True)
-    torch.cuda.synchronize()
-    tref = (time.perf_counter() - t0) / reps
-    errB = ((B - B64).abs().amax(0) / B64.abs().amax(0)).cpu().numpy()
     emiss = 0.95
 
     def temp(Bx):
         Phi = E64 + S.from_internal(S.apply(S.to_internal(Bx)))
         return ((1.0 - rho_d)[:, None] * Phi / (emiss * Stefan_Boltzmann)).clamp_min(0) ** 0.25
 
-    T64, Tr = temp(B64), temp(B)
-    errT = ((Tr - T64).abs().amax(0) / T64.abs().amax(0)).cpu().numpy()
+    T64 = temp(B64)
+
+    def refined(kind):
+        lo = S.sweep_operator(kind)
+        B, info = solve_radiosity_refined(lo, S, E64, rho_d, rtol=1e-7, inner_iters=1,
+                                          return_info=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            B, info = solve_radiosity_refined(lo, S, E64, rho_d, rtol=1e-7, inner_iters=1,
+                                              return_info=True)
+        torch.cuda.synchronize()
+        tref = (time.perf_counter() - t0) / reps
+        errB = ((B - B64).abs().amax(0) / B64.abs().amax(0)).cpu().numpy()
+        errT = ((temp(B) - T64).abs().amax(0) / T64.abs().amax(0)).cpu().numpy()
+        return tref, info, errB, errT
+
+    tref, info, errB, errT = refined("bf16")
+    t16, info16, errB16, errT16 = refined("fp16")
+    log(f"[bench] c4: FP16 sweeps (values x {S.sweep_scale['fp16']:g}): refined "
+        f"{t16 * 1e3:.0f} ms ({info16['rounds']} rounds); max err B {errB16.max():.1e}, "
+        f"T {errT16.max():.1e}")
     res = {"metric": "precision configuration: 8-sun radiosity solves, BF16 sweeps + FP64 "
                      "residual vs FP64", "facets": nf, "grid": grid, "suns": suns,
            "build_s": round(t_build, 1), "rays": int(st["rays"]),
@@ -680,6 +690,11 @@ def c4_line(torch, grid=641, suns=8, reps=3):
            "solves_per_s_fp64": round(suns / t64, 2),
            "max_rel_err_B_per_sun": [float(x) for x in errB],
            "max_rel_err_T_per_sun": [float(x) for x in errT],
+           "fp16_sweeps": {"refined_ms": round(t16 * 1e3, 1), "refine_rounds": info16["rounds"],
+                           "solves_per_s_refined": round(suns / t16, 2),
+                           "value_scale": S.sweep_scale["fp16"],
+                           "max_rel_err_B_per_sun": [float(x) for x in errB16],
+                           "max_rel_err_T_per_sun": [float(x) for x in errT16]},
            "data": "synthetic (seeded fractal terrain, suns and albedo)"}
     log(f"[bench] c4: refined {tref * 1e3:.0f} ms ({info['rounds']} rounds) vs FP64 "
         f"{t64 * 1e3:.0f} ms; max err B {errB.max():.1e}, T {errT.max():.1e}")

@@ -83,16 +83,21 @@ class StreamedFormFactorMatrix:
     shape_model: a TrimeshShapeModel (either backend); eps: the prefilter
     threshold (config.DEFAULT_EPS); chunk_rows: rows per chunk (default:
     as many as one chunk's candidate mask allows within MASK_BYTES);
-    sweep_values: None or 'bf16' (a BF16 copy of the values for the
-    low-precision sweeps); log: a callable receiving progress lines;
+    sweep_values: None, 'bf16', 'fp16' or both in a tuple: 16-bit copies of
+    the values for the low-precision sweeps (FP16 holds sigma F, sigma a
+    power of two that puts max|F| in (2^13, 2^14], so small form factors
+    keep FP16's range; the sweeps divide rho by sigma); log: a callable
+    receiving progress lines;
     count_rays: also count the pairs that get a ray (a prefilter-only pass
     per chunk), for stats["rays"]."""
 
     def __init__(self, shape_model, eps=None, chunk_rows=None, sweep_values=None, log=None,
                  count_rays=False):
         t = _lib.require_gpu()
-        if sweep_values not in (None, "bf16"):
-            raise ValueError("sweep_values must be None or 'bf16'")
+        kinds = () if sweep_values is None else (
+            (sweep_values,) if isinstance(sweep_values, str) else tuple(sweep_values))
+        if not set(kinds) <= {"bf16", "fp16"} or len(set(kinds)) != len(kinds):
+            raise ValueError("sweep_values must be None, 'bf16', 'fp16' or a tuple of them")
         self.shape_model = shape_model
         self.eps = config.DEFAULT_EPS if eps is None else float(eps)
         mdt = np.dtype(shape_model.dtype)
@@ -109,8 +114,9 @@ class StreamedFormFactorMatrix:
             chunk_rows = max(64, MASK_BYTES // max(1, W * 8))
         self.chunk_rows = int(min(max(1, chunk_rows), max(n, 1)))
         self.chunks = []
-        self.sweep_values = sweep_values
-        self._sweep_data = []
+        self.sweep_values = kinds
+        self._sweep_data = {kind: [] for kind in kinds}
+        self.sweep_scale = {kind: 1.0 for kind in kinds}
         self.stats = {}
         self._build(log, count_rays)
 
@@ -158,14 +164,25 @@ class StreamedFormFactorMatrix:
                       indices.data_ptr(), data.data_ptr(), st)
             del mask, row_nnz
             self.chunks.append(_Chunk(r0, m, indptr, indices, data))
-            if self.sweep_values == "bf16":
-                self._sweep_data.append(_kernels.convert(data, t.bfloat16))
+            if "bf16" in self.sweep_values:
+                self._sweep_data["bf16"].append(_kernels.convert(data, t.bfloat16))
             nnz_total += nnz
             now = time.perf_counter()
             if log is not None and (now - last > 20.0 or r0 + m == n):
                 log(f"[streamed] rows {r0 + m}/{n}, nnz {nnz_total}, {now - t0:.1f} s")
                 last = now
             _log.debug("chunk %d: rows %d..%d nnz %d", len(self.chunks), r0, r0 + m, nnz)
+        if "fp16" in self.sweep_values:
+            fmax = 0.0
+            for c in self.chunks:
+                nz = int(c.indptr[-1].item())
+                if nz > 0:
+                    fmax = max(fmax, float(c.data[:nz].abs().max().item()))
+            sigma = 2.0 ** (14 - int(np.ceil(np.log2(fmax)))) if fmax > 0 else 1.0
+            self.sweep_scale["fp16"] = sigma
+            for c in self.chunks:
+                self._sweep_data["fp16"].append(
+                    _kernels.convert(c.data * sigma if sigma != 1.0 else c.data, t.float16))
         t.cuda.synchronize()
         self.stats = {"facets": n, "nnz": nnz_total, "chunks": len(self.chunks),
                       "chunk_rows": self.chunk_rows, "assembly_s": time.perf_counter() - t0}
@@ -181,11 +198,11 @@ class StreamedFormFactorMatrix:
     @property
     def nbytes(self):
         """Device bytes of the operator: per chunk indptr, indices and
-        values, plus the BF16 sweep values if kept."""
+        values, plus the 16-bit sweep values kept."""
         b = 0
         for c in self.chunks:
             b += c.indptr.numel() * 8 + c.indices.numel() * 4 + c.data.numel() * c.data.element_size()
-        b += sum(d.numel() * 2 for d in self._sweep_data)
+        b += sum(d.numel() * 2 for copies in self._sweep_data.values() for d in copies)
         return b
 
     # -- device operator protocol -------------------------------------------
@@ -193,12 +210,18 @@ class StreamedFormFactorMatrix:
     def device_operator(self):
         return self
 
-    def sweep_operator(self):
+    def sweep_operator(self, values=None):
         """The operator for low-precision sweeps: the same chunks with the
-        BF16 values (or this operator if no BF16 copy was kept)."""
-        if self.sweep_values is None:
+        16-bit values `values` ('bf16' or 'fp16'; default the first kept),
+        or this operator if no 16-bit copy was kept."""
+        if not self.sweep_values:
+            if values is not None:
+                raise ValueError("no 16-bit sweep values were kept")
             return self
-        return _SweepView(self)
+        kind = self.sweep_values[0] if values is None else values
+        if kind not in self.sweep_values:
+            raise ValueError(f"sweep values {kind!r} were not kept (kept: {self.sweep_values})")
+        return _SweepView(self, kind)
 
     def to_internal(self, X):
         """Rows of X (facet order) in Morton order."""
@@ -253,11 +276,35 @@ class StreamedFormFactorMatrix:
 
 
 class _SweepView(StreamedFormFactorMatrix):
-    """The chunks of a StreamedFormFactorMatrix with its BF16 values."""
+    """The chunks of a StreamedFormFactorMatrix with 16-bit values
+    (sigma F: products are divided by sigma, Jacobi steps take rho / sigma,
+    both exact for a power of two)."""
 
-    def __init__(self, base):  # pylint: disable=super-init-not-called
+    def __init__(self, base, kind):  # pylint: disable=super-init-not-called
         self.__dict__.update(base.__dict__)
         self._base = base
+        self.kind = kind
+        self.scale = base.sweep_scale[kind]
+        self._rho_cache = (None, None)
 
     def _values(self, i):
-        return self._sweep_data[i]
+        return self._sweep_data[self.kind][i]
+
+    def apply(self, X, out=None):
+        Y = super().apply(X, out=out)
+        if self.scale != 1.0:
+            Y.mul_(1.0 / self.scale)
+        return Y
+
+    def jacobi_step(self, E, B, rho, left, B1, colmax):
+        t = _lib.torch()
+        if self.scale != 1.0:
+            if isinstance(rho, t.Tensor):
+                key, val = self._rho_cache
+                if key is not rho:
+                    val = rho * (1.0 / self.scale)
+                    self._rho_cache = (rho, val)
+                rho = val
+            else:
+                rho = float(rho) / self.scale
+        super().jacobi_step(E, B, rho, left, B1, colmax)

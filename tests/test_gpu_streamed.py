@@ -1,8 +1,8 @@
 """GPU tests of the row-chunked form-factor matrix for large meshes: the
 chunks hold exactly the entries of the one-pass assembly (bit for bit), its
 products and Jacobi steps equal those of the single CSR, and the solves
-through it (FP64 Jacobi, and BF16 sweeps with FP64 residuals) agree with
-the solves on the plain form-factor matrix."""
+through it (FP64 Jacobi, and BF16 or FP16 sweeps with FP64 residuals)
+agree with the solves on the plain form-factor matrix."""
 import numpy as np
 import pytest
 import scipy.sparse
@@ -30,7 +30,7 @@ def terrain(request, torch_dev):
 @pytest.fixture(scope="module")
 def streamed(terrain):
     from fluxpy_amd.streamed import StreamedFormFactorMatrix
-    return StreamedFormFactorMatrix(terrain, chunk_rows=517, sweep_values="bf16")
+    return StreamedFormFactorMatrix(terrain, chunk_rows=517, sweep_values=("bf16", "fp16"))
 
 
 def test_chunks_equal_one_pass_assembly(torch_dev, terrain, streamed):
@@ -92,10 +92,49 @@ def test_solves_match_plain_matrix(torch_dev, terrain, streamed):
     Bp, _ = solve_radiosity(FF, E, rho)
     B64, _ = solve_radiosity(S, E, rho)
     assert np.abs(B64 - Bp).max() <= 1e-12 * np.abs(Bp).max()
-    for lo in (S, S.sweep_operator()):
+    for lo in (S, S.sweep_operator(), S.sweep_operator("fp16")):
         B, info = solve_radiosity_refined(lo, S, E, rho, rtol=1e-7, return_info=True)
         err = (np.abs(B - B64).max(0) / np.abs(B64).max(0)).max()
         assert err <= 1e-6, (err, info)
+
+
+def test_sweep_values(torch_dev, terrain, streamed):
+    """The 16-bit sweep copies: BF16 holds F, FP16 holds sigma F with sigma
+    a power of two putting max|F| in (2^13, 2^14]; the views' products and
+    Jacobi steps equal F's to the 16-bit rounding of the values (unit
+    roundoff 2^-8 for BF16's 8 significant bits, 2^-11 for FP16's 11) plus
+    FP32 accumulation ((n + 2) 2^-24), of |F| |X| per row, for both albedo
+    placements."""
+    t = torch_dev
+    from fluxpy_amd import _lib
+    S = streamed
+    n = terrain.num_faces
+    fmax = max(float(c.data[:int(c.indptr[-1].item())].abs().max().item()) for c in S.chunks)
+    sig = S.sweep_scale["fp16"]
+    assert S.sweep_scale["bf16"] == 1.0
+    assert np.log2(sig) == int(np.log2(sig)) and 2 ** 13 < sig * fmax <= 2 ** 14
+    with pytest.raises(ValueError):
+        S.sweep_operator("fp8")
+    rng = np.random.default_rng(12)
+    X64 = _lib.to_dev(rng.standard_normal((n, 8)))
+    X = X64.to(t.float32)
+    ref = S.apply(X64)
+    mag = S.apply(X64.abs())                      # |F| |X| per row
+    rho = _lib.to_dev(rng.uniform(0.05, 0.3, n)).to(t.float32)
+    E = _lib.to_dev(rng.standard_normal((n, 8))).to(t.float32)
+    acc = (n + 2) * 2.0 ** -24
+    for kind, rel in (("bf16", 2.0 ** -8 + acc), ("fp16", 2.0 ** -11 + acc)):
+        V = S.sweep_operator(kind)
+        Y = V.apply(X).to(t.float64)
+        assert ((Y - ref).abs() <= rel * mag + 1e-6 * mag.amax()).all(), kind
+        for left in (False, True):
+            B1 = t.empty_like(X)
+            cm = t.empty(8, dtype=t.float64, device=X.device)
+            V.jacobi_step(E, X, rho, left, B1, cm)
+            r64 = rho.to(t.float64)[:, None]
+            want = E.to(t.float64) + (r64 * ref if left else S.apply(r64 * X64))
+            bound = rel * (r64 * mag if left else S.apply(r64 * X64.abs())) + 1e-5
+            assert ((B1.to(t.float64) - want).abs() <= bound).all(), (kind, left)
 
 
 def test_admissible_blocks_do_not_compress(torch_dev):
