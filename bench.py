@@ -604,8 +604,9 @@ def c4_line(torch, grid=641, suns=8, reps=3):
     sparse F plus BF16 and FP16 copies of its values) and the 8 suns are solved
     (a) in FP64 (Jacobi, FP64 vectors, reference stopping rule) and (b) by
     one BF16 sweep per round with FP64 residuals to max|r| <= 1e-7 max|E|
-    per sun, and (c) as (b) with FP16 sweep values (sigma F, sigma a power
-    of two).
+    per sun, (c) as (b) with FP16 sweep values (sigma F, sigma a power
+    of two), and (d) as (b) on row-block panels of F built in column pieces
+    (BF16 panel sweeps, the FP64-accurate int8-sliced residual).
     Reports build time and ray rate, nnz and bytes, both solve times, and
     per sun the relative error of B and of T = ((1 - rho)(E + F B) /
     (emiss sigma))^(1/4) (emiss 0.95) of (b) against (a)."""
@@ -678,6 +679,27 @@ def c4_line(torch, grid=641, suns=8, reps=3):
     log(f"[bench] c4: FP16 sweeps (values x {S.sweep_scale['fp16']:g}): refined "
         f"{t16 * 1e3:.0f} ms ({info16['rounds']} rounds); max err B {errB16.max():.1e}, "
         f"T {errT16.max():.1e}")
+    # the same solve on row-block panels of F in column pieces (BF16 sweeps,
+    # FP64-accurate int8 residual on packed panels)
+    t_pan = S.build_panels()
+    log(f"[bench] c4: panels built in {t_pan:.1f} s ({S.stats['panel_pieces']} column pieces, "
+        f"{S.stats['panel_nbytes'] / 1e9:.1f} GB)")
+
+    def panel_solve():
+        return solve_radiosity_refined(S, S, E64, rho_d, rtol=1e-7, inner_iters=1,
+                                       return_info=True)
+
+    Bp, infop = panel_solve()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        Bp, infop = panel_solve()
+    torch.cuda.synchronize()
+    tpan = (time.perf_counter() - t0) / reps
+    errBp = ((Bp - B64).abs().amax(0) / B64.abs().amax(0)).cpu().numpy()
+    errTp = ((temp(Bp) - T64).abs().amax(0) / T64.abs().amax(0)).cpu().numpy()
+    log(f"[bench] c4: panel solve {tpan * 1e3:.1f} ms ({infop['rounds']} rounds); max err B "
+        f"{errBp.max():.1e}, T {errTp.max():.1e}")
     res = {"metric": "precision configuration: 8-sun radiosity solves, BF16 sweeps + FP64 "
                      "residual vs FP64", "facets": nf, "grid": grid, "suns": suns,
            "build_s": round(t_build, 1), "rays": int(st["rays"]),
@@ -690,6 +712,12 @@ def c4_line(torch, grid=641, suns=8, reps=3):
            "solves_per_s_fp64": round(suns / t64, 2),
            "max_rel_err_B_per_sun": [float(x) for x in errB],
            "max_rel_err_T_per_sun": [float(x) for x in errT],
+           "panels": {"build_s": round(t_pan, 1), "column_pieces": S.stats["panel_pieces"],
+                      "nbytes": int(S.stats["panel_nbytes"]),
+                      "refined_ms": round(tpan * 1e3, 2), "refine_rounds": infop["rounds"],
+                      "solves_per_s_refined": round(suns / tpan, 2),
+                      "max_rel_err_B_per_sun": [float(x) for x in errBp],
+                      "max_rel_err_T_per_sun": [float(x) for x in errTp]},
            "fp16_sweeps": {"refined_ms": round(t16 * 1e3, 1), "refine_rounds": info16["rounds"],
                            "solves_per_s_refined": round(suns / t16, 2),
                            "value_scale": S.sweep_scale["fp16"],

@@ -295,10 +295,10 @@ class SlicedPanelMatrix:
                   int(xscale is not None), work.data_ptr(), _lib.stream())
 
     def product(self, k, out, E=None, B=None, rho=None, rho_s=1.0, left=False, colmax=None,
-                nslices=5, slot=0):
+                nslices=5, slot=0, work=None):
         """The product from the slices last written by slice_x (second half
-        of mm_f64)."""
-        work = self.workspace(k, slot)
+        of mm_f64); `work` overrides the workspace holding them."""
+        work = self.workspace(k, slot) if work is None else work
         st = _lib.stream()
         plan = self.plan.args("int8", k, 8) if self.plan else _NO_PLAN
         if self.packed:
@@ -610,3 +610,97 @@ class PanelRefinedSolver:
         if finish is not None:
             info["finished"] = pre if (pre is not None and rounds == 2) else finish(Bs[cur])
         return Bs[cur], info
+
+
+class ColumnPieces:
+    """BF16 panels of F = [F_0 | F_1 | ...] kept as column pieces (each
+    piece's panels gather X rows by global column ids).  mm_bf16 chains the
+    pieces: out = base + s o (P_0 X) + s o (P_1 X) + ..., through two scratch
+    buffers, the next BF16 iterate from the last piece."""
+
+    def __init__(self, pieces, n):
+        self.pieces = list(pieces)
+        self.shape = (n, n)
+        self.panel = self.pieces[0].panel          # device of the panels
+        self.nnz = sum(p.nnz for p in self.pieces)
+        self._tmp = {}
+        self.timing = None
+
+    @property
+    def nbytes(self):
+        return sum(p.nbytes for p in self.pieces)
+
+    def _scratch(self, like, i):
+        t = _lib.torch()
+        key = (tuple(like.shape), like.dtype, i)
+        buf = self._tmp.get(key)
+        if buf is None:
+            buf = self._tmp[key] = t.empty_like(like)
+        return buf
+
+    def mm_bf16(self, Xb, out, base=None, rho=None, rho_s=1.0, left=False, xnext=None,
+                prev=None, colmax=None):
+        if prev is not None or colmax is not None:
+            raise NotImplementedError("column pieces: prev / colmax epilogues")
+        src = base
+        last = len(self.pieces) - 1
+        with _launch_timer(self.timing, "bf16"):
+            for i, p in enumerate(self.pieces):
+                dst = out if i == last else self._scratch(out, i & 1)
+                p.mm_bf16(Xb, dst, base=src, rho=rho, rho_s=rho_s, left=left,
+                          xnext=xnext if i == last else None)
+                src = dst
+        return out
+
+
+class SlicedColumnPieces(SlicedPanelMatrix):
+    """Int8-sliced panels of F = [F_0 | F_1 | ...] kept as column pieces
+    (built with each piece's local column ids, then read with global ones).
+    The X slices are written once for all of X; product chains the pieces:
+    out = E + s o (P_0 X) - B, then + s o (P_i X) for the others, through
+    two scratch buffers, and takes the column maxima of the result."""
+
+    def __init__(self, pieces, n):  # pylint: disable=super-init-not-called
+        self.pieces = list(pieces)
+        self.shape = (n, n)
+        self.values = "int8x5"
+        self.panel = self.pieces[0].panel
+        self.nnz = sum(p.nnz for p in self.pieces)
+        self.plan = None
+        self._work = {}
+        self._tmp = {}
+        self.packed = True
+        self.timing = None
+
+    @property
+    def nbytes(self):
+        return sum(p.nbytes for p in self.pieces)
+
+    def _scratch(self, like, i):
+        t = _lib.torch()
+        key = (tuple(like.shape), like.dtype, i)
+        buf = self._tmp.get(key)
+        if buf is None:
+            buf = self._tmp[key] = (t.zeros_like(like) if i == "zero" else t.empty_like(like))
+        return buf
+
+    def product(self, k, out, E=None, B=None, rho=None, rho_s=1.0, left=False, colmax=None,
+                nslices=5, slot=0, work=None):
+        if E is None:
+            raise NotImplementedError("column pieces: the residual form (E given) only")
+        work = self.workspace(k, slot) if work is None else work
+        last = len(self.pieces) - 1
+        src_e, src_b = E, B
+        with _launch_timer(self.timing, f"int8x{nslices}"):
+            for i, p in enumerate(self.pieces):
+                dst = out if i == last else self._scratch(out, i & 1)
+                p.product(k, dst, E=src_e, B=src_b, rho=rho, rho_s=rho_s, left=left,
+                          colmax=None, nslices=nslices, slot=slot, work=work)
+                src_e, src_b = dst, self._scratch(out, "zero")
+            if colmax is not None:
+                vo = self._tmp.get(("vo", k))
+                if vo is None:
+                    vo = self._tmp[("vo", k)] = _kernels.VecOps(k)
+                colmax.copy_(vo.maxabs(out))
+        return out
+

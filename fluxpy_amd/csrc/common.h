@@ -41,6 +41,15 @@ static inline unsigned int ceil_div_u(int64_t a, int64_t b) {
   return (unsigned int)((a + b - 1) / b);
 }
 
+// Workgroups of a grid-stride launch over `total` elements: the dispatch
+// packet counts work-items in 32 bits, so a launch of one thread per element
+// must stay far below 2^32 threads; kernels launched this way loop.
+constexpr int64_t kMaxGrid1D = 1 << 20;
+static inline unsigned int grid_1d(int64_t total, int64_t tb) {
+  const int64_t g = (total + tb - 1) / tb;
+  return (unsigned int)(g < 1 ? 1 : (g < kMaxGrid1D ? g : kMaxGrid1D));
+}
+
 constexpr int kWave = 64;
 
 }  // namespace flux

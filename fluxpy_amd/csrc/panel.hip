@@ -196,8 +196,8 @@ __global__ void __launch_bounds__(256) k_panel_fill(int64_t nrows, int64_t ncols
 
 template <typename T>
 __global__ void k_zero_any(T *p, int64_t n) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) p[k] = T(0);
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += step) p[k] = T(0);
 }
 
 // ---------------------------------------------------------------------------
@@ -874,7 +874,7 @@ extern "C" int flux_panel_fill(int64_t nrows, int64_t ncols, const int64_t *indp
              "flux_panel_fill: NULL array");
   hipStream_t st = as_stream(stream);
   const int64_t nel = total_cols * kPanelRows;
-  const unsigned gz = ceil_div_u(nel > 0 ? nel : 1, 256);
+  const unsigned gz = grid_1d(nel, 256);
   const size_t lds = 2 * panel_lds_words(ncols) * 4;
 #define PF(TA, TP)                                                                              \
   do {                                                                                          \

@@ -236,8 +236,8 @@ __global__ void __launch_bounds__(256) k_panel8_pack(int64_t nsub,
 __device__ __align__(256) int8_t g_zero_quad[kQuadBytes];
 
 __global__ void k_zero_u32(uint32_t *p, int64_t n) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) p[k] = 0u;
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += step) p[k] = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -710,7 +710,7 @@ extern "C" int flux_panel8_fill(int64_t nrows, int64_t ncols, const int64_t *ind
   hipStream_t st = as_stream(stream);
   const int64_t nword = total_quads * (kQuadBytes / 4);
   if (nword > 0)
-    hipLaunchKernelGGL(k_zero_u32, dim3(ceil_div_u(nword, 256)), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_zero_u32, dim3(grid_1d(nword, 256)), dim3(256), 0, st,
                        (uint32_t *)panel8, nword);
   const size_t lds = 2 * panel_lds_words(ncols) * 4;
   if (dtype == FLUX_F32) {

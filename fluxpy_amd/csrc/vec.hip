@@ -122,8 +122,9 @@ __device__ __forceinline__ void cvt(double x, double &y) { y = x; }
 
 template <typename TI, typename TO>
 __global__ void k_convert(int64_t total, const TI *__restrict__ x, TO *__restrict__ y) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < total) cvt(x[k], y[k]);
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += step)
+    cvt(x[k], y[k]);
 }
 
 }  // namespace flux
@@ -274,7 +275,7 @@ extern "C" int flux_vec_convert(int64_t total, const void *x, int xdtype, void *
   if (total == 0) return 0;
   FLUX_CHECK(x && y, "flux_vec_convert: NULL array");
   hipStream_t st = as_stream(stream);
-  const unsigned g = ceil_div_u(total, 256);
+  const unsigned g = grid_1d(total, 256);
   if (xdtype == FLUX_F64 && ydtype == FLUX_F32)
     hipLaunchKernelGGL((k_convert<double, float>), dim3(g), dim3(256), 0, st, total,
                        (const double *)x, (float *)y);

@@ -325,8 +325,12 @@ def _gmres(op, E, rho, left, rtol, maxiter, restart, x0):
 
 def _panel_ready(FF):
     """True for a compressed matrix: its batched solves run on the row-block
-    panels of its near field, plus its low-rank far field if it has one."""
+    panels of its near field, plus its low-rank far field if it has one; and
+    for a row-chunked matrix whose panels were built (build_panels)."""
     from .compressed_form_factors import CompressedFormFactorMatrix
+    from .streamed import StreamedFormFactorMatrix
+    if isinstance(FF, StreamedFormFactorMatrix):
+        return FF.panels is not None
     return (isinstance(FF, CompressedFormFactorMatrix) and not getattr(FF, "_derived", False)
             and getattr(FF, "_near", None) is not None)
 
@@ -337,6 +341,12 @@ def _panel_engine(FF, residual):
     are None without low-rank leaves."""
     from . import _hbuild
     from ._panel import PanelMatrix, SlicedPanelMatrix
+    from .streamed import StreamedFormFactorMatrix
+    if isinstance(FF, StreamedFormFactorMatrix):
+        if residual != "int8":
+            raise ValueError("a row-chunked matrix's panels take residual='int8'")
+        lo, hi = FF.panels
+        return lo, hi, None, None
     cache = FF.__dict__.setdefault("_panel_cache", {})
     if "bf16" not in cache:
         cache["bf16"] = PanelMatrix(FF._near, "bf16")

@@ -205,6 +205,67 @@ class StreamedFormFactorMatrix:
         b += sum(d.numel() * 2 for copies in self._sweep_data.values() for d in copies)
         return b
 
+    # -- row-block panels ---------------------------------------------------
+
+    PANEL_MAX_COLS = 1 << 18
+
+    def build_panels(self, max_cols=None):
+        """Row-block panels of F for the batched refined solve (BF16 sweeps,
+        FP64-accurate int8-sliced residual on packed panels), built once and
+        kept.  F is cut into column pieces of at most `max_cols` columns
+        (default 2^18, so each piece's column bitmap fits in LDS); each
+        piece covers all rows.  solve_radiosity_refined(S, S, ...) then runs
+        on them.  Returns the build time in seconds."""
+        from ._panel import ColumnPieces, PanelMatrix, SlicedColumnPieces, SlicedPanelMatrix
+        t = _lib.torch()
+        t0 = time.perf_counter()
+        n = self.shape[0]
+        max_cols = self.PANEL_MAX_COLS if max_cols is None else int(max_cols)
+        npieces = max(1, -(-n // max(1, max_cols)))
+        bounds = [n * p // npieces for p in range(npieces + 1)]
+        lo_pieces, hi_pieces = [], []
+        for p in range(npieces):
+            c0, c1 = bounds[p], bounds[p + 1]
+            csr = self._column_piece(c0, c1)
+            lo = PanelMatrix(csr, "bf16")
+            hi = SlicedPanelMatrix(csr, lo)
+            del csr
+            # the builders took the piece's local column ids; the products
+            # gather X rows by global ones (hi shares lo's column list)
+            lo.cols.add_(c0)
+            lo_pieces.append(lo)
+            hi_pieces.append(hi)
+        t.cuda.synchronize()
+        self._panels = (ColumnPieces(lo_pieces, n), SlicedColumnPieces(hi_pieces, n))
+        self.stats["panel_pieces"] = npieces
+        self.stats["panel_build_s"] = time.perf_counter() - t0
+        self.stats["panel_nbytes"] = self._panels[0].nbytes + self._panels[1].nbytes
+        return self.stats["panel_build_s"]
+
+    def _column_piece(self, c0, c1):
+        """F[:, c0:c1] over all rows as a DeviceCSR with local column ids."""
+        t = _lib.torch()
+        dev = _lib.device()
+        counts, idx, vals = [], [], []
+        for c in self.chunks:
+            nz = int(c.indptr[-1].item())
+            ix = c.indices[:nz]
+            sel = (ix >= c0) & (ix < c1)
+            rows = t.repeat_interleave(t.arange(c.m, device=dev), c.indptr.diff())
+            counts.append(t.bincount(rows[sel], minlength=c.m))
+            idx.append((ix[sel] - c0).to(t.int32))
+            vals.append(c.data[:nz][sel])
+            del rows, sel
+        n = self.shape[0]
+        indptr = t.zeros(n + 1, dtype=t.int64, device=dev)
+        t.cumsum(t.cat(counts), 0, out=indptr[1:])
+        return _kernels.DeviceCSR(indptr, t.cat(idx), t.cat(vals), (n, c1 - c0))
+
+    @property
+    def panels(self):
+        """(BF16 column pieces, int8 column pieces) after build_panels, or None."""
+        return getattr(self, "_panels", None)
+
     # -- device operator protocol -------------------------------------------
 
     def device_operator(self):

@@ -188,3 +188,68 @@ def test_admissible_blocks_do_not_compress(torch_dev):
         stack.extend((x, y) for x in kids_a for y in kids_b)
     assert nnz > 0.5 * A.nnz
     assert entries > 3 * nnz, (entries, nnz)
+
+
+def test_panels_in_column_pieces(torch_dev, terrain, streamed):
+    """Row-block panels of the row-chunked F, built in four column pieces
+    (each gathers X by global column ids; the products chain the pieces):
+    the int8 residual E + F(rho B) - B against the FP64 CSR within the
+    sliced product's a-priori bound (6.2 2^(e_r + e_k - 35) nnz_r per row
+    and column, e_r and e_k the exponents of max|F_r.| and max|rho B_.k|),
+    its column maxima exactly max|r|; the BF16 sweep within BF16 rounding;
+    and the refined solve on them within 1e-6 of the FP64 solution."""
+    t = torch_dev
+    from fluxpy_amd import _kernels, _lib
+    from fluxpy_amd.solve import solve_radiosity, solve_radiosity_refined
+    S = streamed
+    n = terrain.num_faces
+    S.build_panels(max_cols=-(-n // 4))
+    assert S.stats["panel_pieces"] == 4
+    lo, hi = S.panels
+    rng = np.random.default_rng(21)
+    k = 8
+    Bx = _lib.to_dev(rng.uniform(0.1, 1.0, (n, k)))
+    E = _lib.to_dev(rng.uniform(0.0, 1.0, (n, k)))
+    rho = _lib.to_dev(rng.uniform(0.08, 0.16, n))
+    B1 = t.empty_like(Bx)
+    cm = t.empty(k, dtype=t.float64, device=Bx.device)
+    S.jacobi_step(E, Bx, rho, False, B1, cm)
+    r_ref = B1 - Bx
+    mag = S.apply(rho[:, None] * Bx)                  # |F| |rho B| (F >= 0)
+    out = t.empty_like(Bx)
+    cm2 = t.empty_like(cm)
+    hi.mm_f64(Bx, out, E=E, B=Bx, rho=rho, colmax=cm2, xscale=rho)
+    rows = []
+    for c in S.chunks:
+        ip = c.indptr.cpu().numpy()
+        nnz = int(ip[-1])
+        rows.append(scipy.sparse.csr_matrix(
+            (np.abs(c.data[:nnz].cpu().numpy().astype(np.float64)),
+             c.indices[:nnz].cpu().numpy(), ip), shape=(c.m, n)))
+    Fh = scipy.sparse.vstack(rows).tocsr()
+    er = np.frexp(np.asarray(Fh.max(axis=1).todense()).ravel())[1]
+    ek = np.frexp((rho[:, None] * Bx).abs().amax(0).cpu().numpy())[1]
+    bound = (6.2 * np.ldexp(1.0, er[:, None] + ek[None, :] - 35) * np.diff(Fh.indptr)[:, None]
+             + 4e-16 * (mag.cpu().numpy() + E.abs().cpu().numpy() + Bx.abs().cpu().numpy()))
+    err = (out - r_ref).abs().cpu().numpy()
+    assert (err <= bound).all(), (err / bound).max()
+    assert t.equal(cm2, out.abs().amax(0))
+    # BF16 sweep: out = base + F Xb with Xb = bf16(rho D)
+    D = _lib.to_dev(rng.uniform(0.0, 1.0, (n, k))).to(t.float32)
+    base = _lib.to_dev(rng.uniform(0.0, 1.0, (n, k))).to(t.float32)
+    Xb = (rho[:, None].to(t.float32) * D).to(t.bfloat16)
+    o32 = t.empty_like(D)
+    lo.mm_bf16(Xb, o32, base=base, rho=rho.to(t.float32))
+    Xb64 = Xb.to(t.float64)
+    want = base.to(t.float64) + S.apply(Xb64)
+    bound = (2.0 ** -8 + (n + 4) * 2.0 ** -24) * S.apply(Xb64.abs()) + 1e-6
+    assert ((o32.to(t.float64) - want).abs() <= bound).all()
+    # the refined solve on the panels
+    D8 = meshes.sun_directions(8, seed=31)
+    Eh = terrain.get_direct_irradiance_multi(1365.0, D8).astype(np.float64)
+    rh = np.random.default_rng(10).uniform(0.08, 0.16, n)
+    B64, _ = solve_radiosity(S, Eh, rh)
+    B, info = solve_radiosity_refined(S, S, Eh, rh, rtol=1e-7, inner_iters=1, return_info=True)
+    err = (np.abs(B - B64).max(0) / np.abs(B64).max(0)).max()
+    assert err <= 1e-6, (err, info)
+    assert info["rounds"] >= 2
