@@ -270,7 +270,8 @@ class SlicedPanelMatrix:
         """max_i |s_i X_ic| per column of the X last sliced into `slot` (a
         device view into the workspace)."""
         t = _lib.torch()
-        np_ = (k + 63) // 64 * 64
+        g = 16 if k <= 16 else 64                 # right-hand sides per X group
+        np_ = (k + g - 1) // g * g
         w = self.workspace(k, slot)
         return w[np_ * 4:np_ * 12].view(t.float64)[:k]
 

@@ -30,8 +30,8 @@
 // of row rr at byte (qptr[b] + q) * 20480 + i * 4096 + rr * 64 + column in
 // quad.  Columns past the block's distinct columns are zero.
 // X slices (written per residual by flux_slice_x): right-hand sides in groups
-// of 64; facet c, group h: 320 bytes at (c * nh + h) * 320, slice j of rhs
-// 64 h + m at byte j * 64 + m.
+// of G = 64 (16 for at most 16 right-hand sides); facet c, group h: 5 G bytes
+// at (c * nh + h) * 5 G, slice j of rhs G h + m at byte j * G + m.
 #include "panel.h"
 
 namespace flux {
@@ -43,8 +43,6 @@ constexpr int kSlices = 5;
 constexpr int kQuadCols = 64;
 constexpr int kSliceTile = kPanelRows * kQuadCols;    // 4096 bytes
 constexpr int kQuadBytes = kSlices * kSliceTile;      // 20480 bytes
-constexpr int kGroup = 64;                            // right-hand sides per workgroup
-constexpr int kXRow = kSlices * kGroup;               // 320 bytes per (facet, group)
 // LDS tile of one stage, as gathered (no transposition on the way in):
 // slice j, column c of the quad, the 64 right-hand-side bytes at
 //   j * kXsSlice + c * kXsRow  (bytes; rows padded to 80 bytes and slices
@@ -52,6 +50,23 @@ constexpr int kXRow = kSlices * kGroup;               // 320 bytes per (facet, g
 // 16-lane group and the 16-byte stores at most twice per phase).
 constexpr int kXsRow = 80;
 constexpr int kXsSlice = kQuadCols * kXsRow + 64;     // bytes per slice (5184)
+
+// Right-hand sides per group G: 64, or 16 for products with at most 16
+// right-hand sides (a few suns: a group of 64 would carry 4x padding in the
+// X slices, their LDS staging and the MFMAs).  Per G: bytes of one (facet,
+// group) X row, and the LDS tile's row and slice strides.  G = 16 rows are
+// 16 bytes, slices skewed by 16 bytes: a 16-lane transposing read covers
+// 128 contiguous bytes, and the staging stores of one column's slices fall
+// on different banks.
+template <int G>
+struct XLayout {
+  static constexpr int row = kSlices * G;
+  static constexpr int srow = G == 64 ? kXsRow : 16;
+  static constexpr int sslice = G == 64 ? kXsSlice : kQuadCols * 16 + 16;
+  static constexpr int tiles = G / 16;
+};
+// group width for nrhs right-hand sides
+static inline int x8_width(int64_t nrhs) { return nrhs <= 16 ? 16 : 64; }
 
 // ds_read_b64_tr_b8 (gfx950): within each group of 16 lanes, byte e of lane
 // i's result is byte (i & 7) of the 8 bytes lane 2 e + (i >> 3) read.  With
@@ -294,14 +309,15 @@ __global__ void __launch_bounds__(256) k_colmax_final(const double *__restrict__
   if (threadIdx.x == 0) cmax[k] = red[0];
 }
 
-// x8 / xexp from s o x (rhs padded to nh * 32 with zeros).
+// x8 / xexp from s o x (rhs padded to nh * G with zeros).
+template <int G>
 __global__ void __launch_bounds__(256) k_slice_x(int64_t n, int64_t nrhs, int nh,
                                                  const double *__restrict__ x,
                                                  const double *__restrict__ rho, double rho_s,
                                                  int scale, const double *__restrict__ cmax,
                                                  int32_t *__restrict__ xexp,
                                                  int8_t *__restrict__ x8) {
-  const int64_t np = (int64_t)nh * kGroup;
+  const int64_t np = (int64_t)nh * G;
   const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= n * np) return;
   const int64_t i = id / np, k = id - i * np;
@@ -315,9 +331,9 @@ __global__ void __launch_bounds__(256) k_slice_x(int64_t n, int64_t nrhs, int nh
   if (i == 0) xexp[k] = e;
   int8_t a[kSlices];
   slice5(v, e, a);
-  const int h = (int)(k / kGroup), m = (int)(k % kGroup);
-  int8_t *dst = x8 + (i * nh + h) * (int64_t)kXRow + m;
-  for (int j = 0; j < kSlices; ++j) dst[j * kGroup] = a[j];
+  const int h = (int)(k / G), m = (int)(k % G);
+  int8_t *dst = x8 + (i * nh + h) * (int64_t)XLayout<G>::row + m;
+  for (int j = 0; j < kSlices; ++j) dst[j * G] = a[j];
 }
 
 // ---------------------------------------------------------------------------
@@ -347,6 +363,7 @@ __global__ void __launch_bounds__(256) k_slice_x(int64_t n, int64_t nrhs, int nh
 // 64-row block: y[q] = the combined sums sum_d 2^-7d S_d for row 16 w + li
 // and rhs 64 h + 16 t + 4 lg + q (w = wave, li = lane % 16, lg = lane / 16).
 // With E, the wave's column maxima of |out| go to s_cm[w][16 t + 4 lg + q].
+template <int G>
 __device__ __forceinline__ void panel8_epilogue(
     int t, double (&y)[4], int64_t b, int h, int64_t nrows, int64_t nrhs,
     const int32_t *__restrict__ rowexp, const int32_t *__restrict__ xexp,
@@ -358,7 +375,7 @@ __device__ __forceinline__ void panel8_epilogue(
   const bool row_ok = r < nrows;
   const int er = row_ok ? rowexp[r] : 0;
   const double rr = row_ok ? (rho ? rho[r] : rho_s) : 0.0;
-  const int64_t k0 = (int64_t)h * kGroup + 16 * t + 4 * lg;
+  const int64_t k0 = (int64_t)h * G + 16 * t + 4 * lg;
   double cm[4];
   for (int q = 0; q < 4; ++q) {
     const int64_t k = k0 + q;
@@ -382,33 +399,34 @@ __device__ __forceinline__ void panel8_epilogue(
   if (E) {
     for (int q = 0; q < 4; ++q) {
       const double m = max16(cm[q]);
-      if (li == 0) s_cm[w * kGroup + 16 * t + 4 * lg + q] = m;
+      if (li == 0) s_cm[w * G + 16 * t + 4 * lg + q] = m;
     }
   }
 }
 
 // Column maxima of the block (64 rows) from the waves' s_cm: one row of
 // bcm per (block, group), no atomics; k_colmax_blocks reduces over blocks.
+template <int G>
 __device__ __forceinline__ void panel8_block_colmax(const double *s_cm, int64_t b, int h, int nh,
                                                     double *__restrict__ bcm) {
   __syncthreads();
-  if (threadIdx.x < kGroup) {
+  if (threadIdx.x < G) {
     const int c = threadIdx.x;
-    const double m = fmax(fmax(s_cm[c], s_cm[kGroup + c]),
-                          fmax(s_cm[2 * kGroup + c], s_cm[3 * kGroup + c]));
-    bcm[(b * nh + h) * kGroup + c] = m;
+    const double m = fmax(fmax(s_cm[c], s_cm[G + c]), fmax(s_cm[2 * G + c], s_cm[3 * G + c]));
+    bcm[(b * nh + h) * G + c] = m;
   }
 }
 
-// colmax[k] = max over blocks of bcm[(b nh + k / 64) 64 + k % 64].
+// colmax[k] = max over blocks of bcm[(b nh + k / G) G + k % G].
+template <int G>
 __global__ void __launch_bounds__(256) k_colmax_blocks(const double *__restrict__ bcm,
                                                        int64_t nblk, int nh, int64_t nrhs,
                                                        double *__restrict__ colmax) {
   __shared__ double red[256];
   const int64_t k = blockIdx.x;
-  const int64_t h = k / kGroup, c = k % kGroup;
+  const int64_t h = k / G, c = k % G;
   double m = 0.0;
-  for (int64_t b = threadIdx.x; b < nblk; b += 256) m = fmax(m, bcm[(b * nh + h) * kGroup + c]);
+  for (int64_t b = threadIdx.x; b < nblk; b += 256) m = fmax(m, bcm[(b * nh + h) * G + c]);
   red[threadIdx.x] = m;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
@@ -428,7 +446,7 @@ __global__ void __launch_bounds__(256) k_colmax_blocks(const double *__restrict_
 // describe them); a lane whose chunk is zero reads the zero quad, and a wave
 // whose whole subtile is zero skips its MFMAs.  Otherwise panel8 holds the
 // dense quads and every subtile is read.
-template <int S, bool PACKED>
+template <int S, bool PACKED, int G>
 __global__ void __launch_bounds__(256, 2) k_panel8_mm(
     int64_t nrows, int64_t nblk, int nh, const int64_t *__restrict__ colptr,
     const int32_t *__restrict__ cols, const int64_t *__restrict__ qptr,
@@ -439,7 +457,11 @@ __global__ void __launch_bounds__(256, 2) k_panel8_mm(
     const int32_t *__restrict__ items, const int32_t *__restrict__ nsplit,
     const int32_t *__restrict__ slot0, double *__restrict__ ws, int64_t nitems,
     const uint64_t *__restrict__ lmask, const int64_t *__restrict__ sbase) {
-  __shared__ __align__(16) uint8_t xs[2][S * kXsSlice];
+  using L = XLayout<G>;
+  constexpr int NT = L::tiles;           // tiles of 16 right-hand sides
+  constexpr int UPC = S * NT;            // 16-byte X units per column and stage
+  constexpr int NX = (kQuadCols * UPC + 255) / 256;  // X units per thread
+  __shared__ __align__(16) uint8_t xs[2][S * L::sslice];
   __shared__ __align__(16) int32_t ids[4][kQuadCols];
   // per stage and wave: the subtile's lane mask and packed base (PACKED)
   __shared__ uint64_t lms[4][4];
@@ -461,11 +483,11 @@ __global__ void __launch_bounds__(256, 2) k_panel8_mm(
   const v4i *pz = reinterpret_cast<const v4i *>(g_zero_quad + (16 * w + li) * kQuadCols + 16 * lg);
   const v4i *pk = reinterpret_cast<const v4i *>(panel8);
   const uint64_t below = (1ull << lane) - 1ull;
-  const int8_t *xg = x8 + h * (int64_t)kXRow;
-  const int64_t xstride = (int64_t)nh * kXRow;
-  v4i acc[S][4];
+  const int8_t *xg = x8 + h * (int64_t)L::row;
+  const int64_t xstride = (int64_t)nh * L::row;
+  v4i acc[S][NT];
   for (int d = 0; d < S; ++d)
-    for (int t = 0; t < 4; ++t) acc[d][t] = v4i{0, 0, 0, 0};
+    for (int t = 0; t < NT; ++t) acc[d][t] = v4i{0, 0, 0, 0};
   if (nq > 0) {
     // every load address is clamped into the block, so nothing is masked
     auto load_id = [&](int64_t s) -> int32_t {
@@ -477,20 +499,24 @@ __global__ void __launch_bounds__(256, 2) k_panel8_mm(
       m = lmask[sub];
       bs = sbase[sub];
     };
-    auto load_x = [&](int64_t s, v4i (&xr)[S]) {
+    // X unit u of a stage: column c = u / UPC, unit ch = u % UPC of its
+    // row (slice ch / NT, right-hand sides 16 (ch % NT) ..); past the last
+    // unit the index is clamped (a duplicate load and store of the same
+    // bytes), so no load sits behind a branch
+    auto load_x = [&](int64_t s, v4i (&xr)[NX]) {
       const int32_t *id = ids[s & 3];
-      for (int m = 0; m < S; ++m) {
-        const int u = tid + 256 * m;
-        const int c = u / (4 * S), ch = u - c * (4 * S);
+      for (int m = 0; m < NX; ++m) {
+        const int u = min(tid + 256 * m, kQuadCols * UPC - 1);
+        const int c = u / UPC, ch = u - c * UPC;
         xr[m] = *reinterpret_cast<const v4i *>(xg + (int64_t)id[c] * xstride + 16 * ch);
       }
     };
-    auto store_x = [&](int buf, const v4i (&xr)[S]) {
-      for (int m = 0; m < S; ++m) {
-        const int u = tid + 256 * m;
-        const int c = u / (4 * S), ch = u - c * (4 * S);
-        *reinterpret_cast<v4i *>(&xs[buf][(ch >> 2) * kXsSlice + c * kXsRow + 16 * (ch & 3)]) =
-            xr[m];
+    auto store_x = [&](int buf, const v4i (&xr)[NX]) {
+      for (int m = 0; m < NX; ++m) {
+        const int u = min(tid + 256 * m, kQuadCols * UPC - 1);
+        const int c = u / UPC, ch = u - c * UPC;
+        *reinterpret_cast<v4i *>(
+            &xs[buf][(ch / NT) * L::sslice + c * L::srow + 16 * (ch % NT)]) = xr[m];
       }
     };
     auto load_p = [&](int64_t s, v4i (&pf)[S]) {
@@ -514,7 +540,7 @@ __global__ void __launch_bounds__(256, 2) k_panel8_mm(
       // streaming loads: the panels are read once, the X slices many times
       for (int i = 0; i < S; ++i) pf[i] = __builtin_nontemporal_load(p + i * stride);
     };
-    v4i xr[S];
+    v4i xr[NX];
     v4i p0[S], p1[S];
     // prologue: ids(0..2) to LDS (every wave stores the same ids, so no
     // branch sits next to a load), X(0) to LDS, panel(0) in flight
@@ -548,7 +574,7 @@ __global__ void __launch_bounds__(256, 2) k_panel8_mm(
       __builtin_amdgcn_sched_barrier(0);
       // A operand of tile t, slice j: columns 16 lg + 8 h + e (h = 0, 1) of
       // right-hand side 16 t + li, two transposing reads
-      const uint8_t *xq = xs[buf] + (16 * lg + (li >> 1)) * kXsRow + 8 * (li & 1);
+      const uint8_t *xq = xs[buf] + (16 * lg + (li >> 1)) * L::srow + 8 * (li & 1);
       // a zero subtile of this wave contributes nothing: its MFMAs are skipped
       // (a wave-uniform branch with no load inside)
       const uint64_t msk = PACKED ? uniform_u64(lms[s & 3][w]) : ~0ull;
@@ -558,22 +584,22 @@ __global__ void __launch_bounds__(256, 2) k_panel8_mm(
       for (int i = 0; i < S; ++i)
         pz4[i] = (!PACKED || ((msk >> lane) & 1ull)) ? pc[i] : v4i{0, 0, 0, 0};
       if (nzs) {
-        // X slice j of all four tiles is read one slice ahead of its
-        // 4 (S - j) MFMAs (independent accumulators), so the transposing
-        // LDS reads overlap the matrix work
-        auto read_a = [&](int j, v4i (&dst)[4]) {
-          for (int t = 0; t < 4; ++t) {
-            const uint8_t *pa = xq + j * kXsSlice + 16 * t;
-            const v2i a0 = lds_read_tr8(pa), a1 = lds_read_tr8(pa + 8 * kXsRow);
+        // X slice j of all tiles is read one slice ahead of its NT (S - j)
+        // MFMAs (independent accumulators), so the transposing LDS reads
+        // overlap the matrix work
+        auto read_a = [&](int j, v4i (&dst)[NT]) {
+          for (int t = 0; t < NT; ++t) {
+            const uint8_t *pa = xq + j * L::sslice + 16 * t;
+            const v2i a0 = lds_read_tr8(pa), a1 = lds_read_tr8(pa + 8 * L::srow);
             dst[t] = v4i{a0[0], a0[1], a1[0], a1[1]};
           }
         };
-        v4i ab[2][4];
+        v4i ab[2][NT];
         read_a(0, ab[0]);
         for (int j = 0; j < S; ++j) {
           if (j + 1 < S) read_a(j + 1, ab[(j + 1) & 1]);
           for (int i = 0; i + j < S; ++i)
-            for (int t = 0; t < 4; ++t)
+            for (int t = 0; t < NT; ++t)
               acc[i + j][t] =
                   __builtin_amdgcn_mfma_i32_16x16x64_i8(ab[j & 1][t], pz4[i], acc[i + j][t], 0, 0, 0);
         }
@@ -601,42 +627,45 @@ __global__ void __launch_bounds__(256, 2) k_panel8_mm(
     return v;
   };
   if (itm.nsplit > 1) {
-    // partial of a split block: to its slot (k_panel8_reduce finishes)
-    double *slot = ws + ((int64_t)(itm.slot0 + itm.piece) * nh + h) * 4096 + tid;
-    for (int t = 0; t < 4; ++t)
+    // partial of a split block: to its slot of 64 G values (k_panel8_reduce
+    // finishes)
+    double *slot = ws + ((int64_t)(itm.slot0 + itm.piece) * nh + h) * (64 * G) + tid;
+    for (int t = 0; t < NT; ++t)
       for (int q = 0; q < 4; ++q) __builtin_nontemporal_store(combine(t, q), slot + (4 * t + q) * 256);
     return;
   }
-  __shared__ double s_cm[4 * kGroup];
-  for (int t = 0; t < 4; ++t) {
+  __shared__ double s_cm[4 * G];
+  for (int t = 0; t < NT; ++t) {
     double y[4];
     for (int q = 0; q < 4; ++q) y[q] = combine(t, q);
-    panel8_epilogue(t, y, b, h, nrows, nrhs, rowexp, xexp, E, B, rho, rho_s, left, out, s_cm);
+    panel8_epilogue<G>(t, y, b, h, nrows, nrhs, rowexp, xexp, E, B, rho, rho_s, left, out, s_cm);
   }
-  if (bcm && E) panel8_block_colmax(s_cm, b, h, nh, bcm);
+  if (bcm && E) panel8_block_colmax<G>(s_cm, b, h, nh, bcm);
 }
 
 // Split blocks of a planned product: the sum of the block's partials, in
-// slot order, and the epilogue.  One workgroup per (split block, group of 64
-// right-hand sides, tile t of 16 of them), so a block's four tiles reduce in
+// slot order, and the epilogue.  One workgroup per (split block, group of G
+// right-hand sides, tile t of 16 of them), so a block's tiles reduce in
 // parallel; each thread sums 4 values of every slot.  Slots go four at a
 // time with all 16 loads issued before any add; the index of a slot past the
 // last is clamped to the last (a cached re-read) and its add skipped, so no
 // load sits behind a branch.
+template <int G>
 __global__ void __launch_bounds__(256) k_panel8_reduce(
     const int32_t *__restrict__ split_blocks, const int32_t *__restrict__ nsplit,
     const int32_t *__restrict__ slot0, const double *__restrict__ ws, int nh, int64_t nrows,
     const int32_t *__restrict__ rowexp, const int32_t *__restrict__ xexp, int64_t nrhs,
     const double *__restrict__ E, const double *__restrict__ B, const double *__restrict__ rho,
     double rho_s, int left, double *__restrict__ out, double *__restrict__ bcm) {
-  __shared__ double s_cm[4 * kGroup];
-  const int t = (int)(blockIdx.x & 3);
-  const int64_t g = blockIdx.x >> 2;
+  constexpr int NT = G / 16;
+  __shared__ double s_cm[4 * G];
+  const int t = (int)(blockIdx.x % NT);
+  const int64_t g = blockIdx.x / NT;
   const int64_t b = split_blocks[g / nh];
   const int h = (int)(g % nh);
   const int ns = nsplit[b], s0 = slot0[b];
-  const int64_t stride = (int64_t)nh * 4096;
-  const double *src = ws + ((int64_t)s0 * nh + h) * 4096 + 1024 * t + threadIdx.x;
+  const int64_t stride = (int64_t)nh * (64 * G);
+  const double *src = ws + ((int64_t)s0 * nh + h) * (64 * G) + 1024 * t + threadIdx.x;
   double y[4] = {0.0, 0.0, 0.0, 0.0};
   for (int k = 0; k < ns; k += 4) {
     double v[4][4];
@@ -648,14 +677,13 @@ __global__ void __launch_bounds__(256) k_panel8_reduce(
       if (k + j < ns)
         for (int q = 0; q < 4; ++q) y[q] += v[j][q];
   }
-  panel8_epilogue(t, y, b, h, nrows, nrhs, rowexp, xexp, E, B, rho, rho_s, left, out, s_cm);
+  panel8_epilogue<G>(t, y, b, h, nrows, nrhs, rowexp, xexp, E, B, rho, rho_s, left, out, s_cm);
   if (bcm && E) {
     __syncthreads();
     if (threadIdx.x < 16) {
       const int c = 16 * t + (int)threadIdx.x;
-      const double m = fmax(fmax(s_cm[c], s_cm[kGroup + c]),
-                            fmax(s_cm[2 * kGroup + c], s_cm[3 * kGroup + c]));
-      bcm[(b * nh + h) * kGroup + c] = m;
+      const double m = fmax(fmax(s_cm[c], s_cm[G + c]), fmax(s_cm[2 * G + c], s_cm[3 * G + c]));
+      bcm[(b * nh + h) * G + c] = m;
     }
   }
 }
@@ -764,19 +792,23 @@ extern "C" int flux_panel8_pack(int64_t total_quads, const int8_t *panel8, const
   return 0;
 }
 
-static int64_t x8_groups(int64_t nrhs) { return (nrhs + kGroup - 1) / kGroup; }
+static int64_t x8_groups(int64_t nrhs) {
+  const int g = x8_width(nrhs);
+  return (nrhs + g - 1) / g;
+}
 
-// workspace: xexp (nh * 64 int32), cmax (nh * 64 double), x8 (n * nh * 320
-// bytes), then the block column maxima of the product (ceil(n / 64) * nh *
-// 64 doubles; the product of a square matrix of n rows), then the
-// per-workgroup maxima of the slicer (kColmaxGrid * nh * 64 doubles)
+// workspace, with G = x8_width(nrhs) right-hand sides per group and np =
+// nh G: xexp (np int32), cmax (np double), x8 (n * nh * 5 G bytes), then the
+// block column maxima of the product (ceil(n / 64) * np doubles; the
+// product of a square matrix of n rows), then the per-workgroup maxima of
+// the slicer (kColmaxGrid * np doubles)
 constexpr int64_t kColmaxGrid = 512;
 extern "C" int64_t flux_slice_x_bytes(int64_t n, int64_t nrhs) {
   if (n < 0 || nrhs < 0) return -1;
   const int64_t nh = x8_groups(nrhs);
   const int64_t nblk = (n + kPanelRows - 1) / kPanelRows;
-  return nh * kGroup * (4 + 8) + n * nh * kXRow + nblk * nh * kGroup * 8 +
-         kColmaxGrid * nh * kGroup * 8;
+  const int64_t g = x8_width(nrhs), np = nh * g;
+  return np * (4 + 8) + n * nh * kSlices * g + nblk * np * 8 + kColmaxGrid * np * 8;
 }
 
 extern "C" int flux_slice_x(int64_t n, int64_t nrhs, const double *X, const double *rho,
@@ -786,18 +818,23 @@ extern "C" int flux_slice_x(int64_t n, int64_t nrhs, const double *X, const doub
   FLUX_CHECK(X && work, "flux_slice_x: NULL array");
   FLUX_CHECK(((uintptr_t)work & 15) == 0, "flux_slice_x: workspace must be 16-byte aligned");
   hipStream_t st = as_stream(stream);
-  const int64_t nh = x8_groups(nrhs), np = nh * kGroup;
+  const int g = x8_width(nrhs);
+  const int64_t nh = x8_groups(nrhs), np = nh * g;
   int32_t *xexp = (int32_t *)work;
   double *cmax = (double *)((char *)work + np * 4);
   int8_t *x8 = (int8_t *)((char *)work + np * 12);
   const int64_t nblk = (n + kPanelRows - 1) / kPanelRows;
-  double *pmax = (double *)((char *)work + np * 12 + n * nh * kXRow + nblk * np * 8);
+  double *pmax = (double *)((char *)work + np * 12 + n * nh * kSlices * g + nblk * np * 8);
   const int64_t gx = (n + 3) / 4 < kColmaxGrid ? (n + 3) / 4 : kColmaxGrid;
-  hipLaunchKernelGGL(k_colmax_scaled, dim3((unsigned)gx, (unsigned)nh), dim3(64, 4), 0, st, n,
-                     nrhs, np, X, rho, rho_s, scale, pmax);
+  hipLaunchKernelGGL(k_colmax_scaled, dim3((unsigned)gx, ceil_div_u(np, 64)), dim3(64, 4), 0, st,
+                     n, nrhs, np, X, rho, rho_s, scale, pmax);
   hipLaunchKernelGGL(k_colmax_final, dim3((unsigned)np), dim3(256), 0, st, pmax, gx, np, cmax);
-  hipLaunchKernelGGL(k_slice_x, dim3(ceil_div_u(n * np, 256)), dim3(256), 0, st, n, nrhs,
-                     (int)nh, X, rho, rho_s, scale, cmax, xexp, x8);
+  if (g == 16)
+    hipLaunchKernelGGL(k_slice_x<16>, dim3(ceil_div_u(n * np, 256)), dim3(256), 0, st, n, nrhs,
+                       (int)nh, X, rho, rho_s, scale, cmax, xexp, x8);
+  else
+    hipLaunchKernelGGL(k_slice_x<64>, dim3(ceil_div_u(n * np, 256)), dim3(256), 0, st, n, nrhs,
+                       (int)nh, X, rho, rho_s, scale, cmax, xexp, x8);
   FLUX_LAUNCH_CHECK();
   return 0;
 }
@@ -826,30 +863,40 @@ extern "C" int flux_panel8_mm(int64_t nrows, const int64_t *colptr, const int32_
              "flux_panel8_mm: a plan needs nsplit, slot0, split_blocks and ws");
   const int64_t nit = items ? n_items : nblk;
   hipStream_t st = as_stream(stream);
-  const int64_t nh = x8_groups(nrhs), np = nh * kGroup;
+  const int g = x8_width(nrhs);
+  const int64_t nh = x8_groups(nrhs), np = nh * g;
   const int32_t *xexp = (const int32_t *)work;
   const int8_t *x8 = (const int8_t *)work + np * 12;
   // block column maxima after x8 (the matrix is square: X has nrows rows)
-  double *bcm = (colmax && E) ? (double *)((char *)work + np * 12 + nrows * nh * kXRow)
-                              : nullptr;
+  double *bcm = (colmax && E)
+                    ? (double *)((char *)work + np * 12 + nrows * nh * kSlices * g)
+                    : nullptr;
   FLUX_CHECK(nit * nh < (1ll << 31) && n_split * nh * 4 < (1ll << 31),
              "flux_panel8_mm: too many workgroups");
-  auto kernel = lmask ? (nslices == 5   ? k_panel8_mm<5, true>
-                         : nslices == 4 ? k_panel8_mm<4, true>
-                                        : k_panel8_mm<3, true>)
-                      : (nslices == 5   ? k_panel8_mm<5, false>
-                         : nslices == 4 ? k_panel8_mm<4, false>
-                                        : k_panel8_mm<3, false>);
-  hipLaunchKernelGGL(kernel, dim3((unsigned)(nit * nh)), dim3(256), 0, st, nrows, nblk,
-                     (int)nh, colptr, cols, qptr, panel8, rowexp, x8, xexp, nrhs, E, B, rho,
-                     rho_s, left, out, bcm, items, nsplit, slot0, ws, nit, lmask, sbase);
-  if (items && n_split > 0)
-    hipLaunchKernelGGL(k_panel8_reduce, dim3((unsigned)(n_split * nh * 4)), dim3(256), 0, st,
-                       split_blocks, nsplit, slot0, ws, (int)nh, nrows, rowexp, xexp, nrhs, E, B,
-                       rho, rho_s, left, out, bcm);
-  if (bcm)
-    hipLaunchKernelGGL(k_colmax_blocks, dim3((unsigned)nrhs), dim3(256), 0, st, bcm, nblk,
-                       (int)nh, nrhs, colmax);
+#define P8MM(G_)                                                                              \
+  do {                                                                                        \
+    auto kernel = lmask ? (nslices == 5   ? k_panel8_mm<5, true, G_>                           \
+                           : nslices == 4 ? k_panel8_mm<4, true, G_>                           \
+                                          : k_panel8_mm<3, true, G_>)                          \
+                        : (nslices == 5   ? k_panel8_mm<5, false, G_>                          \
+                           : nslices == 4 ? k_panel8_mm<4, false, G_>                          \
+                                          : k_panel8_mm<3, false, G_>);                        \
+    hipLaunchKernelGGL(kernel, dim3((unsigned)(nit * nh)), dim3(256), 0, st, nrows, nblk,     \
+                       (int)nh, colptr, cols, qptr, panel8, rowexp, x8, xexp, nrhs, E, B, rho, \
+                       rho_s, left, out, bcm, items, nsplit, slot0, ws, nit, lmask, sbase);    \
+    if (items && n_split > 0)                                                                 \
+      hipLaunchKernelGGL(k_panel8_reduce<G_>, dim3((unsigned)(n_split * nh * (G_ / 16))),     \
+                         dim3(256), 0, st, split_blocks, nsplit, slot0, ws, (int)nh, nrows,    \
+                         rowexp, xexp, nrhs, E, B, rho, rho_s, left, out, bcm);                \
+    if (bcm)                                                                                  \
+      hipLaunchKernelGGL(k_colmax_blocks<G_>, dim3((unsigned)nrhs), dim3(256), 0, st, bcm,    \
+                         nblk, (int)nh, nrhs, colmax);                                         \
+  } while (0)
+  if (g == 16)
+    P8MM(16);
+  else
+    P8MM(64);
+#undef P8MM
   FLUX_LAUNCH_CHECK();
   return 0;
 }

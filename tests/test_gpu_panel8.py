@@ -115,8 +115,9 @@ def test_panel8_packed_equals_dense(setup):
     every slice count."""
     torch, sm, csr, Pb, P8, ct = setup
     n = csr.shape[0]
-    X = torch.rand((n, 64), dtype=torch.float64, device=_dev())
-    for ns in (5, 4, 3):
+    for k in (64, 8):
+      X = torch.rand((n, k), dtype=torch.float64, device=_dev())
+      for ns in (5, 4, 3):
         Y1, Y0 = torch.empty_like(X), torch.empty_like(X)
         P8.mm_f64(X, Y1, nslices=ns)
         P8.packed = False
@@ -154,11 +155,12 @@ def test_panel8_product_within_bound(setup, k):
     assert rel.size == 0 or np.median(rel) < 1e-9, np.median(rel)
 
 
+@pytest.mark.parametrize("k", [64, 8])
 @pytest.mark.parametrize("left", [False, True])
-def test_panel8_residual(setup, left):
+def test_panel8_residual(setup, left, k):
+    """k = 8 runs the groups of 16 right-hand sides, k = 64 those of 64."""
     torch, sm, csr, Pb, P8, ct = setup
     n = csr.shape[0]
-    k = 64
     sp = csr.to_scipy().astype(np.float64)
     rng = np.random.default_rng(7 + int(left))
     E = rng.uniform(0, 1000, (n, k))
@@ -350,7 +352,7 @@ def test_plan_covers_blocks(setup):
         assert np.array_equal(np.sort(used), np.arange(pl.n_slots))
 
 
-@pytest.mark.parametrize("k", [64, 70])
+@pytest.mark.parametrize("k", [64, 70, 8])
 def test_planned_products_match_unplanned(setup, k):
     """Split blocks (cap of 2 quads, so nearly every block is cut) give the
     same products, residuals and residual column maxima as one workgroup per
