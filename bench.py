@@ -48,7 +48,7 @@ INT8_PEAK_TOPS = 5000.0
 MEASURED_TRAFFIC = {
     "k_panel8_mm<5> + k_panel8_reduce + k_colmax_blocks": 1092188887,
     "k_panel8_mm<4> + k_panel8_reduce + k_colmax_blocks": 961555057,
-    "k_panel_mm_bf16 + k_panel_reduce_bf16": 624443802,
+    "k_panel_mm_bf16 + k_panel_reduce_bf16": 338632704,
 }
 
 

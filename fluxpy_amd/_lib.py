@@ -84,7 +84,9 @@ SIGNATURES = {
     "flux_panel_plan": (_c_int, [_c_i64, _vp, _c_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "flux_panel_mm_bf16": (_c_int, [_c_i64, _vp, _vp, _vp, _vp, _c_i64, _vp, _vp,
                                     ctypes.c_float, _c_int, _vp, _vp, _vp, _vp, _vp, _c_i64,
-                                    _vp, _vp, _vp, _c_i64, _vp, _c_int, _vp]),
+                                    _vp, _vp, _vp, _c_i64, _vp, _c_int, _vp, _vp, _vp]),
+    "flux_panel_lanemask": (_c_int, [_c_i64, _vp, _vp, _vp, _vp, _vp]),
+    "flux_panel_pack": (_c_int, [_c_i64, _vp, _vp, _vp, _vp, _vp]),
     "flux_panel_mm_f64": (_c_int, [_c_i64, _vp, _vp, _vp, _vp, _c_i64, _vp, _vp, _vp, _c_dbl,
                                    _c_int, _vp, _vp, _vp]),
     "flux_refine_prepare_work_bytes": (_c_i64, [_c_i64]),

@@ -66,15 +66,23 @@ _NO_PLAN = (None, 0, None, None, None, 0, None)
 
 
 class PanelMatrix:
-    """Panels of a square CSR in one value precision ('bf16' or 'fp32')."""
+    """Panels of a square CSR in one value precision ('bf16' or 'fp32').
+    BF16 panels are kept packed: of every 16-row x 16-column subtile only
+    the 8-byte lane chunks (one row, 4 columns) that hold a nonzero, with a
+    64-bit lane mask and a base offset per subtile (see csrc/panel.hip).
+    ``keep_dense`` also keeps the dense panels they were packed from (for
+    tests)."""
 
     ROWS = 64
+    # False keeps BF16 panels dense
+    PACK_BF16 = True
 
     # columns of each block ordered by their 16-row occupancy mask (see
     # flux_panel_order); False keeps them in increasing order
     ORDER_BY_MASK = True
 
-    def __init__(self, csr, values="bf16", colptr=None, cols=None, plan=None, order=None):
+    def __init__(self, csr, values="bf16", colptr=None, cols=None, plan=None, order=None,
+                 keep_dense=False):
         t = _lib.require_gpu()
         m, n = csr.shape
         self.shape = (m, n)
@@ -117,6 +125,35 @@ class PanelMatrix:
         self.nnz = csr.nnz
         # balanced work items (shared with siblings); None -> one per block
         self.plan = plan if plan is not None else PanelPlan(colptr, nblk)
+        self.lmask = self.sbase = self.dense = None
+        self.packed = False         # products read the packed panels (False: dense)
+        if values == "bf16" and self.PACK_BF16:
+            self._pack(keep_dense)
+
+    def _pack(self, keep_dense):
+        t = _lib.torch()
+        st = _lib.stream()
+        dev = self.panel.device
+        nch = self.total_cols // 16
+        nsub = 4 * nch
+        self.lmask = t.empty(max(nsub, 1), dtype=t.int64, device=dev)   # uint64 bits
+        self.sbase = t.zeros(nsub + 1, dtype=t.int64, device=dev)
+        lanes = ctypes.c_int64()
+        _lib.call("flux_panel_lanemask", nch, self.panel.data_ptr(), self.lmask.data_ptr(),
+                  self.sbase.data_ptr(), ctypes.byref(lanes), st)
+        self.lanes = lanes.value
+        packed = t.empty(max(4 * self.lanes, 4), dtype=t.bfloat16, device=dev)
+        _lib.call("flux_panel_pack", nch, self.panel.data_ptr(), self.lmask.data_ptr(),
+                  self.sbase.data_ptr(), packed.data_ptr(), st)
+        self.dense = self.panel if keep_dense else None
+        self.panel = packed
+        self.packed = True
+
+    @property
+    def nonzero_subtiles(self):
+        """Number of 16 x 16 subtiles of packed BF16 panels that hold a
+        nonzero (the products skip the MFMAs of the others)."""
+        return int((self.lmask[:self.total_cols // 4] != 0).sum().item())
 
     def sibling(self, csr, values):
         """Panels of the same pattern in another precision (shares cols)."""
@@ -125,8 +162,9 @@ class PanelMatrix:
 
     @property
     def nbytes(self):
+        meta = 0 if self.lmask is None else (self.lmask.numel() + self.sbase.numel()) * 8
         return (self.panel.numel() * self.panel.element_size() + self.cols.numel() * 4
-                + self.colptr.numel() * 8)
+                + self.colptr.numel() * 8 + meta)
 
     @property
     def fill(self):
@@ -139,12 +177,20 @@ class PanelMatrix:
         """out = base + s (P Xb) with the Jacobi epilogue (see panel.hip)."""
         k = Xb.shape[1]
         plan = self.plan.args("bf16", k, 4) if self.plan else _NO_PLAN
+        if self.packed:
+            panel, lmask, sbase = self.panel, self.lmask.data_ptr(), self.sbase.data_ptr()
+        elif self.lmask is None:
+            panel, lmask, sbase = self.panel, None, None
+        elif self.dense is not None:
+            panel, lmask, sbase = self.dense, None, None
+        else:
+            raise ValueError("packed=False needs the dense panels (keep_dense=True)")
         with _launch_timer(self.timing, "bf16"):
             _lib.call("flux_panel_mm_bf16", self.shape[0], self.colptr.data_ptr(),
-                      self.cols.data_ptr(), self.panel.data_ptr(), Xb.data_ptr(), k,
+                      self.cols.data_ptr(), panel.data_ptr(), Xb.data_ptr(), k,
                       _lib.ptr(base), _lib.ptr(rho), float(rho_s), int(left), out.data_ptr(),
                       _lib.ptr(xnext), _lib.ptr(prev), _lib.ptr(colmax), *plan,
-                      int(self.stage), _lib.stream())
+                      int(self.stage), lmask, sbase, _lib.stream())
         return out
 
     # 16-column chunks per pipeline stage of the BF16 product (0: library

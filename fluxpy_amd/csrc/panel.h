@@ -14,6 +14,14 @@ __device__ __forceinline__ int64_t xcd_block(int64_t wg, int64_t nblk) {
   return x * q + min(x, r) + j;
 }
 
+// A value the whole wave holds (read from LDS at a wave-uniform address),
+// moved to scalar registers so that branches on it are scalar.
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
 // Max over the 16 lanes that share lane / 16 (bits 0-3 of the lane id).
 __device__ __forceinline__ double max16(double v) {
   for (int m = 1; m < 16; m <<= 1) v = fmax(v, __shfl_xor(v, m, 64));

@@ -24,6 +24,15 @@
 // stage s and a workgroup waits at one barrier per stage.  Row blocks are
 // dealt to the 8 XCDs in contiguous runs so neighbouring blocks, which
 // gather mostly the same X rows, share an L2.
+//
+// Packed BF16 panels: two thirds of a panel's values are zero, and within
+// a chunk the zeros cluster, so the BF16 panels are kept packed.  Subtile
+// s = 4 chunk + w is rows 16 w .. 16 w + 15 of a chunk; its lane chunk l is
+// row 16 w + (l & 15), columns 4 (l >> 4) .. + 3 (8 bytes, the B operand
+// lane l feeds one MFMA).  Only the nonzero lane chunks are stored, in lane
+// order: chunk l of subtile s at 8-byte unit
+//   sbase[s] + popcount(lmask[s] & ((1 << l) - 1)),
+// with lmask[s] bit l set when chunk l holds a nonzero bit pattern.
 #include <hip/hip_bf16.h>
 
 #include <algorithm>
@@ -290,13 +299,55 @@ __device__ __forceinline__ void bf16_epilogue(
                        colmax);
 }
 
+// Lane chunk l of subtile s of the dense BF16 panels (8-byte units).
+__device__ __forceinline__ int64_t bf16_lane_unit(int64_t s, int lane) {
+  return (s >> 2) * 256 + (s & 3) * 64 + (lane & 15) * 4 + (lane >> 4);
+}
+
+// lmask[s] has bit l set when lane chunk l of subtile s is not all zero
+// bits; count[s] is its number of set bits.  One wave per subtile, the
+// waves striding over the subtiles.
+__global__ void __launch_bounds__(256) k_panel_lanemask(int64_t nsub,
+                                                        const uint64_t *__restrict__ panel,
+                                                        uint64_t *__restrict__ lmask,
+                                                        int64_t *__restrict__ count) {
+  const int lane = threadIdx.x & 63;
+  const int64_t nwave = (int64_t)gridDim.x * (blockDim.x >> 6);
+  for (int64_t s = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; s < nsub; s += nwave) {
+    const uint64_t m = __ballot(panel[bf16_lane_unit(s, lane)] != 0ull);
+    if (lane == 0) {
+      lmask[s] = m;
+      count[s] = __popcll(m);
+    }
+  }
+}
+
+// Packed panels from the dense ones (layout in the header).  One wave per
+// subtile, the waves striding over the subtiles.
+__global__ void __launch_bounds__(256) k_panel_pack(int64_t nsub,
+                                                    const uint64_t *__restrict__ panel,
+                                                    const uint64_t *__restrict__ lmask,
+                                                    const int64_t *__restrict__ sbase,
+                                                    uint64_t *__restrict__ packed) {
+  const int lane = threadIdx.x & 63;
+  const int64_t nwave = (int64_t)gridDim.x * (blockDim.x >> 6);
+  for (int64_t s = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; s < nsub; s += nwave) {
+    const uint64_t m = lmask[s];
+    if ((m >> lane) & 1ull)
+      packed[sbase[s] + __popcll(m & ((1ull << lane) - 1ull))] = panel[bf16_lane_unit(s, lane)];
+  }
+}
+
+// What a lane of the packed BF16 product loads for a zero lane chunk.
+__device__ __align__(256) unsigned long long g_zero_lanes[16];
+
 // SC chunks of 16 columns per stage.  LDS block per stage (bf16):
 // xs[kg][rhs][4], kg = group of 4 consecutive columns of the stage, so the
 // A fragment of a lane (4 consecutive columns at one right-hand side) is one
 // 8-byte read.
 // Waves per SIMD: 3 for SC = 4 (all 648 row blocks of the 41k-facet
 // terrain resident at once, no spills), 2 for SC = 8.
-template <int SC, bool VEC>
+template <int SC, bool VEC, bool PACKED>
 __global__ void __launch_bounds__(256, SC == 4 ? 3 : 2) k_panel_mm_bf16(
     int64_t nrows, int64_t nblk, const int64_t *__restrict__ colptr,
     const int32_t *__restrict__ cols, const __hip_bfloat16 *__restrict__ panel,
@@ -305,10 +356,17 @@ __global__ void __launch_bounds__(256, SC == 4 ? 3 : 2) k_panel_mm_bf16(
     __hip_bfloat16 *__restrict__ xnext, const float *__restrict__ prev,
     double *__restrict__ colmax, const int32_t *__restrict__ items,
     const int32_t *__restrict__ nsplit, const int32_t *__restrict__ slot0,
-    float *__restrict__ ws, int64_t nitems) {
+    float *__restrict__ ws, int64_t nitems, const uint64_t *__restrict__ lmask,
+    const int64_t *__restrict__ sbase) {
   static_assert(SC % 4 == 0, "a stage holds a multiple of 4 chunks");
   constexpr int KG = SC * 4, GPT = KG / 16;  // column groups per stage / per thread
   __shared__ __align__(16) uint2 xs[2][KG * 64];
+  // PACKED: lane masks and packed bases of the (chunk, wave) subtiles in
+  // windows of MW stages, double-buffered: window j holds stages j MW ..
+  // j MW + MW - 1, entry e = (stage % MW) 4 SC + 4 chunk + wave, its mask at
+  // [j % 2][2 e] and its base at [j % 2][2 e + 1] (one value per thread)
+  constexpr int MW = 32 / SC;
+  __shared__ __align__(16) uint64_t wmeta[PACKED ? 2 : 1][PACKED ? 256 : 2];
   const PanelItem itm =
       panel_item(items, nsplit, slot0, colptr, xcd_block(blockIdx.x, nitems));
   const int64_t b = itm.b;
@@ -324,7 +382,8 @@ __global__ void __launch_bounds__(256, SC == 4 ? 3 : 2) k_panel_mm_bf16(
   const int64_t nh = (nrhs + 63) / 64;
   const int64_t row_base = b * kPanelRows;
   const unsigned short *Xs = reinterpret_cast<const unsigned short *>(X);
-  const uint2 *P2 = reinterpret_cast<const uint2 *>(panel);
+  const unsigned long long *PK = reinterpret_cast<const unsigned long long *>(panel);
+  const uint64_t below = (1ull << lane) - 1ull;
   const bool vec_x = (nrhs & 3) == 0;
   // staging role: column group kg = tid / 16 (+16 i), right-hand sides 4 rq .. 4 rq + 3
   const int sg = tid >> 4, rq = tid & 15;
@@ -382,15 +441,33 @@ __global__ void __launch_bounds__(256, SC == 4 ? 3 : 2) k_panel_mm_bf16(
         }
       }
     };
+    // PACKED: this thread's value of window j (entry tid / 2, its mask for
+    // even tid, else its base; the chunk clamped into the item)
+    auto load_window = [&](int64_t j) -> uint64_t {
+      const int e = tid >> 1;
+      const int64_t ch = min((j * MW + e / (4 * SC)) * SC + ((e >> 2) & (SC - 1)), nch - 1);
+      const int64_t sub = ((c0 >> 4) + ch) * 4 + (e & 3);
+      return (tid & 1) ? (uint64_t)sbase[sub] : lmask[sub];
+    };
     auto load_p = [&](int64_t s, uint2 (&pf)[SC]) {
       for (int c = 0; c < SC; ++c) {
         const int64_t ch = s * SC + c;
         const bool ok = ch < nch;
         // B fragment: row 16 w + li, columns 4 lg .. 4 lg + 3 of chunk ch
         // a streaming load: the panels are read once, the X rows many times
-        const unsigned long long raw = __builtin_nontemporal_load(
-            reinterpret_cast<const unsigned long long *>(P2) +
-            ((c0 + (ok ? ch : 0) * 16) * kPanelRows + (w * 16 + li) * 16 + lg * 4) / 4);
+        unsigned long long raw;
+        if constexpr (PACKED) {
+          // the subtile's mask and base (one LDS read); a lane whose chunk
+          // is zero loads a zero, so the MFMAs take the value as it is
+          const ulonglong2 mb = *reinterpret_cast<const ulonglong2 *>(
+              &wmeta[(s / MW) & 1][2 * ((s % MW) * 4 * SC + 4 * c + w)]);
+          const bool has = (mb.x >> lane) & 1ull;
+          raw = __builtin_nontemporal_load(
+              has ? PK + (int64_t)mb.y + __popcll(mb.x & below) : g_zero_lanes);
+        } else {
+          raw = __builtin_nontemporal_load(
+              PK + ((c0 + (ok ? ch : 0) * 16) * kPanelRows + (w * 16 + li) * 16 + lg * 4) / 4);
+        }
         const uint2 v = make_uint2((unsigned)raw, (unsigned)(raw >> 32));
         if constexpr (VEC)
           pf[c] = v;
@@ -414,12 +491,22 @@ __global__ void __launch_bounds__(256, SC == 4 ? 3 : 2) k_panel_mm_bf16(
     };
     v4f acc[4];
     for (int t = 0; t < 4; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
+    // PACKED: window j + 1 is loaded by stage j MW into wv and stored by
+    // stage j MW + 1, two stages before its first reader (MW >= 4)
+    uint64_t wv = 0ull;
     if (nst > 0) {
+      // PACKED: window 0 is loaded with the first ids and X rows, so its
+      // latency overlaps theirs; the panel loads wait for it in LDS
+      if constexpr (PACKED) wv = load_window(0);
       load_ids(0, ci);
       load_x(ci, xa);
       load_ids(1, ci);
       load_x(ci, xb);
       load_ids(2, ci);
+      if constexpr (PACKED) {
+        wmeta[0][tid] = wv;
+        __syncthreads();
+      }
       load_p(0, p0);
       load_p(1, p1);
       store(0, xa);
@@ -433,6 +520,7 @@ __global__ void __launch_bounds__(256, SC == 4 ? 3 : 2) k_panel_mm_bf16(
     auto step = [&](int64_t s, const uint2 (&pc)[SC], uint2 (&pl)[SC], uint2 (&xl)[GPT][4],
                     const uint2 (&xst)[GPT][4]) {
       const int buf = (int)(s & 1);
+      if (PACKED && s % MW == 0) wv = load_window(s / MW + 1);
       // VEC: loads for stages past the end are clamped and harmless, so
       // they go out unconditionally (a join with a different number of
       // pending loads would make the compiler wait conservatively)
@@ -444,11 +532,12 @@ __global__ void __launch_bounds__(256, SC == 4 ? 3 : 2) k_panel_mm_bf16(
       for (int c = 0; c < SC; ++c) {
         // chunks past the block's end hold clamped, unmasked loads: skip them
         if (VEC && s * SC + c >= nch) break;
+        const uint2 pv = pc[c];
         v4s bfrag;
-        bfrag[0] = (short)(pc[c].x & 0xffffu);
-        bfrag[1] = (short)(pc[c].x >> 16);
-        bfrag[2] = (short)(pc[c].y & 0xffffu);
-        bfrag[3] = (short)(pc[c].y >> 16);
+        bfrag[0] = (short)(pv.x & 0xffffu);
+        bfrag[1] = (short)(pv.x >> 16);
+        bfrag[2] = (short)(pv.y & 0xffffu);
+        bfrag[3] = (short)(pv.y >> 16);
         for (int t = 0; t < 4; ++t) {
           // A fragment: X^T[rhs 16 t + li][columns 4 (4c + lg) .. +3]
           const uint2 av = xs[buf][(c * 4 + lg) * 64 + t * 16 + li];
@@ -461,6 +550,7 @@ __global__ void __launch_bounds__(256, SC == 4 ? 3 : 2) k_panel_mm_bf16(
         }
       }
       if (VEC || s + 1 < nst) store(buf ^ 1, xst);
+      if (PACKED && s % MW == 1) wmeta[(s / MW + 1) & 1][tid] = wv;
       __syncthreads();
     };
     // unrolled by 6 = lcm of the panel ring (3) and the X ring (2)
@@ -974,8 +1064,11 @@ extern "C" int flux_panel_mm_bf16(int64_t nrows, const int64_t *colptr, const in
                                   float *out, void *xnext, const float *prev, double *colmax,
                                   const int32_t *items, int64_t n_items, const int32_t *nsplit,
                                   const int32_t *slot0, const int32_t *split_blocks,
-                                  int64_t n_split, float *ws, int stage, void *stream) {
+                                  int64_t n_split, float *ws, int stage,
+                                  const uint64_t *lmask, const int64_t *sbase, void *stream) {
   FLUX_CHECK(nrows >= 0 && nrhs >= 0, "flux_panel_mm_bf16: bad sizes");
+  FLUX_CHECK((lmask == nullptr) == (sbase == nullptr),
+             "flux_panel_mm_bf16: packed panels need both lmask and sbase");
   FLUX_CHECK(stage == 0 || stage == 4 || stage == 8, "flux_panel_mm_bf16: stage must be 0, 4 or 8");
   const int64_t nblk = (nrows + kPanelRows - 1) / kPanelRows;
   if (nblk == 0 || nrhs == 0) return 0;
@@ -991,29 +1084,73 @@ extern "C" int flux_panel_mm_bf16(int64_t nrows, const int64_t *colptr, const in
   if (colmax)
     hipLaunchKernelGGL(k_zero_f64_p, dim3(ceil_div_u(nrhs, 256)), dim3(256), 0, st, colmax,
                        nrhs);
-#define MMB(SCV, VECV)                                                                         \
-  hipLaunchKernelGGL((k_panel_mm_bf16<SCV, VECV>), dim3((unsigned)ngrid), dim3(256), 0, st,      \
+#define MMB(SCV, VECV, PV)                                                                     \
+  hipLaunchKernelGGL((k_panel_mm_bf16<SCV, VECV, PV>), dim3((unsigned)ngrid), dim3(256), 0, st,  \
                      nrows, nblk, colptr, cols, (const __hip_bfloat16 *)panel,                     \
                      (const __hip_bfloat16 *)X, nrhs, base, rho, rho_s, left, out,                 \
-                     (__hip_bfloat16 *)xnext, prev, colmax, items, nsplit, slot0, ws, ngrid)
+                     (__hip_bfloat16 *)xnext, prev, colmax, items, nsplit, slot0, ws, ngrid,      \
+                     lmask, sbase)
+#define MMB_P(SCV, VECV)   \
+  do {                     \
+    if (lmask)             \
+      MMB(SCV, VECV, true); \
+    else                   \
+      MMB(SCV, VECV, false); \
+  } while (0)
   const bool vec = (nrhs & 3) == 0;
   if ((stage ? stage : panel_stage()) == 4) {
     if (vec)
-      MMB(4, true);
+      MMB_P(4, true);
     else
-      MMB(4, false);
+      MMB_P(4, false);
   } else {
     if (vec)
-      MMB(8, true);
+      MMB_P(8, true);
     else
-      MMB(8, false);
+      MMB_P(8, false);
   }
+#undef MMB_P
 #undef MMB
   if (items && n_split > 0)
     hipLaunchKernelGGL(k_panel_reduce_bf16, dim3((unsigned)(n_split * ((nrhs + 63) / 64) * 4)),
                        dim3(256), 0, st,
                        split_blocks, nsplit, slot0, ws, nrows, nrhs, base, rho, rho_s, left, out,
                        (__hip_bfloat16 *)xnext, prev, colmax);
+  FLUX_LAUNCH_CHECK();
+  return 0;
+}
+
+extern "C" int flux_panel_lanemask(int64_t total_chunks, const void *panel, uint64_t *lmask,
+                                   int64_t *sbase, int64_t *total_lanes, void *stream) {
+  FLUX_CHECK(total_chunks >= 0 && total_lanes, "flux_panel_lanemask: bad arguments");
+  FLUX_CHECK(total_chunks == 0 || (panel && lmask && sbase), "flux_panel_lanemask: NULL array");
+  *total_lanes = 0;
+  if (total_chunks == 0) return 0;
+  hipStream_t st = as_stream(stream);
+  const int64_t nsub = total_chunks * 4;
+  int64_t *count = nullptr;
+  FLUX_TRY_HIP(hipMallocAsync((void **)&count, nsub * sizeof(int64_t), st));
+  hipLaunchKernelGGL(k_panel_lanemask, dim3(grid_1d(nsub * 64, 256)), dim3(256), 0, st, nsub,
+                     (const uint64_t *)panel, lmask, count);
+  hipError_t e1 = hipGetLastError();
+  const int r = e1 == hipSuccess ? flux_scan_i64(count, nsub, sbase, total_lanes, st) : -1;
+  hipError_t e2 = hipFreeAsync(count, st);
+  FLUX_TRY_HIP(e1);
+  if (r != 0) return r;
+  FLUX_TRY_HIP(e2);
+  return 0;
+}
+
+extern "C" int flux_panel_pack(int64_t total_chunks, const void *panel, const uint64_t *lmask,
+                               const int64_t *sbase, void *packed, void *stream) {
+  FLUX_CHECK(total_chunks >= 0, "flux_panel_pack: bad sizes");
+  if (total_chunks == 0) return 0;
+  FLUX_CHECK(panel && lmask && sbase && packed, "flux_panel_pack: NULL array");
+  FLUX_CHECK(((uintptr_t)packed & 7) == 0 && ((uintptr_t)panel & 7) == 0,
+             "flux_panel_pack: panels must be 8-byte aligned");
+  const int64_t nsub = total_chunks * 4;
+  hipLaunchKernelGGL(k_panel_pack, dim3(grid_1d(nsub * 64, 256)), dim3(256), 0, as_stream(stream),
+                     nsub, (const uint64_t *)panel, lmask, sbase, (uint64_t *)packed);
   FLUX_LAUNCH_CHECK();
   return 0;
 }

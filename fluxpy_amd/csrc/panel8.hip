@@ -78,14 +78,6 @@ __device__ __forceinline__ v2i lds_read_tr8(const uint8_t *p) {
 }
 constexpr int64_t kMaxBlockCols = 104832;             // int32 bound of the sums S_d
 
-// A value the whole wave holds (read from LDS at a wave-uniform address),
-// moved to scalar registers so that branches on it are scalar.
-__device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
-  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
-  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-  return ((uint64_t)hi << 32) | lo;
-}
-
 // a_0..a_4 of v with |v| < 2^e (see the header).
 __device__ __forceinline__ void slice5(double v, int e, int8_t a[kSlices]) {
   double t = ldexp(v, 6 - e);

@@ -411,7 +411,9 @@ int flux_panel_plan(int64_t nblk, const int64_t *colptr, int64_t cap_quads,
  * workgroup per row block; with a plan, ws holds n_slots * ceil(nrhs / 64)
  * * 4096 floats.  stage: 16-column chunks per pipeline stage (4 or 8; 0 =
  * the library default, 4 unless FLUX_PANEL_STAGE=8), a tuning choice that
- * leaves the result unchanged. */
+ * leaves the result unchanged.  lmask and sbase NULL: `panel` holds the
+ * dense panels; otherwise the packed panels of flux_panel_pack with their
+ * lmask and sbase (the same result bit for bit). */
 int flux_panel_mm_bf16(int64_t nrows, const int64_t *colptr, const int32_t *cols,
                        const void *panel, const void *X, int64_t nrhs,
                        const float *base, const float *rho, float rho_s,
@@ -419,7 +421,25 @@ int flux_panel_mm_bf16(int64_t nrows, const int64_t *colptr, const int32_t *cols
                        double *colmax, const int32_t *items, int64_t n_items,
                        const int32_t *nsplit, const int32_t *slot0,
                        const int32_t *split_blocks, int64_t n_split, float *ws,
-                       int stage, void *stream);
+                       int stage, const uint64_t *lmask, const int64_t *sbase,
+                       void *stream);
+
+/* Lane chunks of the dense BF16 panels (total_chunks = colptr[nblk] / 16
+ * chunks of 64 x 16): subtile s = 4 c + w is rows 16 w .. 16 w + 15 of
+ * chunk c; its lane chunk l is row 16 w + (l & 15), columns 4 (l >> 4) ..
+ * + 3 (8 bytes, one BF16 MFMA operand).  lmask[s] (total_chunks * 4 uint64)
+ * has bit l set when chunk l is not all zero bits; sbase (total_chunks * 4
+ * + 1) is the exclusive prefix sum of the popcounts, and *total_lanes its
+ * total on the host (synchronises). */
+int flux_panel_lanemask(int64_t total_chunks, const void *panel, uint64_t *lmask,
+                        int64_t *sbase, int64_t *total_lanes, void *stream);
+
+/* Packed BF16 panels (total_lanes * 8 bytes, 8-byte aligned): the nonzero
+ * lane chunks of subtile s in lane order, chunk l at 8-byte unit
+ * sbase[s] + popcount(lmask[s] & (2^l - 1)).  The dense panels may be
+ * freed afterwards. */
+int flux_panel_pack(int64_t total_chunks, const void *panel, const uint64_t *lmask,
+                    const int64_t *sbase, void *packed, void *stream);
 
 /* FP32 panels x FP64 X on v_mfma_f64_16x16x4_f64 (values widened):
  *   acc = P X; with E: out = E + s o acc - B and colmax[c] = max_i |out|

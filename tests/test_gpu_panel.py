@@ -91,6 +91,60 @@ def test_panel_products_match_csr(setup, k):
     assert np.allclose(out.cpu().numpy(), refb, rtol=2e-5, atol=2e-6 * abs(refb).max())
 
 
+def test_bf16_packed_panels(setup):
+    """Packed BF16 panels: every nonzero 8-byte lane chunk of the dense
+    panels sits at its packed position and every other chunk is zero; the
+    products read from the packed panels equal the dense panels' bit for
+    bit, with and without a split-block plan, at both stage widths, for
+    right-hand-side counts on the vector and the scalar paths."""
+    from fluxpy_amd._panel import PanelMatrix, PanelPlan
+    torch, sm, csr = setup
+    n = csr.shape[0]
+    P = PanelMatrix(csr, "bf16", keep_dense=True)
+    assert P.packed
+    dense = P.dense.view(torch.int64).cpu().numpy()           # 8-byte units
+    packed = P.panel.view(torch.int64).cpu().numpy()
+    lmask = P.lmask.cpu().numpy().view(np.uint64)
+    sbase = P.sbase.cpu().numpy()
+    nsub = P.total_cols // 4
+    s = np.arange(nsub)[:, None]
+    lane = np.arange(64)[None, :]
+    unit = (s >> 2) * 256 + (s & 3) * 64 + (lane & 15) * 4 + (lane >> 4)
+    assert np.array_equal(np.sort(unit.ravel()), np.arange(dense.size))
+    chunks = dense[unit]                                        # (subtile, lane)
+    bits = ((lmask[:nsub, None] >> lane.astype(np.uint64)) & np.uint64(1)).astype(bool)
+    assert np.array_equal(bits, chunks != 0)
+    below = np.cumsum(bits, axis=1) - bits
+    pos = sbase[:nsub, None] + below
+    assert P.lanes == bits.sum() and np.array_equal(np.sort(pos[bits]), np.arange(P.lanes))
+    assert np.array_equal(packed[pos[bits]], chunks[bits])
+    assert 0 < P.nonzero_subtiles < nsub and P.lanes < dense.size
+    rng = np.random.default_rng(3)
+    tiny = PanelPlan(P.colptr, (n + 63) // 64, cap=2)
+    assert tiny.n_slots > 0
+    for k in (64, 70, 16, 5):
+        Xb = torch.tensor(rng.standard_normal((n, k)), dtype=torch.float32,
+                          device=_dev()).to(torch.bfloat16)
+        base = torch.tensor(rng.uniform(0, 1, (n, k)), dtype=torch.float32, device=_dev())
+        rho = torch.tensor(rng.uniform(0.1, 0.5, n), dtype=torch.float32, device=_dev())
+        for plan in (None, tiny):
+            for stage in (4, 8):
+                P.plan, P.stage = plan, stage
+                res = {}
+                for pk in (True, False):
+                    P.packed = pk
+                    y = torch.empty((n, k), dtype=torch.float32, device=_dev())
+                    P.mm_bf16(Xb, y)
+                    s32 = torch.empty_like(y)
+                    xn = torch.empty_like(Xb)
+                    cm = torch.empty(k, dtype=torch.float64, device=_dev())
+                    P.mm_bf16(Xb, s32, base=base, rho=rho, xnext=xn, prev=base, colmax=cm)
+                    res[pk] = (y, s32, xn, cm)
+                for a, b in zip(res[True], res[False]):
+                    assert torch.equal(a, b), (k, plan is None, stage)
+    P.packed = True
+
+
 @pytest.mark.parametrize("left", [False, True])
 def test_refined_panel_solve(setup, oracle_mod, left):
     from fluxpy_amd._panel import PanelMatrix, PanelRefinedSolver
