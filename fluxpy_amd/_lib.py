@@ -86,6 +86,10 @@ SIGNATURES = {
                                     ctypes.c_float, _c_int, _vp, _vp, _vp, _vp, _vp, _c_i64,
                                     _vp, _vp, _vp, _c_i64, _vp, _c_int, _vp, _vp, _vp]),
     "flux_panel_lanemask": (_c_int, [_c_i64, _vp, _vp, _vp, _vp, _vp]),
+    "flux_timer_create": (_c_int, [_vp]),
+    "flux_timer_record": (_c_int, [_vp, _vp]),
+    "flux_timer_elapsed": (_c_int, [_vp, _vp, _vp]),
+    "flux_timer_destroy": (_c_int, [_vp]),
     "flux_panel_pack": (_c_int, [_c_i64, _vp, _vp, _vp, _vp, _vp]),
     "flux_panel_mm_f64": (_c_int, [_c_i64, _vp, _vp, _vp, _vp, _c_i64, _vp, _vp, _vp, _c_dbl,
                                    _c_int, _vp, _vp, _vp]),

@@ -215,24 +215,49 @@ class PanelMatrix:
         return out
 
 
+class TimingEvent:
+    """A timing event of libfluxhip (flux_timer_*): recording it does not
+    write back and invalidate the caches, so it does not slow the launches
+    around it the way a plain event does.  elapsed_time(end) in ms, like a
+    torch event's."""
+
+    def __init__(self):
+        self._ev = ctypes.c_void_p()
+        _lib.call("flux_timer_create", ctypes.byref(self._ev))
+
+    def record(self):
+        _lib.call("flux_timer_record", self._ev, _lib.stream())
+
+    def elapsed_time(self, end):
+        ms = ctypes.c_float()
+        _lib.call("flux_timer_elapsed", self._ev, end._ev, ctypes.byref(ms))
+        return ms.value
+
+    def __del__(self):
+        ev, self._ev = getattr(self, "_ev", None), None
+        try:
+            if ev and _lib._lib is not None:
+                _lib._lib.flux_timer_destroy(ev)
+        except Exception:  # interpreter shutdown
+            pass
+
+
 class _launch_timer:
-    """Records HIP events on the current stream around a launch when `rec`
-    is a list (appending (kind, start, end)); does nothing for None."""
+    """Records timing events on the current stream around a launch when
+    `rec` is a list (appending (kind, start, end)); does nothing for None."""
 
     def __init__(self, rec, kind):
         self.rec, self.kind = rec, kind
 
     def __enter__(self):
         if self.rec is not None:
-            t = _lib.torch()
-            self.e0 = t.cuda.Event(enable_timing=True)
+            self.e0 = TimingEvent()
             self.e0.record()
         return self
 
     def __exit__(self, *exc):
         if self.rec is not None and exc[0] is None:
-            t = _lib.torch()
-            e1 = t.cuda.Event(enable_timing=True)
+            e1 = TimingEvent()
             e1.record()
             self.rec.append((self.kind, self.e0, e1))
         return False

@@ -690,6 +690,18 @@ int flux_thermal_update(int64_t n, const void *Y, int y_dtype, const double *E,
                         const double *rho, const double *emiss, int clamp, double *Qrefl,
                         double *QIR, double *Q, void *stream);
 
+/* Timing events (no counterpart in the reference: measurement plumbing for
+ * callers that time launches on a stream, such as a benchmark).  They are
+ * created with hipEventDisableSystemFence, so recording one does not write
+ * back and invalidate the caches and does not slow the work after it; an
+ * event gives timestamps only, not visibility of the work before it.
+ *   create:  *ev = a new event;  record: on `stream`;
+ *   elapsed: *ms = time from start to end (waits for end);  destroy. */
+int flux_timer_create(void **ev);
+int flux_timer_record(void *ev, void *stream);
+int flux_timer_elapsed(void *start, void *end, float *ms);
+int flux_timer_destroy(void *ev);
+
 #ifdef __cplusplus
 }
 #endif

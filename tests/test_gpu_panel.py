@@ -200,3 +200,27 @@ def test_refine_prepare_with_column_maxima(n, k, scale):
     assert torch.equal(bxb, axb)
     want = r.abs().amax(dim=0).cpu() if n else torch.zeros(k, dtype=torch.float64)
     assert torch.equal(bcm, want)
+
+
+def test_timing_events(setup):
+    """The launch timer's events (no cache writeback on record) time a BF16
+    product: a positive duration no longer than the host-side wall time of
+    the synchronised launch, and one (kind, start, end) record per launch."""
+    import time
+    from fluxpy_amd._panel import PanelMatrix
+    torch, sm, csr = setup
+    n = csr.shape[0]
+    P = PanelMatrix(csr, "bf16")
+    Xb = torch.ones((n, 64), dtype=torch.bfloat16, device=_dev())
+    out = torch.empty((n, 64), dtype=torch.float32, device=_dev())
+    P.mm_bf16(Xb, out)
+    torch.cuda.synchronize()
+    P.timing = []
+    t0 = time.perf_counter()
+    P.mm_bf16(Xb, out)
+    torch.cuda.synchronize()
+    wall_ms = (time.perf_counter() - t0) * 1e3
+    P.timing, rec = None, P.timing
+    assert len(rec) == 1 and rec[0][0] == "bf16"
+    ms = rec[0][1].elapsed_time(rec[0][2])
+    assert 0.0 < ms <= wall_ms
