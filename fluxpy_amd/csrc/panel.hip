@@ -367,6 +367,9 @@ __global__ void __launch_bounds__(256, SC == 4 ? 3 : 2) k_panel_mm_bf16(
   // [j % 2][2 e] and its base at [j % 2][2 e + 1] (one value per thread)
   constexpr int MW = 32 / SC;
   __shared__ __align__(16) uint64_t wmeta[PACKED ? 2 : 1][PACKED ? 256 : 2];
+  // a window is stored two stages after its load and first read two stages
+  // before it begins (the panel loads run two stages ahead)
+  static_assert(!PACKED || MW >= 4, "mask windows shorter than 4 stages");
   const PanelItem itm =
       panel_item(items, nsplit, slot0, colptr, xcd_block(blockIdx.x, nitems));
   const int64_t b = itm.b;
