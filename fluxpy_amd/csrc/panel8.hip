@@ -238,9 +238,10 @@ __global__ void __launch_bounds__(256) k_panel8_pack(int64_t nsub,
   for (int i = 0; i < kSlices; ++i) d[i * cnt] = p[i * (kSliceTile / 16)];
 }
 
-// A quad of zero slices: the panel loads of a zero lane chunk read it instead
-// (an L2-resident line), so packed panels need not store zero chunks.
-__device__ __align__(256) int8_t g_zero_quad[kQuadBytes];
+// What a lane of the packed product loads for a zero lane chunk (every
+// slice the same 16 zero bytes, an L2-resident line), so packed panels need
+// not store zero chunks and the MFMAs need no select.
+__device__ __align__(256) int8_t g_zero16[16];
 
 __global__ void k_zero_u32(uint32_t *p, int64_t n) {
   const int64_t step = (int64_t)gridDim.x * blockDim.x;
@@ -435,8 +436,8 @@ __global__ void __launch_bounds__(256) k_colmax_blocks(const double *__restrict_
 // which the refinement uses for a residual update whose X is small.  Each
 // thread stages S 16-byte units of X per stage.
 // PACKED: panel8 holds the packed panels of k_panel8_pack (lmask, sbase
-// describe them); a lane whose chunk is zero reads the zero quad, and a wave
-// whose whole subtile is zero skips its MFMAs.  Otherwise panel8 holds the
+// describe them); a lane whose chunk is zero loads zeros (g_zero16), and a
+// wave whose whole subtile is zero skips its MFMAs.  Otherwise panel8 holds the
 // dense quads and every subtile is read.
 template <int S, bool PACKED, int G>
 __global__ void __launch_bounds__(256, 2) k_panel8_mm(
@@ -472,7 +473,7 @@ __global__ void __launch_bounds__(256, 2) k_panel8_mm(
   const int64_t qbase = qptr[b] + itm.q0;
   const v4i *pb = reinterpret_cast<const v4i *>(panel8 + qbase * kQuadBytes +
                                                 (16 * w + li) * kQuadCols + 16 * lg);
-  const v4i *pz = reinterpret_cast<const v4i *>(g_zero_quad + (16 * w + li) * kQuadCols + 16 * lg);
+  const v4i *z16 = reinterpret_cast<const v4i *>(g_zero16);
   const v4i *pk = reinterpret_cast<const v4i *>(panel8);
   const uint64_t below = (1ull << lane) - 1ull;
   const int8_t *xg = x8 + h * (int64_t)L::row;
@@ -511,29 +512,31 @@ __global__ void __launch_bounds__(256, 2) k_panel8_mm(
             &xs[buf][(ch / NT) * L::sslice + c * L::srow + 16 * (ch % NT)]) = xr[m];
       }
     };
-    auto load_p = [&](int64_t s, v4i (&pf)[S]) {
+    // panel fragments of stage s, and whether this wave's subtile holds a
+    // nonzero (nz, wave-uniform)
+    auto load_p = [&](int64_t s, v4i (&pf)[S], bool &nz) {
       const v4i *p;
       int64_t stride;
       if (PACKED) {
-        // a lane whose chunk is zero reads the next nonzero lane's chunk (a
-        // line the wave fetches anyway; the value is zeroed before its
-        // MFMAs); a zero subtile reads the zero quad
+        // a lane whose chunk is zero loads zeros (every slice from the same
+        // 16 zero bytes), so the MFMAs take the values as they are
         const uint64_t m = uniform_u64(lms[s & 3][w]);
         const int64_t bs = (int64_t)uniform_u64((uint64_t)sbs[s & 3][w]);
-        const int cnt = (int)__popcll(m);
-        const int rank = (int)__popcll(m & below);
-        const int off = rank < cnt - 1 ? rank : cnt - 1;
-        p = cnt > 0 ? pk + bs * kSlices + off : pz;
-        stride = cnt > 0 ? (int64_t)cnt : kSliceTile / 16;
+        const bool has = (m >> lane) & 1ull;
+        p = has ? pk + bs * kSlices + __popcll(m & below) : z16;
+        stride = has ? (int64_t)__popcll(m) : 0;
+        nz = m != 0ull;
       } else {
         p = pb + min(s, nq - 1) * (kQuadBytes / 16);
         stride = kSliceTile / 16;
+        nz = true;
       }
       // streaming loads: the panels are read once, the X slices many times
       for (int i = 0; i < S; ++i) pf[i] = __builtin_nontemporal_load(p + i * stride);
     };
     v4i xr[NX];
     v4i p0[S], p1[S];
+    bool z0 = true, z1 = true;
     // prologue: ids(0..2) to LDS (every wave stores the same ids, so no
     // branch sits next to a load), X(0) to LDS, panel(0) in flight
     ids[0][lane] = load_id(0);
@@ -551,17 +554,17 @@ __global__ void __launch_bounds__(256, 2) k_panel8_mm(
       }
     __syncthreads();
     load_x(0, xr);
-    load_p(0, p0);
+    load_p(0, p0, z0);
     store_x(0, xr);
     __syncthreads();
-    auto step = [&](int64_t s, const v4i (&pc)[S], v4i (&pl)[S]) {
+    auto step = [&](int64_t s, const v4i (&pc)[S], v4i (&pl)[S], bool nzc, bool &nzl) {
       const int buf = (int)(s & 1);
       load_x(s + 1, xr);
       const int32_t idn = load_id(s + 3);
       uint64_t mn = 0;
       int64_t bn = 0;
       if (PACKED) load_sub(s + 3, mn, bn);
-      load_p(s + 1, pl);
+      load_p(s + 1, pl, nzl);
       // keep every load of the stage ahead of its MFMAs
       __builtin_amdgcn_sched_barrier(0);
       // A operand of tile t, slice j: columns 16 lg + 8 h + e (h = 0, 1) of
@@ -569,13 +572,7 @@ __global__ void __launch_bounds__(256, 2) k_panel8_mm(
       const uint8_t *xq = xs[buf] + (16 * lg + (li >> 1)) * L::srow + 8 * (li & 1);
       // a zero subtile of this wave contributes nothing: its MFMAs are skipped
       // (a wave-uniform branch with no load inside)
-      const uint64_t msk = PACKED ? uniform_u64(lms[s & 3][w]) : ~0ull;
-      const bool nzs = msk != 0ull;
-      // a zero lane chunk holds another lane's bytes: zero them
-      v4i pz4[S];
-      for (int i = 0; i < S; ++i)
-        pz4[i] = (!PACKED || ((msk >> lane) & 1ull)) ? pc[i] : v4i{0, 0, 0, 0};
-      if (nzs) {
+      if (nzc) {
         // X slice j of all tiles is read one slice ahead of its NT (S - j)
         // MFMAs (independent accumulators), so the transposing LDS reads
         // overlap the matrix work
@@ -593,7 +590,7 @@ __global__ void __launch_bounds__(256, 2) k_panel8_mm(
           for (int i = 0; i + j < S; ++i)
             for (int t = 0; t < NT; ++t)
               acc[i + j][t] =
-                  __builtin_amdgcn_mfma_i32_16x16x64_i8(ab[j & 1][t], pz4[i], acc[i + j][t], 0, 0, 0);
+                  __builtin_amdgcn_mfma_i32_16x16x64_i8(ab[j & 1][t], pc[i], acc[i + j][t], 0, 0, 0);
         }
       }
       store_x(buf ^ 1, xr);
@@ -606,10 +603,10 @@ __global__ void __launch_bounds__(256, 2) k_panel8_mm(
     };
     int64_t s = 0;
     for (; s + 2 <= nq; s += 2) {
-      step(s, p0, p1);
-      step(s + 1, p1, p0);
+      step(s, p0, p1, z0, z1);
+      step(s + 1, p1, p0, z1, z0);
     }
-    if (s < nq) step(s, p0, p1);
+    if (s < nq) step(s, p0, p1, z0, z1);
   }
   // lane l item q of tile t = S^T[rhs 16 t + 4 lg + q][row li]; the S
   // sums combine exactly term by term in FP64
