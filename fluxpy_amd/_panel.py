@@ -65,6 +65,12 @@ class PanelPlan:
 _NO_PLAN = (None, 0, None, None, None, 0, None)
 
 
+def x8_width(k):
+    """Right-hand sides per group of the int8 X slices for k right-hand
+    sides (csrc/panel8.hip: 16 for at most 16, else 64)."""
+    return 16 if k <= 16 else 64
+
+
 class PanelMatrix:
     """Panels of a square CSR in one value precision ('bf16' or 'fp32').
     BF16 panels are kept packed: of every 16-row x 16-column subtile only
@@ -341,7 +347,7 @@ class SlicedPanelMatrix:
         """max_i |s_i X_ic| per column of the X last sliced into `slot` (a
         device view into the workspace)."""
         t = _lib.torch()
-        g = 16 if k <= 16 else 64                 # right-hand sides per X group
+        g = x8_width(k)
         np_ = (k + g - 1) // g * g
         w = self.workspace(k, slot)
         return w[np_ * 4:np_ * 12].view(t.float64)[:k]
